@@ -1,0 +1,125 @@
+/*
+ * C ABI of the MI355X (gfx950) OCT reconstruction library, liboct_hip.so.
+ *
+ * Plain pointers and sizes only.  Every device pointer is a HIP device allocation on the
+ * plan's device; `stream` is a hipStream_t (NULL = default stream).  Calls enqueue work and
+ * return without synchronising the host, like the reference's backend calls between its
+ * synchronize() points (octrecon/pipeline.py:276, 281, 285).  Every function returns 0 on
+ * success or a negative status; oct_last_error() gives the message of the calling thread's
+ * last failure.
+ *
+ * Reference interfaces these replace (atesfet/oct_recon_AF_v1):
+ *   oct_plan_create / oct_plan_destroy
+ *       SpectralProcessor.__init__ uploading sinc_idx/sinc_w/window to the device
+ *       (octrecon/core/spectral.py:91-110; octrecon/core/torch_backend.py:69-85)
+ *   oct_background
+ *       background rows of the pre-FFT stage (octrecon/core/cpu_kernels.py:20-25,
+ *       octrecon/core/gpu_kernels.py:57; frame_mean: octrecon/core/spectral.py:54-66)
+ *   oct_spectral (kind OCT_OUT_MAG)
+ *       SpectralProcessor.magnitude (octrecon/core/spectral.py:136-153), i.e. the fused pre-FFT
+ *       kernel (octrecon/core/gpu_kernels.py:54-68) + cupy.fft.ifft + abs(.[..., :N/2])
+ *   oct_spectral (kind OCT_OUT_COMPLEX)   SpectralProcessor.complex_scan (spectral.py:155-158)
+ *   oct_spectral (kind OCT_OUT_LINEARISED) SpectralProcessor.linearise (spectral.py:118-134)
+ *   oct_stitch
+ *       TileStitcher.contribute + the accumulation into num/den
+ *       (octrecon/core/stitching.py:122-140; octrecon/pipeline.py:282-284)
+ *   oct_finalize
+ *       finalise of a y-tile row + plane/global clim (octrecon/pipeline.py:296-300, 376-388;
+ *       octrecon/io/tiff_writer.py:48-50)
+ *   oct_quantize
+ *       db_to_bits with optional legacy double quantisation (octrecon/io/tiff_writer.py:26-45, 62-65)
+ */
+#ifndef OCT_HIP_H
+#define OCT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OCT_OK = 0, OCT_EINVAL = -1, OCT_EHIP = -2, OCT_EUNSUPPORTED = -3 };
+enum { OCT_RAW_I16 = 0, OCT_RAW_U16 = 1, OCT_RAW_F32 = 2 };
+enum { OCT_APOD_LINES = 0, OCT_APOD_FRAME_MEAN = 1 };
+enum { OCT_OUT_MAG = 0, OCT_OUT_COMPLEX = 1, OCT_OUT_LINEARISED = 2 };
+
+typedef struct oct_plan oct_plan;
+
+/* Upload the spectral operators (host arrays) to device `device`.
+ *   n_lambda     spectral pixels N (2048)
+ *   taps         tap window width T of band_w (12 or 16)
+ *   band_start   (N) int32: output i uses source samples band_start[i] .. band_start[i]+T-1
+ *                (0 <= band_start[i] <= N-T)
+ *   band_w       (N, T) float32 weights (0 outside the sinc band)
+ *   window       (N, 2) float32 complex window Hann/rms * exp(-i beta (k - k0)^2), unscaled */
+int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const int32_t* band_start,
+                    const float* band_w, const float* window);
+int oct_plan_destroy(oct_plan* plan);
+
+/* Device workspace (bytes) for the background rows of `n_files` raw files. */
+size_t oct_spectral_workspace_bytes(const oct_plan* plan, int n_files);
+
+/* Background row per raw file into `bg` (n_files, N) float32:
+ *   apod_mode OCT_APOD_LINES: mean of the first apod_size A-lines of the file;
+ *   OCT_APOD_FRAME_MEAN: mean of all A-lines of each group of apod_group files. */
+int oct_background(const oct_plan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                   int apod_size, int apod_group, float* bg, void* stream);
+
+/* raw (n_files, lines, N) of raw_dtype -> out, with background rows `bg` from oct_background.
+ *   apod_mode OCT_APOD_LINES: the first apod_size A-lines of each file are skipped, the other
+ *             lines - apod_size are scan A-lines; OCT_APOD_FRAME_MEAN: every line is a scan A-line.
+ *   out_kind OCT_OUT_MAG: float32 |ifft(eq * window)| rows [row0, row0+rows) of N/2;
+ *            OCT_OUT_COMPLEX: complex64 (float pairs) of the same rows;
+ *            OCT_OUT_LINEARISED: float32 equispaced interferogram, N samples (row0/rows ignored).
+ *   out_stride: elements (float, or complex for OCT_OUT_COMPLEX) between consecutive A-scans.
+ *   raw must be 16-byte aligned. */
+int oct_spectral(const oct_plan* plan, const void* raw, int raw_dtype, int n_files, int lines,
+                 int apod_mode, int apod_size, int out_kind, int row0, int rows, void* out,
+                 long long out_stride, const float* bg, void* stream);
+
+/* One tile's stitching contribution (see oct_stitch_desc). */
+typedef struct oct_stitch_desc {
+    const float* mag;         /* magnitude rows [mag_row0, mag_row0 + mag_rows) per A-scan */
+    long long mag_stride;     /* floats between A-scans */
+    int mag_row0, mag_rows;
+    int n_x;                  /* A-scans per frame */
+    int n_z;                  /* rows of the tile depth grid (N/2) */
+    const int32_t* frame_idx; /* (B) tile-local y frame of each batch entry */
+    const int32_t* shift;     /* (nY, n_x) optical-path row shift, or NULL */
+    const int32_t* valid_lo;  /* (nY, n_x) first valid row; NULL when shift is NULL */
+    const int32_t* valid_hi;  /* (nY, n_x) last valid row */
+    const float* fz;          /* (n_z) focus weight per tile row */
+    const int32_t* z_rows;    /* (n_out_z, 2) tile rows of the two z interpolation taps */
+    const float* z_w;         /* (n_out_z, 2) */
+    const int32_t* x_cols;    /* (C, 2) tile columns of the two x interpolation taps */
+    const float* x_w;         /* (C, 2) */
+    float* num;               /* element (b, zo, c) at [(b+out_b0)*n_out_z*ldx + zo*ldx + out_c0 + c] */
+    float* den;
+    int B, n_out_z, C;
+    long long ldx;
+    int out_b0, out_c0;
+    int accumulate;           /* 1: +=, 0: = */
+} oct_stitch_desc;
+
+int oct_stitch(const oct_stitch_desc* d, void* stream);
+
+/* num/den (n_planes, plane_elems) -> db (same layout) with den < thr -> NaN, 20 log10(num/den);
+ * with den == NULL, db = 20 log10(num) (a magnitude volume without stitching);
+ * plane_clim (n_planes, 2) and clim (2) = finite min / max (NaN if none). */
+size_t oct_finalize_workspace_bytes(int n_planes, long long plane_elems);
+int oct_finalize(const float* num, const float* den, float* db, int n_planes, long long plane_elems,
+                 float thr, float* plane_clim, float* clim, void* workspace, size_t workspace_bytes,
+                 void* stream);
+
+/* db (n_planes, plane_elems) -> uint16 codes with the device clim (2 floats); legacy_double
+ * first quantises each plane with plane_clim (n_planes, 2) and dequantises it. */
+int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_elems, const float* clim,
+                 const float* plane_clim, int legacy_double, void* stream);
+
+const char* oct_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

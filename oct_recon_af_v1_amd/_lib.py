@@ -1,0 +1,78 @@
+"""ctypes binding of liboct_hip.so (C ABI declared in include/oct_hip.h).
+
+The library is the only compute path: if it is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liboct_hip.so"
+
+OCT_RAW_I16, OCT_RAW_U16, OCT_RAW_F32 = 0, 1, 2
+OCT_APOD_LINES, OCT_APOD_FRAME_MEAN = 0, 1
+OCT_OUT_MAG, OCT_OUT_COMPLEX, OCT_OUT_LINEARISED = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_ll = ctypes.c_longlong
+_sz = ctypes.c_size_t
+_f = ctypes.c_float
+
+
+class StitchDesc(ctypes.Structure):
+    """Mirror of oct_stitch_desc."""
+    _fields_ = [
+        ("mag", _vp), ("mag_stride", _ll), ("mag_row0", _i), ("mag_rows", _i),
+        ("n_x", _i), ("n_z", _i),
+        ("frame_idx", _vp), ("shift", _vp), ("valid_lo", _vp), ("valid_hi", _vp),
+        ("fz", _vp), ("z_rows", _vp), ("z_w", _vp), ("x_cols", _vp), ("x_w", _vp),
+        ("num", _vp), ("den", _vp),
+        ("B", _i), ("n_out_z", _i), ("C", _i), ("ldx", _ll), ("out_b0", _i), ("out_c0", _i),
+        ("accumulate", _i),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/oct_hip.h declares
+SIGNATURES = {
+    "oct_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp]),
+    "oct_plan_destroy": (_i, [_vp]),
+    "oct_spectral_workspace_bytes": (_sz, [_vp, _i]),
+    "oct_background": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "oct_spectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
+    "oct_stitch": (_i, [ctypes.POINTER(StitchDesc), _vp]),
+    "oct_finalize_workspace_bytes": (_sz, [_i, _ll]),
+    "oct_finalize": (_i, [_vp, _vp, _vp, _i, _ll, _f, _vp, _vp, _vp, _sz, _vp]),
+    "oct_quantize": (_i, [_vp, _vp, _i, _ll, _vp, _vp, _i, _vp]),
+    "oct_last_error": (ctypes.c_char_p, []),
+}
+
+_LIB = None
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load liboct_hip.so once; raise if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise HipLibraryError(f"{LIB_PATH} is not built: run `python -m oct_recon_af_v1_amd.build` "
+                                  "(there is no CPU fallback)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().oct_last_error().decode(errors="replace")
+        if rc == -1:
+            raise ValueError(f"{what}: {msg}")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,133 @@
+// Shared device helpers for the OCT reconstruction kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oct {
+
+struct cf {
+    float x, y;
+};
+
+__device__ __forceinline__ cf cadd(cf a, cf b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cf csub(cf a, cf b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cf cmul(cf a, cf b) {
+    return {__builtin_fmaf(a.x, b.x, -a.y * b.y), __builtin_fmaf(a.x, b.y, a.y * b.x)};
+}
+
+// ---------------------------------------------------------------- compile-time trig
+// Taylor series in double on [-pi/4, pi/4] after octant reduction; exact enough that the
+// float twiddle constants are the correctly rounded values of cos/sin(2 pi k / n).
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double ct_sin_small(double x) {
+    double term = x, sum = x;
+    for (int i = 1; i < 16; ++i) {
+        term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ct_cos_small(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i < 16; ++i) {
+        term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+// cos and sin of 2*pi*k/n for integers k, n (n a power of two >= 8)
+constexpr double ct_cos2pi(long k, long n) {
+    k %= n;
+    if (k < 0) k += n;
+    // reduce to octant: angle = 2 pi k / n
+    long o = (8 * k) / n;          // octant 0..7
+    double r = 2.0 * kPi * (double)(8 * k - o * n) / (8.0 * (double)n);   // in [0, pi/4)
+    switch (o) {
+        case 0: return ct_cos_small(r);
+        case 1: return ct_sin_small(kPi / 4 - r);
+        case 2: return -ct_sin_small(r);
+        case 3: return -ct_cos_small(kPi / 4 - r);
+        case 4: return -ct_cos_small(r);
+        case 5: return -ct_sin_small(kPi / 4 - r);
+        case 6: return ct_sin_small(r);
+        default: return ct_cos_small(kPi / 4 - r);
+    }
+}
+constexpr double ct_sin2pi(long k, long n) { return ct_cos2pi(k - n / 4, n); }
+
+// multiply by exp(+2 pi i K / NN) with K, NN compile-time
+template <long K, long NN>
+__device__ __forceinline__ cf twiddle(cf a) {
+    constexpr long k = ((K % NN) + NN) % NN;
+    if constexpr (k == 0) {
+        return a;
+    } else if constexpr (4 * k == NN) {
+        return {-a.y, a.x};
+    } else if constexpr (2 * k == NN) {
+        return {-a.x, -a.y};
+    } else if constexpr (4 * k == 3 * NN) {
+        return {a.y, -a.x};
+    } else if constexpr (8 * k == NN || 8 * k == 3 * NN || 8 * k == 5 * NN || 8 * k == 7 * NN) {
+        // odd multiples of pi/4: c = +-sqrt(1/2), s = +-c -> one add and one multiply each
+        constexpr float c = (float)ct_cos2pi(k, NN);
+        constexpr float sc = (ct_sin2pi(k, NN) > 0) == (ct_cos2pi(k, NN) > 0) ? 1.0f : -1.0f;
+        return {c * (a.x - sc * a.y), c * (sc * a.x + a.y)};
+    } else {
+        constexpr float c = (float)ct_cos2pi(k, NN);
+        constexpr float s = (float)ct_sin2pi(k, NN);
+        return {__builtin_fmaf(a.x, c, -a.y * s), __builtin_fmaf(a.x, s, a.y * c)};
+    }
+}
+
+// ---------------------------------------------------------------- compile-time loops
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+constexpr int ct_log2(int n) { return n <= 1 ? 0 : 1 + ct_log2(n / 2); }
+constexpr int bitrev(int v, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; ++i) r |= ((v >> i) & 1) << (bits - 1 - i);
+    return r;
+}
+
+// In-register radix-2 decimation-in-frequency DFT of size NP with kernel exp(+2 pi i/NP)
+// (unnormalised inverse DFT).  Input natural order; output Y[k] in v[bitrev(k)].
+template <int NP>
+__device__ __forceinline__ void dft_inv_dif(cf (&v)[NP]) {
+    constexpr int L = ct_log2(NP);
+    static_for<0, L>([&](auto s) {
+        constexpr int len = NP >> decltype(s)::value;
+        constexpr int half = len / 2;
+        static_for<0, NP / len>([&](auto blk) {
+            static_for<0, half>([&](auto j) {
+                constexpr int a = decltype(blk)::value * len + decltype(j)::value;
+                constexpr int b = a + half;
+                cf x = v[a], y = v[b];
+                v[a] = cadd(x, y);
+                v[b] = twiddle<decltype(j)::value, len>(csub(x, y));
+            });
+        });
+    });
+}
+
+// Wave-local LDS ordering point: orders this wave's LDS accesses before/after, without a
+// workgroup barrier (all lanes of a wave execute in lockstep).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// value of the neighbouring lane (lane ^ 1) through a DPP quad permutation [1,0,3,2]
+__device__ __forceinline__ float swap_pair(float v) {
+    int i = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(float, i);
+}
+
+}  // namespace oct

@@ -1,0 +1,306 @@
+// liboct_hip: C ABI over the gfx950 OCT reconstruction kernels (declarations: include/oct_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/oct_hip.h"
+#include "spectral.hpp"
+#include "stitch.hpp"
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) return fail(OCT_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct oct_plan {
+    int device;
+    int n;
+    int taps;
+    int cus;
+    int* band_start;
+    float* band_w;
+    float* win;   // window / N
+};
+
+namespace {
+
+template <typename RawT, int TP>
+int launch_spectral_t(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
+    const long long ngroups = (a.M + oct::kWaves - 1) / oct::kWaves;
+    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus);
+    if (grid <= 0) return OCT_OK;
+    switch (kind) {
+        case OCT_OUT_MAG:
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_MAG>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            break;
+        case OCT_OUT_COMPLEX:
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_CPLX>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_EQ>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            break;
+    }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+template <typename RawT>
+int launch_spectral(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
+    if (p->taps == 12) return launch_spectral_t<RawT, 12>(p, a, kind, s);
+    return launch_spectral_t<RawT, 16>(p, a, kind, s);
+}
+
+template <typename RawT>
+int launch_background(const void* raw, float* bg, int B, int lines, int N, int apod, int mode, int group,
+                      hipStream_t s) {
+    const long long n = (long long)B * N;
+    const int threads = 256;
+    const long long blocks = (n + threads - 1) / threads;
+    hipLaunchKernelGGL(oct::background_kernel<RawT>, dim3((unsigned)blocks), dim3(threads), 0, s,
+                       reinterpret_cast<const RawT*>(raw), bg, B, lines, N, apod, mode, group);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oct_last_error(void) { return g_err; }
+
+int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const int32_t* band_start,
+                    const float* band_w, const float* window) {
+    if (!plan || !band_start || !band_w || !window) return fail(OCT_EINVAL, "null argument");
+    *plan = nullptr;
+    if (n_lambda != oct::kN) return fail(OCT_EUNSUPPORTED, "n_lambda %d unsupported (2048 only)", n_lambda);
+    if (taps != 12 && taps != 16) return fail(OCT_EUNSUPPORTED, "tap window %d unsupported (12 or 16)", taps);
+    for (int i = 0; i < n_lambda; ++i)
+        if (band_start[i] < 0 || band_start[i] > n_lambda - taps)
+            return fail(OCT_EINVAL, "band_start[%d]=%d outside [0, %d]", i, band_start[i], n_lambda - taps);
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(device));
+    oct_plan* p = new oct_plan();
+    p->device = device;
+    p->n = n_lambda;
+    p->taps = taps;
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    p->cus = (e == hipSuccess && cus > 0) ? cus : 256;
+    float* win = new float[2 * n_lambda];
+    for (int i = 0; i < 2 * n_lambda; ++i) win[i] = window[i] * (1.0f / (float)n_lambda);   // exact: 2^-11
+    int rc = OCT_OK;
+    if (hipMalloc(&p->band_start, sizeof(int) * n_lambda) != hipSuccess ||
+        hipMalloc(&p->band_w, sizeof(float) * n_lambda * taps) != hipSuccess ||
+        hipMalloc(&p->win, sizeof(float) * 2 * n_lambda) != hipSuccess ||
+        hipMemcpy(p->band_start, band_start, sizeof(int) * n_lambda, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->band_w, band_w, sizeof(float) * n_lambda * taps, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->win, win, sizeof(float) * 2 * n_lambda, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(OCT_EHIP, "plan upload failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+    delete[] win;
+    hipSetDevice(prev);
+    if (rc != OCT_OK) {
+        oct_plan_destroy(p);
+        return rc;
+    }
+    *plan = p;
+    return OCT_OK;
+}
+
+int oct_plan_destroy(oct_plan* p) {
+    if (!p) return OCT_OK;
+    int prev = 0;
+    hipGetDevice(&prev);
+    hipSetDevice(p->device);
+    if (p->band_start) hipFree(p->band_start);
+    if (p->band_w) hipFree(p->band_w);
+    if (p->win) hipFree(p->win);
+    hipSetDevice(prev);
+    delete p;
+    return OCT_OK;
+}
+
+size_t oct_spectral_workspace_bytes(const oct_plan* p, int n_files) {
+    if (!p || n_files <= 0) return 0;
+    return sizeof(float) * (size_t)n_files * (size_t)p->n;
+}
+
+static int check_raw(const void* raw, int raw_dtype, int n_files, int lines, int apod_mode, int apod_size,
+                     int apod_group, int* line0, int* nX) {
+    if (!raw) return fail(OCT_EINVAL, "null raw pointer");
+    if (n_files < 0 || lines <= 0) return fail(OCT_EINVAL, "bad raw shape (%d, %d)", n_files, lines);
+    if (reinterpret_cast<uintptr_t>(raw) % 16) return fail(OCT_EINVAL, "raw pointer must be 16-byte aligned");
+    if (raw_dtype < OCT_RAW_I16 || raw_dtype > OCT_RAW_F32) return fail(OCT_EINVAL, "raw dtype %d", raw_dtype);
+    if (apod_mode == OCT_APOD_LINES) {
+        if (apod_size <= 0 || apod_size >= lines)
+            return fail(OCT_EINVAL, "apod_size %d must be in [1, %d)", apod_size, lines);
+        *line0 = apod_size;
+        *nX = lines - apod_size;
+    } else if (apod_mode == OCT_APOD_FRAME_MEAN) {
+        if (apod_group <= 0 || (n_files % apod_group))
+            return fail(OCT_EINVAL, "batch of %d files is not a whole number of %d-file frames", n_files, apod_group);
+        *line0 = 0;
+        *nX = lines;
+    } else {
+        return fail(OCT_EINVAL, "apod mode %d", apod_mode);
+    }
+    return OCT_OK;
+}
+
+int oct_background(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                   int apod_size, int apod_group, float* bg, void* stream) {
+    if (!p || !bg) return fail(OCT_EINVAL, "null argument");
+    int line0, nX;
+    int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, apod_group, &line0, &nX);
+    if (rc) return rc;
+    if (n_files == 0) return OCT_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int mode = apod_mode == OCT_APOD_LINES ? 0 : 1;
+    switch (raw_dtype) {
+        case OCT_RAW_I16: return launch_background<int16_t>(raw, bg, n_files, lines, p->n, apod_size, mode, apod_group, s);
+        case OCT_RAW_U16: return launch_background<uint16_t>(raw, bg, n_files, lines, p->n, apod_size, mode, apod_group, s);
+        default: return launch_background<float>(raw, bg, n_files, lines, p->n, apod_size, mode, apod_group, s);
+    }
+}
+
+int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                 int apod_size, int out_kind, int row0, int rows, void* out, long long out_stride,
+                 const float* bg, void* stream) {
+    if (!p || !out || !bg) return fail(OCT_EINVAL, "null argument");
+    int line0, nX;
+    int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, &line0, &nX);
+    if (rc) return rc;
+    if (n_files == 0) return OCT_OK;
+    if (out_kind < OCT_OUT_MAG || out_kind > OCT_OUT_LINEARISED) return fail(OCT_EINVAL, "out kind %d", out_kind);
+    const int nz = p->n / 2;
+    if (out_kind != OCT_OUT_LINEARISED) {
+        if (row0 < 0 || rows <= 0 || row0 + rows > nz)
+            return fail(OCT_EINVAL, "rows [%d, %d) outside [0, %d)", row0, row0 + rows, nz);
+        if (out_stride < rows) return fail(OCT_EINVAL, "out_stride %lld < rows %d", out_stride, rows);
+    } else {
+        row0 = 0;
+        rows = p->n;
+        if (out_stride < p->n) return fail(OCT_EINVAL, "out_stride %lld < N", out_stride);
+    }
+    oct::SpectralArgs a;
+    a.raw = raw;
+    a.bg = bg;
+    a.band_start = p->band_start;
+    a.band_w = p->band_w;
+    a.win = p->win;
+    a.out = out;
+    a.M = (long long)n_files * nX;
+    a.lines = lines;
+    a.line0 = line0;
+    a.nX = nX;
+    a.row0 = row0;
+    a.rows = rows;
+    a.out_stride = out_stride;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (raw_dtype) {
+        case OCT_RAW_I16: return launch_spectral<int16_t>(p, a, out_kind, s);
+        case OCT_RAW_U16: return launch_spectral<uint16_t>(p, a, out_kind, s);
+        default: return launch_spectral<float>(p, a, out_kind, s);
+    }
+}
+
+int oct_stitch(const oct_stitch_desc* d, void* stream) {
+    if (!d || !d->mag || !d->frame_idx || !d->fz || !d->z_rows || !d->z_w || !d->x_cols || !d->x_w || !d->num ||
+        !d->den)
+        return fail(OCT_EINVAL, "null argument");
+    if (d->shift && (!d->valid_lo || !d->valid_hi)) return fail(OCT_EINVAL, "shift without valid bounds");
+    if (d->B <= 0 || d->n_out_z <= 0 || d->C <= 0) return OCT_OK;
+    if (d->n_x <= 0 || d->n_z <= 0 || d->ldx < d->out_c0 + d->C) return fail(OCT_EINVAL, "bad stitch shape");
+    if (d->mag_row0 != 0 || d->mag_rows != d->n_z)
+        if (d->mag_row0 < 0 || d->mag_rows <= 0) return fail(OCT_EINVAL, "bad magnitude row window");
+    oct::StitchArgs a;
+    a.mag = d->mag;
+    a.mstride = d->mag_stride;
+    a.mrow0 = d->mag_row0;
+    a.mrows = d->mag_rows;
+    a.nX = d->n_x;
+    a.nZ = d->n_z;
+    a.frame_idx = d->frame_idx;
+    a.shift = d->shift;
+    a.vlo = d->valid_lo;
+    a.vhi = d->valid_hi;
+    a.fz = d->fz;
+    a.zr = d->z_rows;
+    a.zw = d->z_w;
+    a.xr = d->x_cols;
+    a.xw = d->x_w;
+    a.num = d->num;
+    a.den = d->den;
+    a.B = d->B;
+    a.nOutZ = d->n_out_z;
+    a.C = d->C;
+    a.ldx = d->ldx;
+    a.out_b0 = d->out_b0;
+    a.out_c0 = d->out_c0;
+    a.accumulate = d->accumulate;
+    dim3 grid((d->C + 255) / 256, d->n_out_z, d->B);
+    hipLaunchKernelGGL(oct::stitch_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), a);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+static int finalize_blocks(long long plane_elems) {
+    long long b = (plane_elems + oct::kFinThreads * 8 - 1) / (oct::kFinThreads * 8);
+    return (int)std::max<long long>(1, std::min<long long>(b, 256));
+}
+
+size_t oct_finalize_workspace_bytes(int n_planes, long long plane_elems) {
+    if (n_planes <= 0 || plane_elems <= 0) return 0;
+    return sizeof(int) * 2 * (size_t)n_planes * (size_t)finalize_blocks(plane_elems);
+}
+
+int oct_finalize(const float* num, const float* den, float* db, int n_planes, long long plane_elems, float thr,
+                 float* plane_clim, float* clim, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!num || !db || !plane_clim || !clim) return fail(OCT_EINVAL, "null argument");
+    if (n_planes <= 0 || plane_elems <= 0) return fail(OCT_EINVAL, "empty volume");
+    const size_t need = oct_finalize_workspace_bytes(n_planes, plane_elems);
+    if (!workspace || workspace_bytes < need)
+        return fail(OCT_EINVAL, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int nb = finalize_blocks(plane_elems);
+    int* partial = reinterpret_cast<int*>(workspace);
+    hipLaunchKernelGGL(oct::finalize_kernel, dim3(nb, n_planes), dim3(oct::kFinThreads), 0, s, num, den, db,
+                       plane_elems, thr, partial);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(oct::clim_kernel, dim3(1), dim3(oct::kFinThreads), 0, s, partial, nb, n_planes, plane_clim,
+                       clim);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_elems, const float* clim,
+                 const float* plane_clim, int legacy_double, void* stream) {
+    if (!db || !bits || !clim || (legacy_double && !plane_clim)) return fail(OCT_EINVAL, "null argument");
+    if (n_planes <= 0 || plane_elems <= 0) return OCT_OK;
+    const long long n = (long long)n_planes * plane_elems;
+    const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 16);
+    hipLaunchKernelGGL(oct::quantize_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), db,
+                       bits, plane_elems, n_planes, clim, plane_clim, legacy_double);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+}  // extern "C"

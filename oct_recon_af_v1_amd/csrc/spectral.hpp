@@ -1,0 +1,283 @@
+// Fused spectral kernel: raw spectra -> background subtraction -> mean removal -> banded sinc
+// k-linearisation -> complex window (Hann/rms x dispersion, 1/N folded in) -> 2048-point
+// inverse FFT -> depth crop -> |.| (or complex, or the linearised interferogram).
+//
+// Reference: octrecon/core/spectral.py:136-153 (SpectralProcessor.magnitude),
+// octrecon/core/gpu_kernels.py:10-38 (per-sample fused pre-FFT), octrecon/core/cpu_kernels.py:12-41.
+//
+// Work decomposition (one workgroup = 8 wave64 = 8 A-scans per group, persistent over groups;
+// each A-scan owns one 16.5 KB LDS region):
+//   A  each wave loads one A-scan (16-byte loads), subtracts the file's background row,
+//      reduces the line mean and writes d - m to the first half of its region;
+//   B  sinc: wave w computes output samples i = lane + 64 q, q in [4w, 4w+4), of ALL eight
+//      A-scans with its taps and band starts held in registers (each operator entry is read
+//      once per group of eight A-scans), and writes the equispaced interferogram to the
+//      second half of each region;
+//   D  each wave transforms its own A-scan: window multiply on load, N = 64 n1 + n2
+//      (lane = n2) -> 32-point DFT over n1 in registers, twiddle W_N^(n2 k1) from an LDS
+//      table, one padded LDS transpose, 32-point DFT over the even/odd halves of n2 in lane
+//      pairs, and a DPP pair exchange that forms only the kept bins k = k1 + 32 k2, k2 < 32
+//      (the first N/2 outputs).
+// Two workgroup barriers per group (after A and after B).
+#pragma once
+#include "oct_common.hpp"
+
+namespace oct {
+
+constexpr int kN = 2048;
+constexpr int kNZ = kN / 2;
+constexpr int kWaves = 8;
+constexpr int kThreads = 64 * kWaves;
+constexpr int kQ = kN / 64;            // 32 values per lane in the transform layout
+constexpr int kQW = kQ / kWaves;       // 4 sinc outputs per lane per A-scan for one wave
+constexpr int kStride = 66;            // padded transpose row (complex elements)
+constexpr int kBufCF = 32 * kStride;   // complex elements per A-scan buffer (>= kN)
+
+enum OutKind { KIND_MAG = 0, KIND_CPLX = 1, KIND_EQ = 2 };
+
+struct SpectralArgs {
+    const void* raw;         // (B, lines, N) int16 / uint16 / float32
+    const float* bg;         // (B, N) background row per raw file
+    const int* band_start;   // (N) first source sample of each output's tap window
+    const float* band_w;     // (N, TP) tap weights
+    const float* win;        // (N, 2) complex window scaled by 1/N
+    void* out;
+    long long M;             // A-scans = B * nX
+    int lines;               // A-lines per raw file
+    int line0;               // first scan A-line of a file
+    int nX;                  // scan A-lines per file
+    int row0, rows;          // kept depth rows [row0, row0 + rows)
+    long long out_stride;    // elements between consecutive A-scans in out
+};
+
+template <typename RawT>
+__device__ __forceinline__ void unpack16(const uint4& u, float (&f)[16 / sizeof(RawT)]);
+
+template <>
+__device__ __forceinline__ void unpack16<int16_t>(const uint4& u, float (&f)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = (float)(int16_t)(w[i] & 0xFFFFu);
+        f[2 * i + 1] = (float)(int16_t)(w[i] >> 16);
+    }
+}
+template <>
+__device__ __forceinline__ void unpack16<uint16_t>(const uint4& u, float (&f)[8]) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f[2 * i] = (float)(w[i] & 0xFFFFu);
+        f[2 * i + 1] = (float)(w[i] >> 16);
+    }
+}
+template <>
+__device__ __forceinline__ void unpack16<float>(const uint4& u, float (&f)[4]) {
+    f[0] = __builtin_bit_cast(float, u.x);
+    f[1] = __builtin_bit_cast(float, u.y);
+    f[2] = __builtin_bit_cast(float, u.z);
+    f[3] = __builtin_bit_cast(float, u.w);
+}
+
+template <typename RawT, int TP, int KIND>
+__global__ __launch_bounds__(kThreads, 1) void spectral_kernel(SpectralArgs a) {
+    __shared__ __align__(16) cf buf[kWaves][kBufCF];
+    __shared__ __align__(16) cf tws[kQ][64];     // tws[k1][n2] = exp(+2 pi i n2 k1 / N)
+    __shared__ float lmean[kWaves];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+
+    if constexpr (KIND != KIND_EQ) {
+        for (int e = threadIdx.x; e < kQ * 64; e += kThreads) {
+            const int k1 = e >> 6, n2 = e & 63;
+            double s, c;
+            sincospi((double)(n2 * k1) / (kN / 2), &s, &c);
+            tws[k1][n2] = cf{(float)c, (float)s};
+        }
+        // made visible to all waves by the first barrier of the group loop
+    }
+
+    constexpr int VEC = 16 / sizeof(RawT);
+    constexpr int NLD = kN / (64 * VEC);
+    const long long ngroups = (a.M + kWaves - 1) / kWaves;
+
+    for (long long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        const long long p0 = g * kWaves;
+        const int nvalid = (int)min((long long)kWaves, a.M - p0);
+        const long long p = p0 + wave;
+        float* D = reinterpret_cast<float*>(&buf[wave][0]);
+
+        // this wave's sinc taps and band starts (L2-resident tables, reloaded per group so
+        // that they do not occupy registers during the transform)
+        int bs[kQW];
+        float wt[kQW][TP];
+#pragma unroll
+        for (int qq = 0; qq < kQW; ++qq) {
+            const int i = lane + 64 * (wave * kQW + qq);
+            bs[qq] = a.band_start[i];
+            const float4* wp = reinterpret_cast<const float4*>(a.band_w + (long long)i * TP);
+#pragma unroll
+            for (int t4 = 0; t4 < TP / 4; ++t4) {
+                const float4 v = wp[t4];
+                wt[qq][4 * t4] = v.x;
+                wt[qq][4 * t4 + 1] = v.y;
+                wt[qq][4 * t4 + 2] = v.z;
+                wt[qq][4 * t4 + 3] = v.w;
+            }
+        }
+
+        // ---- A: load, background, mean, d - m -> LDS
+        if (wave < nvalid) {
+            const long long b = p / a.nX;
+            const long long x = p - b * a.nX;
+            const RawT* r = reinterpret_cast<const RawT*>(a.raw) + ((b * a.lines + a.line0 + x) * (long long)kN);
+            const float* bgr = a.bg + b * kN;
+            float d[NLD][VEC];
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int j0 = VEC * lane + 64 * VEC * k;
+                uint4 u = *reinterpret_cast<const uint4*>(r + j0);
+                float f[VEC];
+                unpack16<RawT>(u, f);
+#pragma unroll
+                for (int e4 = 0; e4 < VEC / 4; ++e4) {
+                    float4 bv = *reinterpret_cast<const float4*>(bgr + j0 + 4 * e4);
+                    d[k][4 * e4] = f[4 * e4] - bv.x;
+                    d[k][4 * e4 + 1] = f[4 * e4 + 1] - bv.y;
+                    d[k][4 * e4 + 2] = f[4 * e4 + 2] - bv.z;
+                    d[k][4 * e4 + 3] = f[4 * e4 + 3] - bv.w;
+                }
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) s += d[k][e];
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            const float m = s * (1.0f / kN);
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int j0 = VEC * lane + 64 * VEC * k;
+#pragma unroll
+                for (int e4 = 0; e4 < VEC / 4; ++e4) {
+                    float4 v{d[k][4 * e4] - m, d[k][4 * e4 + 1] - m, d[k][4 * e4 + 2] - m, d[k][4 * e4 + 3] - m};
+                    *reinterpret_cast<float4*>(D + j0 + 4 * e4) = v;
+                }
+            }
+            if (lane == 0) lmean[wave] = m;
+        }
+        __syncthreads();
+
+        // ---- B: banded sinc for all A-scans of the group, this wave's output samples;
+        //      the interferogram goes to the second half of each A-scan's LDS region
+#pragma unroll 1
+        for (int aa = 0; aa < nvalid; ++aa) {
+            const float* Da = reinterpret_cast<const float*>(&buf[aa][0]);
+            float* Ea = reinterpret_cast<float*>(&buf[aa][0]) + kN;
+            const float ma = lmean[aa];
+#pragma unroll
+            for (int qq = 0; qq < kQW; ++qq) {
+                const int i = lane + 64 * (wave * kQW + qq);
+                const float* src = Da + bs[qq];
+                float acc = 0.f;
+#pragma unroll
+                for (int t = 0; t < TP; ++t) acc = __builtin_fmaf(wt[qq][t], src[t], acc);
+                const float v = acc + ma;
+                if constexpr (KIND == KIND_EQ) {
+                    reinterpret_cast<float*>(a.out)[(p0 + aa) * a.out_stride + i] = v;
+                } else {
+                    Ea[i] = v;
+                }
+            }
+        }
+        __syncthreads();
+        if constexpr (KIND == KIND_EQ) continue;
+
+        // ---- D: 2048-point inverse transform of this wave's A-scan
+        if (wave < nvalid) {
+            cf* B = &buf[wave][0];
+            const float* E = reinterpret_cast<const float*>(B) + kN;
+            const float2* wg = reinterpret_cast<const float2*>(a.win);
+            cf v[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const float e = E[lane + 64 * q];
+                const float2 w = wg[lane + 64 * q];
+                v[q] = cf{e * w.x, e * w.y};
+            }
+            wave_lds_sync();
+            dft_inv_dif<kQ>(v);                          // Y[k1] in v[bitrev(k1)]
+            static_for<1, kQ>([&](auto k1) {
+                constexpr int r = bitrev(decltype(k1)::value, 5);
+                v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
+            });
+            wave_lds_sync();
+            static_for<0, kQ>([&](auto k1) {
+                B[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)];
+            });
+            wave_lds_sync();
+            const int kk1 = lane >> 1;
+            const int h = lane & 1;
+#pragma unroll
+            for (int m = 0; m < kQ; ++m) v[m] = B[kk1 * kStride + 2 * m + h];
+            dft_inv_dif<kQ>(v);                          // E_h[k2] in v[bitrev(k2)]
+            if (h) {
+                static_for<1, kQ>([&](auto k2) {
+                    constexpr int r = bitrev(decltype(k2)::value, 5);
+                    v[r] = twiddle<decltype(k2)::value, 64>(v[r]);
+                });
+            }
+            // X[k1 + 32 k2] = F0[k2] + F1[k2]; lane h keeps k2 = 16 h + j, j < 16
+            static_for<0, 16>([&](auto jj) {
+                constexpr int j = decltype(jj)::value;
+                constexpr int rlo = bitrev(j, 5);
+                constexpr int rhi = bitrev(16 + j, 5);
+                const cf send = h ? v[rlo] : v[rhi];
+                const cf keep = h ? v[rhi] : v[rlo];
+                const cf o{keep.x + swap_pair(send.x), keep.y + swap_pair(send.y)};
+                const int k = kk1 + 32 * (16 * h + j);
+                if (k >= a.row0 && k < a.row0 + a.rows) {
+                    if constexpr (KIND == KIND_MAG) {
+                        reinterpret_cast<float*>(a.out)[p * a.out_stride + (k - a.row0)] =
+                            __builtin_sqrtf(o.x * o.x + o.y * o.y);
+                    } else {
+                        reinterpret_cast<cf*>(a.out)[p * a.out_stride + (k - a.row0)] = o;
+                    }
+                }
+            });
+        }
+    }
+}
+
+// Background row per raw file.
+//   mode 0 ("lines"): mean of the first `apod` A-lines of the file (float64 sum, stored float32)
+//   mode 1 ("frame_mean"): mean over all A-lines of each group of `group` files, float32
+//   accumulation in line order, then the mean over the group's files.
+// Reference: octrecon/core/cpu_kernels.py:20-25; octrecon/core/spectral.py:54-66.
+template <typename RawT>
+__global__ void background_kernel(const RawT* __restrict__ raw, float* __restrict__ bg, int B, int lines,
+                                  int N, int apod, int mode, int group) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long long)B * N) return;
+    const int b = (int)(tid / N);
+    const int j = (int)(tid - (long long)b * N);
+    if (mode == 0) {
+        const RawT* r = raw + (long long)b * lines * N + j;
+        double s = 0.0;
+        for (int l = 0; l < apod; ++l) s += (double)r[(long long)l * N];
+        bg[tid] = (float)(s / apod);
+    } else {
+        const int g0 = (b / group) * group;
+        float acc = 0.f;
+        for (int f = 0; f < group; ++f) {
+            const RawT* r = raw + (long long)(g0 + f) * lines * N + j;
+            float s = 0.f;
+            for (int l = 0; l < lines; ++l) s += (float)r[(long long)l * N];
+            const float fm = s / (float)lines;
+            acc = (f == 0) ? fm : acc + fm;
+        }
+        bg[tid] = acc / (float)group;
+    }
+}
+
+}  // namespace oct

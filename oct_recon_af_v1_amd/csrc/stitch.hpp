@@ -1,0 +1,224 @@
+// Tile stitching, finalisation, volume clim and legacy quantisation kernels.
+//
+// Reference: octrecon/core/stitching.py:122-140 (TileStitcher.contribute), octrecon/pipeline.py
+// :282-284 (accumulation), :296-300 (finalise), :376-388 (clim), octrecon/io/tiff_writer.py
+// :26-45, 62-65 (quantisation).
+#pragma once
+#include "oct_common.hpp"
+
+namespace oct {
+
+// One tile's contribution, output-owned: thread (b, zo, c) forms
+//   num = sum_x Wx[c,x] sum_r Wz[zo,r] fz[r] valid(b,x,r) mag[b, x, clip(r + shift)]
+//   den = sum_x Wx[c,x] sum_r Wz[zo,r] fz[r] valid(b,x,r)
+// from the two non-zero taps of each interpolation row, and adds (or stores) them into the
+// accumulators; consecutive threads own consecutive output columns.
+struct StitchArgs {
+    const float* mag;        // magnitude rows [mrow0, mrow0 + mrows) of each A-scan
+    long long mstride;       // floats between A-scans in mag
+    int mrow0, mrows;
+    int nX;                  // A-scans per frame in the tile
+    int nZ;                  // rows of the tile depth grid
+    const int* frame_idx;    // (B) tile-local y frame of each batch entry
+    const int* shift;        // (nY, nX) optical-path row shift, or null
+    const int* vlo;          // (nY, nX) first valid row (row + P/dz >= 0)
+    const int* vhi;          // (nY, nX) last valid row  (row + P/dz <= nZ - 1)
+    const float* fz;         // (nZ) focus weight per tile row
+    const int* zr;           // (nOutZ, 2) tile rows of the two z taps
+    const float* zw;         // (nOutZ, 2) their weights
+    const int* xr;           // (C, 2) tile columns of the two x taps
+    const float* xw;         // (C, 2) their weights
+    float* num;              // accumulators: element (b, zo, c) at
+    float* den;              //   [(b + out_b0) * nOutZ * ldx + zo * ldx + out_c0 + c]
+    int B, nOutZ, C;
+    long long ldx;
+    int out_b0, out_c0;
+    int accumulate;
+};
+
+__global__ __launch_bounds__(256) void stitch_kernel(StitchArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int zo = blockIdx.y;
+    const int b = blockIdx.z;
+    if (c >= a.C) return;
+    const int fb = a.frame_idx[b];
+    float num = 0.f, den = 0.f;
+#pragma unroll
+    for (int tx = 0; tx < 2; ++tx) {
+        const float wx = a.xw[2 * c + tx];
+        if (wx == 0.f) continue;
+        const int x = a.xr[2 * c + tx];
+        int sh = 0, lo = 0, hi = a.nZ - 1;
+        if (a.shift) {
+            const long long o = (long long)fb * a.nX + x;
+            sh = a.shift[o];
+            lo = a.vlo[o];
+            hi = a.vhi[o];
+        }
+        const float* mrow = a.mag + ((long long)b * a.nX + x) * a.mstride - a.mrow0;
+        float sa = 0.f, sf = 0.f;
+#pragma unroll
+        for (int tz = 0; tz < 2; ++tz) {
+            const float wz = a.zw[2 * zo + tz];
+            if (wz == 0.f) continue;
+            const int r = a.zr[2 * zo + tz];
+            const float f = (r >= lo && r <= hi) ? a.fz[r] : 0.f;
+            const int src = min(max(r + sh, 0), a.nZ - 1);
+            const float av = mrow[src] * f;
+            sa = __builtin_fmaf(av, wz, sa);
+            sf = __builtin_fmaf(f, wz, sf);
+        }
+        num = __builtin_fmaf(sa, wx, num);
+        den = __builtin_fmaf(sf, wx, den);
+    }
+    const long long o = ((long long)(b + a.out_b0) * a.nOutZ + zo) * a.ldx + a.out_c0 + c;
+    if (a.accumulate) {
+        a.num[o] += num;
+        a.den[o] += den;
+    } else {
+        a.num[o] = num;
+        a.den[o] = den;
+    }
+}
+
+// Order-preserving map of a float to an int key (for min/max); NaN never reaches it.
+__device__ __forceinline__ int fkey(float f) {
+    int i = __builtin_bit_cast(int, f);
+    return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+__device__ __forceinline__ float fkey_inv(int k) {
+    return __builtin_bit_cast(float, k >= 0 ? k : (k ^ 0x7FFFFFFF));
+}
+
+constexpr int kFinThreads = 256;
+
+// den < thr -> NaN; db = 20 log10(num / den); per-(plane, block) finite min/max partials.
+// Grid: (blocks_per_plane, nPlanes).
+__global__ __launch_bounds__(kFinThreads) void finalize_kernel(const float* __restrict__ num,
+                                                               const float* den,
+                                                               float* __restrict__ db, long long plane_elems,
+                                                               float thr, int* __restrict__ partial) {
+    const int plane = blockIdx.y;
+    const long long base = (long long)plane * plane_elems;
+    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < plane_elems;
+         e += (long long)gridDim.x * blockDim.x) {
+        float v;
+        if (den) {
+            const float d = den[base + e];
+            if (!(d >= thr)) {          // d < thr, or d is NaN
+                v = __builtin_nanf("");
+            } else {
+                v = 20.0f * log10f(num[base + e] / d);
+            }
+        } else {
+            v = 20.0f * log10f(num[base + e]);
+        }
+        db[base + e] = v;
+        if (__builtin_isfinite(v)) {
+            const int k = fkey(v);
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
+    }
+    __shared__ int smin[kFinThreads], smax[kFinThreads];
+    smin[threadIdx.x] = kmin;
+    smax[threadIdx.x] = kmax;
+    __syncthreads();
+    for (int s = kFinThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long long o = ((long long)plane * gridDim.x + blockIdx.x) * 2;
+        partial[o] = smin[0];
+        partial[o + 1] = smax[0];
+    }
+}
+
+// Per-plane clim (NaN when the plane has no finite value) and the global clim
+// (nanmin / nanmax over planes), one block.
+__global__ __launch_bounds__(kFinThreads) void clim_kernel(const int* __restrict__ partial, int nblk, int nplanes,
+                                                           float* __restrict__ plane_clim,
+                                                           float* __restrict__ clim) {
+    int gmin = 0x7FFFFFFF, gmax = (int)0x80000000;
+    for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
+        int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+        for (int i = 0; i < nblk; ++i) {
+            kmin = min(kmin, partial[((long long)p * nblk + i) * 2]);
+            kmax = max(kmax, partial[((long long)p * nblk + i) * 2 + 1]);
+        }
+        const bool any = kmin <= kmax;
+        plane_clim[2 * p] = any ? fkey_inv(kmin) : __builtin_nanf("");
+        plane_clim[2 * p + 1] = any ? fkey_inv(kmax) : __builtin_nanf("");
+        gmin = min(gmin, kmin);
+        gmax = max(gmax, kmax);
+    }
+    __shared__ int smin[kFinThreads], smax[kFinThreads];
+    smin[threadIdx.x] = gmin;
+    smax[threadIdx.x] = gmax;
+    __syncthreads();
+    for (int s = kFinThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const bool any = smin[0] <= smax[0];
+        clim[0] = any ? fkey_inv(smin[0]) : __builtin_nanf("");
+        clim[1] = any ? fkey_inv(smax[0]) : __builtin_nanf("");
+    }
+}
+
+// ---------------------------------------------------------------- legacy quantisation
+// All in float64 with no contraction, operation for operation as the host formulas:
+//   bits = min(uint16(round_half_away((db - c1) / (c2 - c1) * 65534)) + 1, 65535), NaN -> 0
+//   dequantised = (bits - 1) * (c2 - c1) / 65534 + c1, bits == 0 -> NaN
+#pragma clang fp contract(off)
+__device__ __forceinline__ unsigned db_to_bits_1(double db, double c1, double c2) {
+    if (__builtin_isnan(db)) return 0u;
+    const double v = (db - c1) / (c2 - c1) * 65534.0;
+    double r;
+    if (__builtin_isnan(v)) {
+        r = 0.0;
+    } else {
+        r = __builtin_floor(__builtin_fabs(v) + 0.5);
+        if (v < 0) r = -r;
+        if (v == 0) r = 0.0;
+    }
+    r = r < 0.0 ? 0.0 : (r > 65535.0 ? 65535.0 : r);
+    unsigned u = (unsigned)r + 1u;
+    return u > 65535u ? 65535u : u;
+}
+__device__ __forceinline__ double bits_to_db_1(unsigned bits, double c1, double c2) {
+    if (bits == 0u) return __builtin_nan("");
+    return ((double)bits - 1.0) * (c2 - c1) / 65534.0 + c1;
+}
+#pragma clang fp contract(fast)
+
+__global__ __launch_bounds__(256) void quantize_kernel(const float* __restrict__ db, uint16_t* __restrict__ bits,
+                                                       long long plane_elems, int nplanes,
+                                                       const float* __restrict__ clim,
+                                                       const float* __restrict__ plane_clim, int legacy_double) {
+    const long long n = plane_elems * nplanes;
+    double c1 = (double)clim[0], c2 = (double)clim[1];
+    if (c1 > c2) { const double t = c1; c1 = c2; c2 = t; }
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (long long)gridDim.x * blockDim.x) {
+        double v = (double)db[e];
+        if (legacy_double) {
+            const int p = (int)(e / plane_elems);
+            double p1 = (double)plane_clim[2 * p], p2 = (double)plane_clim[2 * p + 1];
+            if (p1 > p2) { const double t = p1; p1 = p2; p2 = t; }
+            v = bits_to_db_1(db_to_bits_1(v, p1, p2), p1, p2);
+        }
+        bits[e] = (uint16_t)db_to_bits_1(v, c1, c2);
+    }
+}
+
+}  // namespace oct

@@ -1,0 +1,182 @@
+"""Tiled-scan reconstruction driver on the hip device, for tiles resident in device memory.
+
+Mirrors octrecon/pipeline.py: ReconConfig (:49-92, the reconstruction and execution fields),
+Reconstructor.__init__ operator build (:155-190), process_row (:223-302), process_planes
+(:307-317) and the clim/quantisation part of run (:376-399).  File reading, TIFF writing and
+the web/CLI layers of the reference are not part of this package: tiles come from a mapping
+(xi, yi, zi) -> raw tensor (nYPixels, lines, N), or a callable returning the frames asked for.
+
+Per y-tile row: for every tile of the row (x-major, z-minor) and batch of frames, the fused
+spectral kernel produces the depth rows the tile's stitcher reads, the stitch kernel adds the
+tile's weighted, interpolated contribution into num/den, and the finalise kernel turns the row
+into dB with per-plane finite min/max.  ``run`` accumulates every row into one volume, so one
+finalise pass yields the global clim, and one quantise pass the uint16 codes.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .backend import get_xp, synchronize
+from .geometry import ScanGrid, build_spectral_geometry, build_tiled_geometry
+from .quantize import DeviceFinalizer
+from .spectral import HipSpectralProcessor
+from .stitching import HipTileStitcher, OpticalPathCorrection, TileStitcher
+
+
+@dataclass
+class ReconConfig:
+    dispersion_quadratic_term: float
+    focus_positions: object                 # number | list (1-based focus pixel per depth) | "none"
+    focus_sigma: float                      # pixels (effective Gaussian std sqrt(2) * sigma)
+    crop_z_range_mm: object = None          # None | [zmin, zmax]
+    output_pixel_size_um: object = None     # None | number (must equal the scan pixel size)
+    n_medium: float = 1.33
+    interp_method: str = "sinc5"
+    apply_path_length_correction: bool = True
+    device: str = "hip"
+    precision: str = "float32"
+    batch_frames: int = 50
+    legacy_double_quantization: bool = False
+    spectral_rows: str = "needed"           # "needed": only the depth rows stitching reads; "all"
+
+
+class Cancelled(Exception):
+    pass
+
+
+class Reconstructor:
+    def __init__(self, cfg: ReconConfig, scan: ScanGrid, lambda_nm, tiles, apod_size: int,
+                 apod_mode: str = "lines", opc_poly=None, xp=None, log=None):
+        if cfg.precision != "float32":
+            raise ValueError("the hip device computes in float32")
+        self.cfg = cfg
+        self.scan = scan
+        self.log = log or (lambda *a: None)
+        self.xp, self.device = (xp, "hip") if xp is not None else get_xp(cfg.device)
+        self.torch = self.xp.torch
+        self.tiles = tiles
+        self.sg = build_spectral_geometry(lambda_nm, cfg.dispersion_quadratic_term, cfg.n_medium,
+                                          cfg.interp_method)
+        nd = len(scan.z_depths_mm)
+        if isinstance(cfg.focus_positions, str) and cfg.focus_positions == "none":
+            focus = np.full(nd, np.nan)
+        else:
+            focus = np.atleast_1d(np.asarray(cfg.focus_positions, float)).ravel()
+            if focus.size == 1:
+                focus = np.full(nd, focus[0])
+        self.focus = focus
+        self.tg = build_tiled_geometry(scan, self.sg.z_um, focus, cfg.output_pixel_size_um, cfg.crop_z_range_mm)
+        poly = opc_poly if cfg.apply_path_length_correction else None
+        self.oc = (OpticalPathCorrection.build(poly, self.tg.tile_x_mm, self.tg.tile_y_mm, self.tg.tile_z_mm)
+                   if poly is not None else None)
+        self.sp = HipSpectralProcessor(self.sg, apod_size, self.xp, apod_mode=apod_mode)
+        self.n_out = (len(self.tg.out_y_mm), len(self.tg.out_z_mm), len(self.tg.out_x_mm))
+        self.stitchers = {}
+        for xi, xc in enumerate(scan.x_centers_mm):
+            for zi, zd in enumerate(scan.z_depths_mm):
+                base = TileStitcher(self.tg, self.oc, xc, zd, focus[zi], cfg.focus_sigma)
+                self.stitchers[(xi, zi)] = HipTileStitcher(base, self.xp, self.n_out[1])
+        self.fin = DeviceFinalizer(self.xp)
+        self._mag = None
+        self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
+
+    # ------------------------------------------------------------------ data access
+    def tile_frames(self, xi, yi, zi, frames):
+        src = self.tiles
+        if callable(src):
+            return src(xi, yi, zi, frames)
+        t = src[(xi, yi, zi)]
+        f = np.asarray(frames)
+        if f.size and np.all(np.diff(f) == 1):
+            return t[int(f[0]):int(f[-1]) + 1]
+        return t[self.torch.as_tensor(f, device=t.device)]
+
+    def _mag_buffer(self, n):
+        if self._mag is None or self._mag.numel() < n:
+            self._mag = self.torch.empty(n, dtype=self.torch.float32, device=self.xp.device)
+        return self._mag
+
+    # ------------------------------------------------------------------ core
+    def _accumulate_row(self, yi, frames, num, den, b_off=0, cancel=None):
+        cfg = self.cfg
+        frames = np.asarray(frames)
+        nF = len(frames)
+        B = max(1, int(cfg.batch_frames))
+        fr_dev = self.torch.as_tensor(frames.astype(np.int32), device=self.xp.device)
+        for (xi, zi) in self.scan.tiles_in_row(yi):
+            st = self.stitchers[(xi, zi)]
+            if st.empty:
+                continue
+            if cfg.spectral_rows == "needed":
+                lo, hi = st.needed_rows()
+            else:
+                lo, hi = 0, self.sp.nz
+            for s in range(0, nF, B):
+                if cancel is not None and cancel.is_set():
+                    raise Cancelled()
+                fr = frames[s:s + B]
+                raw = self.tile_frames(xi, yi, zi, fr)
+                nX = self.sp.n_scan_lines(raw.shape[1])
+                mbuf = self._mag_buffer(len(fr) * nX * (hi - lo)).narrow(0, 0, len(fr) * nX * (hi - lo))
+                mag = self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
+                st.contribute_into(mag, fr_dev[s:s + len(fr)], num, den, b0=b_off + s, mag_row0=lo)
+
+    def process_row(self, yi: int, frames=None, cancel=None):
+        """Reconstruct the output planes of y-tile row `yi` (tile-local frames, default all).
+        Returns (db, plane_clim) device tensors: (nF, nOutZ, nOutX) float32 and (nF, 2)."""
+        frames = np.arange(self.scan.n_y_px) if frames is None else np.asarray(frames)
+        _, nz_out, nx_out = self.n_out
+        num = self.xp.zeros((len(frames), nz_out, nx_out))
+        den = self.xp.zeros((len(frames), nz_out, nx_out))
+        self._accumulate_row(yi, frames, num, den, cancel=cancel)
+        db, pcl, _ = self.fin.finalize(num, den)
+        return db, pcl
+
+    def process_planes(self, y_indices0):
+        """Arbitrary output y planes (0-based) -> host float32 (n, nOutZ, nOutX)."""
+        out, by_row = {}, {}
+        for y in y_indices0:
+            r, f = divmod(int(y), self.scan.n_y_px)
+            by_row.setdefault(r, []).append((y, f))
+        for r, lst in by_row.items():
+            db, _ = self.process_row(r, [f for _, f in lst])
+            host = db.cpu().numpy()
+            for (y, _), p in zip(lst, host):
+                out[y] = p
+        return np.stack([out[y] for y in y_indices0])
+
+    def allocate_volume(self, rows=None):
+        rows = list(range(len(self.scan.y_centers_mm))) if rows is None else list(rows)
+        ny = self.scan.n_y_px
+        _, nz_out, nx_out = self.n_out
+        shape = (len(rows) * ny, nz_out, nx_out)
+        vol = {k: self.xp.zeros(shape) for k in ("num", "den")}
+        vol["db"] = self.xp.empty(shape)
+        vol["bits"] = self.torch.empty(shape, dtype=self.torch.uint16, device=self.xp.device)
+        vol["rows"] = rows
+        return vol
+
+    def reconstruct_volume(self, vol, cancel=None):
+        """All requested y-tile rows into one device volume: dB, plane clims, global clim and
+        uint16 codes (single finalise and quantise passes).  No host synchronisation."""
+        ny = self.scan.n_y_px
+        vol["num"].zero_()
+        vol["den"].zero_()
+        for k, r in enumerate(vol["rows"]):
+            self._accumulate_row(r, np.arange(ny), vol["num"], vol["den"], b_off=k * ny, cancel=cancel)
+        db, pcl, cl = self.fin.finalize(vol["num"], vol["den"], db=vol["db"])
+        self.fin.quantize(db, cl, pcl, legacy_double=self.cfg.legacy_double_quantization, out=vol["bits"])
+        vol["plane_clim"], vol["clim"] = pcl, cl
+        return vol
+
+    def run(self, rows=None):
+        t0 = time.perf_counter()
+        vol = self.reconstruct_volume(self.allocate_volume(rows))
+        synchronize(self.xp)
+        clim = vol["clim"].cpu().numpy().astype(float)
+        return {"device": self.device, "output_shape_yzx": list(vol["db"].shape), "clim_dB": tuple(clim),
+                "reconstruction_seconds": time.perf_counter() - t0, "volume": vol}

@@ -1,0 +1,185 @@
+"""Raw spectral B-scans -> depth-resolved magnitude on the MI355X (``hip`` device).
+
+Drop-in for the reference's SpectralProcessor / TorchSpectralProcessor
+(octrecon/core/spectral.py:84-165, octrecon/core/torch_backend.py:66-119): same constructor
+meaning, same ``apod_size``/``n``/``nz`` attributes, same ``magnitude`` / ``linearise`` /
+``complex_scan`` results, computed by the fused kernel of liboct_hip (background subtraction,
+mean removal, banded sinc, complex window, 2048-point inverse FFT, crop, |.|).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .geometry import SpectralGeometry
+
+_RAW_CODE = {np.dtype(np.int16): _lib.OCT_RAW_I16, np.dtype(np.uint16): _lib.OCT_RAW_U16,
+             np.dtype(np.float32): _lib.OCT_RAW_F32}
+
+
+def _padded_band(sg: SpectralGeometry):
+    """Band operator padded to the kernel's tap window (12 or 16): weights are appended as
+    zeros and the window start is pulled left where it would run past N-1, with the leading
+    positions weighted 0 (zero terms leave the tap sums unchanged)."""
+    N, T = sg.band_w.shape
+    TP = 12 if T <= 12 else 16
+    if T > 16:
+        raise ValueError(f"sinc band of {T} taps exceeds the kernel's 16-tap window")
+    start = sg.band_start.astype(np.int64) + 0
+    w = np.zeros((N, TP), np.float64)
+    newstart = np.minimum(start, N - TP)
+    off = start - newstart
+    rows = np.arange(N)
+    for t in range(T):
+        w[rows, off + t] = sg.band_w[:, t]
+    return newstart.astype(np.int32), w.astype(np.float32), TP
+
+
+class SpectralPlan:
+    """Device copy of the spectral operators (owns an oct_plan)."""
+
+    def __init__(self, sg: SpectralGeometry, device_index: int = 0):
+        if sg.n != 2048:
+            raise ValueError(f"the hip spectral kernel supports N = 2048 spectral pixels (got {sg.n})")
+        self.n = sg.n
+        start, w, TP = _padded_band(sg)
+        win = np.ascontiguousarray(np.stack([sg.window.real, sg.window.imag], -1).astype(np.float32))
+        self.taps = TP
+        self._start, self._w, self._win = start, w, win
+        h = ctypes.c_void_p()
+        L = _lib.lib()
+        _lib.check(L.oct_plan_create(ctypes.byref(h), device_index, self.n, TP, start.ctypes.data,
+                                     w.ctypes.data, win.ctypes.data), "oct_plan_create")
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().oct_plan_destroy(h)
+            except Exception:  # noqa: BLE001
+                pass
+            self.handle = None
+
+
+class HipSpectralProcessor:
+    """Same interface/results as SpectralProcessor.magnitude, on the hip device (float32).
+
+    sg: SpectralGeometry; apod_size: background A-lines at the start of each raw file
+    (apod_mode 'lines') or ignored (apod_mode 'frame_mean': background = mean of all A-lines
+    of each group of apod_group files).  A-scan binning > 1 is not supported by this device."""
+
+    def __init__(self, sg: SpectralGeometry, apod_size: int, xp, ascan_binning: int = 1,
+                 apod_mode: str = "lines", apod_group: int = 1):
+        if apod_mode not in ("lines", "frame_mean"):
+            raise ValueError(apod_mode)
+        if int(ascan_binning) > 1:
+            raise NotImplementedError("A-scan binning is not implemented on the hip device")
+        self.xp = xp
+        self.torch = xp.torch
+        self.dtype = np.dtype(np.float32)
+        self.apod_mode = apod_mode
+        self.apod_group = max(1, int(apod_group))
+        self.ascan_binning = 1
+        self.raw_apod_size = apod_size
+        self.apod_size = 1 if apod_mode == "frame_mean" else apod_size
+        self.n = sg.n
+        self.nz = self.n // 2
+        dev = xp.device
+        self.plan = SpectralPlan(sg, dev.index if dev.index is not None else 0)
+        self._ws = None
+        # optional list collecting (start, end) events around each fused-kernel launch on the
+        # current stream (the stream the kernel is enqueued on)
+        self.timing_events = None
+
+    # ------------------------------------------------------------------ helpers
+    def _device_raw(self, raw):
+        torch = self.torch
+        t = raw if isinstance(raw, torch.Tensor) else self.xp.asarray(np.ascontiguousarray(raw))
+        if t.device != self.xp.device:
+            t = t.to(self.xp.device)
+        if t.dtype == torch.int32:       # the reference's torch backend widens uint16 to int32
+            t = t.to(torch.float32)
+        if t.dtype not in (torch.int16, torch.uint16, torch.float32):
+            t = t.to(torch.float32)
+        t = t.contiguous()
+        if t.data_ptr() % 16:
+            t = t.clone()
+        return t
+
+    def _code(self, t):
+        torch = self.torch
+        return {torch.int16: _lib.OCT_RAW_I16, torch.uint16: _lib.OCT_RAW_U16,
+                torch.float32: _lib.OCT_RAW_F32}[t.dtype]
+
+    def _workspace(self, n_files):
+        need = _lib.lib().oct_spectral_workspace_bytes(self.plan.handle, n_files)
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.xp.device)
+        return self._ws
+
+    def n_scan_lines(self, lines: int) -> int:
+        return lines - self.raw_apod_size if self.apod_mode == "lines" else lines
+
+    def run(self, raw, kind: int, row0: int = 0, rows: int | None = None, out=None, out_stride=None):
+        """Launch the fused kernel on the current stream; returns the output tensor."""
+        torch = self.torch
+        t = self._device_raw(raw)
+        if t.dim() != 3 or t.shape[-1] != self.n:
+            raise ValueError(f"raw must be (B, lines, {self.n}), got {tuple(t.shape)}")
+        B, lines, _ = t.shape
+        nX = self.n_scan_lines(lines)
+        if kind == _lib.OCT_OUT_LINEARISED:
+            rows = self.n
+            row0 = 0
+        elif rows is None:
+            rows = self.nz - row0
+        if out is None:
+            if kind == _lib.OCT_OUT_COMPLEX:
+                out = torch.empty((B, nX, rows), dtype=torch.complex64, device=self.xp.device)
+            else:
+                out = torch.empty((B, nX, rows), dtype=torch.float32, device=self.xp.device)
+            out_stride = rows
+        elif out_stride is None:
+            out_stride = rows
+        if B == 0 or nX <= 0:
+            if nX <= 0:
+                raise ValueError(f"raw has {lines} lines: no scan A-lines after {self.raw_apod_size} background lines")
+            return out
+        ws = self._workspace(B)
+        mode = _lib.OCT_APOD_LINES if self.apod_mode == "lines" else _lib.OCT_APOD_FRAME_MEAN
+        L = _lib.lib()
+        code = self._code(t)
+        _lib.check(L.oct_background(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size,
+                                    self.apod_group, ws.data_ptr(), self.xp.stream()), "oct_background")
+        ev = self.timing_events
+        if ev is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        _lib.check(L.oct_spectral(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size, kind,
+                                  row0, rows, out.data_ptr(), out_stride, ws.data_ptr(), self.xp.stream()),
+                   "oct_spectral")
+        if ev is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            ev.append((e0, e1))
+        return out
+
+    # ------------------------------------------------------------------ reference API
+    def magnitude(self, raw):
+        """raw (B, interfSize, N) int16/uint16/float32 -> |scan| (B, nX, N/2) float32 device tensor."""
+        return self.run(raw, _lib.OCT_OUT_MAG)
+
+    def magnitude_rows(self, raw, row0: int, rows: int):
+        """Depth rows [row0, row0 + rows) of magnitude(raw) only (what a stitcher reads)."""
+        return self.run(raw, _lib.OCT_OUT_MAG, row0, rows)
+
+    def linearise(self, raw):
+        """raw -> equispaced interferogram (B, nX, N) float32 (no window)."""
+        return self.run(raw, _lib.OCT_OUT_LINEARISED)
+
+    def complex_scan(self, raw):
+        """raw -> complex A-scans (B, nX, N/2) complex64 (window, inverse FFT, crop)."""
+        return self.run(raw, _lib.OCT_OUT_COMPLEX)

@@ -1,0 +1,147 @@
+"""The oracle pinned against what does not depend on it: the MATLAB plane stored with the
+reference's tests, a brute-force DFT, closed forms and special cases.  CPU only."""
+import numpy as np
+import pytest
+
+import legacy_case
+from oracle import ref_path as R
+
+
+def _legacy_plane(precision):
+    c = legacy_case.load("telesto")
+    s = c["scan"]
+    geo = R.spectral_geometry(c["lambda_nm"], s["dispersion"], s["tissue_ri"])
+    tg = R.tiled_geometry(s["x_centers_mm"], s["y_centers_mm"], s["z_depths_mm"], s["tile_range_x_mm"],
+                          s["tile_range_y_mm"], s["n_x_px"], s["n_y_px"], geo.z_um, s["focus_pix"],
+                          s["pixel_size_um"])
+    raws = [c["raw"][k][None] for k in range(len(s["tiles"]))]
+    p = R.reconstruct_row(raws, [tuple(t) for t in s["tiles"]], [s["frame"]], geo, tg, s["x_centers_mm"],
+                          s["z_depths_mm"], s["focus_pix"], s["focus_sigma"], s["apod_size"], s["opc_poly"],
+                          precision)[0]
+    return p.astype(float), c["plane_db"].astype(float), c["tie"]
+
+
+def test_oracle_float64_matches_matlab_plane():
+    """Whole path (geometry, pre-FFT, FFT, optical path, focus weights, stitching, finalise)
+    against the legacy MATLAB plane (reference tests/test_formats.py:164-180, 1e-4 dB)."""
+    p, m, tie = _legacy_plane("float64")
+    assert p.shape == m.shape
+    fin = np.isfinite(p) & np.isfinite(m)
+    assert np.abs(p - m)[fin].max() < 1e-4
+    mism = np.isfinite(p) != np.isfinite(m)
+    assert not np.any(mism & ~tie)
+    assert mism.sum() <= max(1, 0.01 * tie.sum())
+
+
+def test_oracle_float32_cpu_path_matches_matlab_plane():
+    """The float32 reference CPU path (C pre-FFT with the Numba typing + pocketfft) is within
+    the reference's float32 tolerance (tests/test_torch_backend.py:66-73: < 1e-2 dB)."""
+    p, m, tie = _legacy_plane("float32")
+    fin = np.isfinite(p) & np.isfinite(m)
+    assert np.abs(p - m)[fin].max() < 1e-2
+    assert not np.any((np.isfinite(p) != np.isfinite(m)) & ~tie)
+
+
+def test_dispersion_window_identity():
+    """Reference tests/test_estimation.py:98-106: only beta changes the window."""
+    lam = np.linspace(800, 1000, 256)
+    g0 = R.spectral_geometry(lam, 0.0, 1.33)
+    g = R.spectral_geometry(lam, 8.949e7, 1.33)
+    k = 2 * np.pi / g0.lambda_eq
+    assert np.allclose(g0.window * np.exp(-1j * 8.949e7 * (k - k.mean()) ** 2), g.window, rtol=1e-9, atol=1e-9)
+    # Hann normalised to unit rms
+    assert abs(np.mean(np.abs(g0.window) ** 2) - 1) < 1e-12
+
+
+def test_sinc_is_identity_on_equispaced_k():
+    """lambda linear in 1/k: the equispaced grid coincides with the pixels, every sinc weight
+    is sinc(integer) -> linearisation returns the (mean-removed, re-added) input."""
+    N = 64
+    k = np.linspace(2 * np.pi / 800, 2 * np.pi / 1000, N)
+    lam = 2 * np.pi / k
+    g = R.spectral_geometry(lam, 0.0, 1.33)
+    W = np.zeros((N, N))
+    for i in range(N):
+        for t in range(g.idx.shape[1]):
+            W[i, g.idx[i, t]] += g.w[i, t]
+    assert np.allclose(W, np.eye(N), atol=1e-12)
+
+
+def test_fft_against_brute_force_dft():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(3, 64)) + 1j * rng.normal(size=(3, 64))
+    import scipy.fft
+    assert np.allclose(scipy.fft.ifft(x.astype(np.complex64), axis=-1), R.dft_brute(x), atol=1e-5)
+
+
+def test_c_prefft_matches_float64_restatement():
+    """Two independent restatements of the pre-FFT stage (C with the Numba typing; numpy
+    float64) agree to float32 rounding, for int16, uint16 and float32 raw input."""
+    rng = np.random.default_rng(1)
+    lam = 1300 + 0.0625 * (np.arange(256) - 127.5)
+    g = R.spectral_geometry(lam, 3e7, 1.33)
+    base = rng.integers(500, 3500, size=(2, 13, 256))
+    for dt in (np.int16, np.uint16, np.float32):
+        raw = base.astype(dt)
+        spec = R.fused_pre_fft(raw, 5, g)
+        x = raw.astype(np.float64)
+        d = x[:, 5:, :] - x[:, :5, :].mean(axis=1, keepdims=True)
+        mv = d.mean(-1, keepdims=True)
+        dd = d - mv
+        eq = np.zeros_like(dd)
+        for t in range(g.idx.shape[1]):
+            eq += g.w[:, t] * dd[..., g.idx[:, t]]
+        ref = (eq + mv) * g.window
+        assert np.abs(spec - ref).max() <= 2e-6 * np.abs(ref).max()
+
+
+def test_c_prefft_background_only_is_zero():
+    lam = 1300 + 0.0625 * (np.arange(256) - 127.5)
+    g = R.spectral_geometry(lam, 0.0, 1.33)
+    line = np.random.default_rng(2).integers(100, 4000, 256).astype(np.int16)
+    raw = np.broadcast_to(line, (2, 9, 256)).copy()
+    assert np.all(R.fused_pre_fft(raw, 4, g) == 0)
+
+
+def test_interp_matrix_properties():
+    src = np.array([0.0, 1.0, 2.5, 4.0])
+    dst = np.array([-0.5, 0.0, 0.5, 1.0 + 1e-12, 2.0, 4.0, 4.5])
+    W = R.interp_matrix(src, dst)
+    f = 3 * src - 1
+    inside = (dst >= 0) & (dst <= 4)
+    assert np.allclose((W @ f)[inside], 3 * dst[inside] - 1)
+    assert np.all(W[~inside] == 0)
+    assert np.allclose(W[inside].sum(1), 1)
+    assert W[3, 1] == 1.0 and W[3, 2] == 0.0          # node snapping within 1e-9
+
+
+def test_quantisation_closed_forms():
+    clim = (-10.0, 50.0)
+    db = np.array([np.nan, -np.inf, np.inf, -10.0, 50.0, 20.0, -20.0, 80.0])
+    b = R.db_to_bits(db, clim)
+    assert b.tolist() == [0, 1, 65535, 1, 65535, 32768, 1, 65535]
+    # round half away from zero (not to even) where the scaling is exact
+    assert R.db_to_bits(np.array([0.5, 1.5, 2.5, 2.4999]), (0.0, 65534.0)).tolist() == [2, 3, 4, 3]
+    lsb = 60.0 / 65534
+    # dequantise(quantise) is within half an LSB
+    x = np.random.default_rng(3).uniform(-10, 50, 1000)
+    back = R.bits_to_db(R.db_to_bits(x, clim), clim)
+    assert np.abs(back - x).max() <= 0.5 * lsb * (1 + 1e-9)
+
+
+def test_factor_z_floor_and_peak():
+    f = R.factor_z(100, 40.0, 10.0)
+    assert f.argmax() == 39 and abs(f.max() - (1 + np.exp(-4.5))) < 1e-15
+    assert f.min() > np.exp(-4.5)
+    # effective Gaussian std is sqrt(2) sigma: e^-1/2 point at 40 +- sqrt(2)*10
+    z = 40 + np.sqrt(2) * 10
+    val = np.exp(-(z - 40) ** 2 / (2 * 10.0) ** 2)
+    assert abs(val - np.exp(-0.5)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 7])
+def test_colon_linspace_endpoints(n):
+    y = R.linspace_m(0.3, 1.7, n + 1)
+    assert y[0] == 0.3 and y[-1] == 1.7
+    c = R.colon_m(0.0, 0.1, 0.3)
+    assert len(c) == 4 and c[-1] == pytest.approx(0.3)
