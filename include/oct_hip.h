@@ -117,6 +117,31 @@ int oct_finalize(const float* num, const float* den, float* db, int n_planes, lo
 int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_elems, const float* clim,
                  const float* plane_clim, int legacy_double, void* stream);
 
+/* ---- dispersion sharpness search (octrecon/estimation/dispersion.py:46-107, 154-239)
+ * One search evaluation is the spectral path with the candidate's window: replace the plan's
+ * window (N, 2) unscaled (synchronises `stream`; no launch with this plan may be in flight on
+ * another stream), then score magnitudes with the top-fraction metric:
+ * out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over x and z >= z_min),
+ * M = n_x (n_z - z_min), mag (n_bscans, n_x, n_z) float32, out float64. */
+int oct_plan_set_window(oct_plan* plan, const float* window, void* stream);
+int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
+                       void* stream);
+
+/* ---- operator (MFMA) form: the whole linear spectral map as one complex operator
+ * A = F_crop diag(window / N) S' (N/2-point crop [z0, z0+nzc), S' = banded sinc + rank-1 mean
+ * term), built on the device in float64, stored as fp16 hi/lo scaled by 2^scale_log2, and
+ * applied as a split-precision (3-product) fp16 GEMM with fp32 accumulation on MFMA.
+ * band_w here is float64 (N, taps).  op64_host (optional, (nzc, N, 2) doubles) receives the
+ * float64 operator.  Replaces, for magnitudes, the same reference calls as oct_spectral. */
+typedef struct oct_opplan oct_opplan;
+int oct_opplan_create(oct_opplan** plan, int device, int n_lambda, int taps, const int32_t* band_start,
+                      const double* band_w, const double* window, int z0, int nzc, int* scale_log2,
+                      double* op64_host);
+int oct_opplan_destroy(oct_opplan* plan);
+/* magnitudes |A (raw - bg)| of bins [z0, z0+nzc) -> out (M, out_stride) float32 */
+int oct_opspectral(const oct_opplan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                   int apod_size, float* out, long long out_stride, const float* bg, void* stream);
+
 const char* oct_last_error(void);
 
 #ifdef __cplusplus

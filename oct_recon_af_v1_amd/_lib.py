@@ -44,6 +44,11 @@ SIGNATURES = {
     "oct_finalize_workspace_bytes": (_sz, [_i, _ll]),
     "oct_finalize": (_i, [_vp, _vp, _vp, _i, _ll, _f, _vp, _vp, _vp, _sz, _vp]),
     "oct_quantize": (_i, [_vp, _vp, _i, _ll, _vp, _vp, _i, _vp]),
+    "oct_opplan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), _vp]),
+    "oct_opplan_destroy": (_i, [_vp]),
+    "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
+    "oct_plan_set_window": (_i, [_vp, _vp, _vp]),
+    "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp]),
     "oct_last_error": (ctypes.c_char_p, []),
 }
 

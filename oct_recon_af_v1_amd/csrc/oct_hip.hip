@@ -8,6 +8,12 @@
 #include "../../include/oct_hip.h"
 #include "spectral.hpp"
 #include "stitch.hpp"
+#include "opform.hpp"
+#include "estimate.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
 
 namespace {
 
@@ -299,6 +305,197 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
     const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(oct::quantize_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), db,
                        bits, plane_elems, n_planes, clim, plane_clim, legacy_double);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+// ---------------------------------------------------------------- dispersion search support
+int oct_plan_set_window(oct_plan* p, const float* window, void* stream) {
+    if (!p || !window) return fail(OCT_EINVAL, "null argument");
+    std::vector<float> w(2 * (size_t)p->n);
+    for (int i = 0; i < 2 * p->n; ++i) w[i] = window[i] * (1.0f / (float)p->n);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(p->win, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return OCT_OK;
+}
+
+int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
+                       void* stream) {
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_bscans <= 0) return OCT_OK;
+    if (n_x <= 0 || z_min < 0 || z_min >= n_z) return fail(OCT_EINVAL, "bad metric shape (%d, %d, zmin %d)", n_x, n_z, z_min);
+    if (!(frac > 0.0 && frac <= 1.0)) return fail(OCT_EINVAL, "fraction %g outside (0, 1]", frac);
+    const long long M = (long long)n_x * (n_z - z_min);
+    long long k = (long long)(M * frac);
+    if (k < 1) k = 1;
+    hipLaunchKernelGGL(oct::topfrac_metric_kernel, dim3(n_bscans), dim3(oct::kSelThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), mag, n_x, n_z, z_min, k, out);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+// ---------------------------------------------------------------- operator (MFMA) form
+struct oct_opplan {
+    int device;
+    int n;
+    int z0, nzc, r_pad;
+    int sa;
+    _Float16* op_hi;
+    _Float16* op_lo;
+};
+
+int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, const int32_t* band_start,
+                      const double* band_w, const double* window, int z0, int nzc, int* scale_log2,
+                      double* op64_host) {
+    if (!out || !band_start || !band_w || !window) return fail(OCT_EINVAL, "null argument");
+    *out = nullptr;
+    const int N = n_lambda;
+    if (N != oct::kN) return fail(OCT_EUNSUPPORTED, "n_lambda %d unsupported (2048 only)", N);
+    if (taps <= 0 || taps > 64) return fail(OCT_EINVAL, "taps %d", taps);
+    if (z0 < 0 || nzc <= 0 || z0 + nzc > N / 2) return fail(OCT_EINVAL, "bins [%d, %d) outside [0, %d)", z0, z0 + nzc, N / 2);
+    // transposed band: for each source j the output rows i that use it, with their weights
+    std::vector<std::vector<std::pair<int, double>>> cols(N);
+    std::vector<double> u(N, 1.0);
+    for (int i = 0; i < N; ++i) {
+        const int s0 = band_start[i];
+        if (s0 < 0 || s0 + taps > N) return fail(OCT_EINVAL, "band_start[%d]=%d", i, s0);
+        for (int t = 0; t < taps; ++t) {
+            const double w = band_w[(size_t)i * taps + t];
+            u[i] -= w;
+            if (w != 0.0) cols[s0 + t].push_back({i, w});
+        }
+    }
+    int TT = 1;
+    for (auto& c : cols) TT = std::max<int>(TT, (int)c.size());
+    std::vector<int> tb_start(N), tb_count(N);
+    std::vector<double> tb_w((size_t)N * TT, 0.0);
+    std::vector<int> tb_idx((size_t)N * TT, 0);
+    for (int j = 0; j < N; ++j) {
+        tb_count[j] = (int)cols[j].size();
+        tb_start[j] = cols[j].empty() ? 0 : cols[j][0].first;
+        for (int t = 0; t < tb_count[j]; ++t) {
+            if (cols[j][t].first != tb_start[j] + t) return fail(OCT_EINVAL, "sinc band is not contiguous in column %d", j);
+            tb_w[(size_t)j * TT + t] = cols[j][t].second;
+        }
+    }
+    // scale: keep |A| * 2^sa below 2^14 (fp16 headroom for the hi part)
+    double amax_win = 0.0;
+    for (int i = 0; i < N; ++i) amax_win = std::max(amax_win, std::hypot(window[2 * i], window[2 * i + 1]));
+    double colsum = 0.0;
+    for (int j = 0; j < N; ++j) {
+        double s = 0.0;
+        for (auto& e : cols[j]) s += std::fabs(e.second);
+        colsum = std::max(colsum, s);
+    }
+    double usum = 0.0;
+    for (int i = 0; i < N; ++i) usum += std::fabs(u[i]);
+    const double bound = amax_win / N * (colsum + usum / N);
+    const int sa = (int)std::floor(std::log2(16384.0 / bound));
+    const int nzc_pad = (nzc + 15) / 16 * 16;
+    const int r_pad = ((2 * nzc_pad + oct::kOpRows - 1) / oct::kOpRows) * oct::kOpRows;
+
+    int prev = 0;
+    HIP_TRY(hipGetDevice(&prev));
+    HIP_TRY(hipSetDevice(device));
+    oct_opplan* p = new oct_opplan();
+    p->device = device;
+    p->n = N;
+    p->z0 = z0;
+    p->nzc = nzc;
+    p->r_pad = r_pad;
+    p->sa = sa;
+    int* d_start = nullptr;
+    int* d_count = nullptr;
+    double *d_w = nullptr, *d_win = nullptr, *d_u = nullptr, *d_op64 = nullptr;
+    double2* d_c = nullptr;
+    int rc = OCT_OK;
+    const size_t opbytes = sizeof(_Float16) * (size_t)r_pad * N;
+    if (hipMalloc(&p->op_hi, opbytes) != hipSuccess || hipMalloc(&p->op_lo, opbytes) != hipSuccess ||
+        hipMemset(p->op_hi, 0, opbytes) != hipSuccess || hipMemset(p->op_lo, 0, opbytes) != hipSuccess ||
+        hipMalloc(&d_start, sizeof(int) * N) != hipSuccess || hipMalloc(&d_count, sizeof(int) * N) != hipSuccess ||
+        hipMalloc(&d_w, sizeof(double) * N * TT) != hipSuccess || hipMalloc(&d_win, sizeof(double) * 2 * N) != hipSuccess ||
+        hipMalloc(&d_u, sizeof(double) * N) != hipSuccess || hipMalloc(&d_c, sizeof(double2) * nzc) != hipSuccess ||
+        (op64_host && hipMalloc(&d_op64, sizeof(double) * 2 * (size_t)nzc * N) != hipSuccess) ||
+        hipMemcpy(d_start, tb_start.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_count, tb_count.data(), sizeof(int) * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_w, tb_w.data(), sizeof(double) * N * TT, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_win, window, sizeof(double) * 2 * N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_u, u.data(), sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail(OCT_EHIP, "operator upload failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+    if (rc == OCT_OK) {
+        hipLaunchKernelGGL(oct::opbuild_c_kernel, dim3(nzc), dim3(256), 0, 0, d_win, d_u, N, z0, d_c);
+        hipLaunchKernelGGL(oct::opbuild_a_kernel, dim3((N + 255) / 256, nzc_pad), dim3(256), 0, 0, d_start, d_count,
+                           d_w, TT, d_win, d_c, N, z0, nzc, nzc_pad, std::ldexp(1.0, sa), p->op_hi, p->op_lo, d_op64);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess && op64_host)
+            e = hipMemcpy(op64_host, d_op64, sizeof(double) * 2 * (size_t)nzc * N, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(OCT_EHIP, "operator build failed: %s", hipGetErrorString(e));
+    }
+    hipFree(d_start);
+    hipFree(d_count);
+    hipFree(d_w);
+    hipFree(d_win);
+    hipFree(d_u);
+    hipFree(d_c);
+    if (d_op64) hipFree(d_op64);
+    hipSetDevice(prev);
+    if (rc != OCT_OK) {
+        oct_opplan_destroy(p);
+        return rc;
+    }
+    if (scale_log2) *scale_log2 = sa;
+    *out = p;
+    return OCT_OK;
+}
+
+int oct_opplan_destroy(oct_opplan* p) {
+    if (!p) return OCT_OK;
+    int prev = 0;
+    hipGetDevice(&prev);
+    hipSetDevice(p->device);
+    if (p->op_hi) hipFree(p->op_hi);
+    if (p->op_lo) hipFree(p->op_lo);
+    hipSetDevice(prev);
+    delete p;
+    return OCT_OK;
+}
+
+int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                   int apod_size, float* out, long long out_stride, const float* bg, void* stream) {
+    if (!p || !out || !bg) return fail(OCT_EINVAL, "null argument");
+    int line0, nX;
+    int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, &line0, &nX);
+    if (rc) return rc;
+    if (n_files == 0) return OCT_OK;
+    if (out_stride < p->nzc) return fail(OCT_EINVAL, "out_stride %lld < bins %d", out_stride, p->nzc);
+    oct::OpGemmArgs a;
+    a.raw = raw;
+    a.bg = bg;
+    a.op_hi = p->op_hi;
+    a.op_lo = p->op_lo;
+    a.out = out;
+    a.M = (long long)n_files * nX;
+    a.lines = lines;
+    a.line0 = line0;
+    a.nX = nX;
+    a.N = p->n;
+    a.nzc = p->nzc;
+    a.row_blocks = p->r_pad / oct::kOpRows;
+    a.col_blocks = (int)((a.M + oct::kOpCols - 1) / oct::kOpCols);
+    a.out_stride = out_stride;
+    a.inv_scale = (float)std::ldexp(1.0, -p->sa);
+    const long long nblk = (long long)a.row_blocks * a.col_blocks;
+    if (nblk > (1LL << 30)) return fail(OCT_EINVAL, "batch too large");
+    const int grid = (int)((nblk + 7) / 8 * 8);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (raw_dtype) {
+        case OCT_RAW_I16: hipLaunchKernelGGL(oct::opgemm_kernel<int16_t>, dim3(grid), dim3(256), 0, s, a); break;
+        case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(oct::opgemm_kernel<float>, dim3(grid), dim3(256), 0, s, a); break;
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

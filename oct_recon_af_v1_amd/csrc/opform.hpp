@@ -1,0 +1,273 @@
+// Operator ("MFMA") form of the spectral stage.
+//
+// The map from one background-subtracted spectrum d (N samples) to its cropped complex A-scan
+// is linear:  X[z] = sum_j A[z][j] d[j],  A = F_crop diag(win / N) S',
+//   S' = S + (1/N) (1 - S 1) 1^T   (banded sinc operator with the mean removal / re-add folded
+//   in as a rank-1 term), F the inverse-DFT kernel exp(+2 pi i z i / N).
+// Reference path it reproduces: octrecon/core/spectral.py:123-153 (linearise, window, ifft, crop).
+//
+// A is built on the GPU in float64 (opbuild_*), scaled by 2^sa and split into fp16 hi + lo
+// parts; rows are interleaved in blocks of 32 = (16 z real parts, the same 16 z imaginary
+// parts).  opgemm_kernel then applies it to batches of A-scans on the matrix cores with
+// v_mfma_f32_32x32x16_f16:  acc = Ahi dhi + Ahi dlo + Alo dhi  (fp32 accumulation), where
+// d = raw - bg is split on load into dhi = fp16(d), dlo = fp16(d - dhi).  The epilogue forms
+// |X| = 2^-sa sqrt(re^2 + im^2) in registers (each lane holds both parts of its bins).
+#pragma once
+#include "oct_common.hpp"
+
+namespace oct {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+// ------------------------------------------------------------------ operator build (FP64)
+// c[zl] = (1/N) sum_i (win_i / N) e^{2 pi i z i / N} u_i,  u = 1 - S 1.  One block per z.
+__global__ __launch_bounds__(256) void opbuild_c_kernel(const double* __restrict__ win, const double* __restrict__ u,
+                                                        int N, int z0, double2* __restrict__ cz) {
+    const int zl = blockIdx.x;
+    const long long z = z0 + zl;
+    double re = 0.0, im = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const long long m = (z * i) % N;
+        double s, c;
+        sincospi(2.0 * (double)m / (double)N, &s, &c);
+        const double wr = win[2 * i] / N, wi = win[2 * i + 1] / N;
+        // (wr + i wi)(c + i s) u_i
+        re += (wr * c - wi * s) * u[i];
+        im += (wr * s + wi * c) * u[i];
+    }
+    __shared__ double sr[256], si[256];
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cz[zl] = make_double2(sr[0] / N, si[0] / N);
+}
+
+// A[zl][j] = c[zl] + sum_t (win_i / N) e^{2 pi i z i / N} S[i][j],  i = tb_start[j] + t,
+// written as fp16 hi/lo (scaled by 2^sa) in the interleaved row layout; rows of z >= z0+nzc
+// (padding up to a multiple of 16) are zero.  Optionally also the float64 operator (nzc, N, 2).
+__global__ __launch_bounds__(256) void opbuild_a_kernel(const int* __restrict__ tb_start, const int* __restrict__ tb_count,
+                                                        const double* __restrict__ tb_w, int TT,
+                                                        const double* __restrict__ win, const double2* __restrict__ cz,
+                                                        int N, int z0, int nzc, int nzc_pad, double scale,
+                                                        _Float16* __restrict__ op_hi, _Float16* __restrict__ op_lo,
+                                                        double* __restrict__ op64) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int zl = blockIdx.y;
+    if (j >= N || zl >= nzc_pad) return;
+    double re = 0.0, im = 0.0;
+    if (zl < nzc) {
+        const long long z = z0 + zl;
+        const double2 c = cz[zl];
+        re = c.x;
+        im = c.y;
+        const int i0 = tb_start[j];
+        const int cnt = tb_count[j];
+        for (int t = 0; t < cnt; ++t) {
+            const int i = i0 + t;
+            const double w = tb_w[(long long)j * TT + t];
+            const long long m = (z * i) % N;
+            double s, cc;
+            sincospi(2.0 * (double)m / (double)N, &s, &cc);
+            const double wr = win[2 * i] / N, wi = win[2 * i + 1] / N;
+            re += (wr * cc - wi * s) * w;
+            im += (wr * s + wi * cc) * w;
+        }
+        if (op64) {
+            op64[((long long)zl * N + j) * 2] = re;
+            op64[((long long)zl * N + j) * 2 + 1] = im;
+        }
+    }
+    const int blk = zl >> 4, zi = zl & 15;
+    const long long rr = (long long)(32 * blk + zi) * N + j;
+    const long long ri = (long long)(32 * blk + 16 + zi) * N + j;
+    const double sre = re * scale, sim = im * scale;
+    const _Float16 hr = (_Float16)sre, hi_ = (_Float16)sim;
+    op_hi[rr] = hr;
+    op_hi[ri] = hi_;
+    op_lo[rr] = (_Float16)(sre - (double)hr);
+    op_lo[ri] = (_Float16)(sim - (double)hi_);
+}
+
+// ------------------------------------------------------------------ GEMM + magnitude
+constexpr int kOpCols = 256;       // A-scans per workgroup (4 waves x 64)
+constexpr int kOpRows = 128;       // operator rows per workgroup (4 tiles of 32 = 64 z bins)
+constexpr int kOpK = 32;           // k per stage
+constexpr int kOpPad = 40;         // LDS row stride in halves (80 B)
+
+struct OpGemmArgs {
+    const void* raw;
+    const float* bg;
+    const _Float16* op_hi;        // (R_pad, N)
+    const _Float16* op_lo;
+    float* out;                   // magnitude (M, out_stride), bins [0, nzc)
+    long long M;
+    int lines, line0, nX;
+    int N;
+    int nzc;                      // kept bins
+    int row_blocks;               // R_pad / kOpRows
+    int col_blocks;               // ceil(M / kOpCols)
+    long long out_stride;
+    float inv_scale;              // 2^-sa
+};
+
+template <typename RawT>
+__global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
+    __shared__ __align__(16) _Float16 dhi[kOpCols * kOpPad];
+    __shared__ __align__(16) _Float16 dlo[kOpCols * kOpPad];
+    __shared__ __align__(16) _Float16 ahi[kOpRows * kOpPad];
+    __shared__ __align__(16) _Float16 alo[kOpRows * kOpPad];
+
+    // XCD-aware tile order: consecutive workgroup ids go to different XCDs, so the row blocks
+    // of one A-scan block are given ids that land on the same XCD (shared L2 for its raw data).
+    const int nblk = a.row_blocks * a.col_blocks;
+    const int id = blockIdx.x;
+    int rb, cb;
+    {
+        const int xcd = id % 8;
+        const int slot = id / 8;
+        const int per_xcd = (nblk + 7) / 8;
+        const int lin = xcd * per_xcd + slot;     // position in the (col, row) order for this XCD
+        if (lin >= nblk) return;
+        cb = lin / a.row_blocks;
+        rb = lin - cb * a.row_blocks;
+    }
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const long long col0 = (long long)cb * kOpCols;
+    const int row0 = rb * kOpRows;
+
+    f32x16_t acc[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[r][c][v] = 0.f;
+
+    // this thread's A-scan for the data tile
+    const long long p = col0 + tid;
+    const bool pvalid = p < a.M;
+    const RawT* rrow = nullptr;
+    const float* brow = nullptr;
+    if (pvalid) {
+        const long long b = p / a.nX;
+        const long long x = p - b * a.nX;
+        rrow = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N;
+        brow = a.bg + b * a.N;
+    }
+    // operator rows loaded by this thread: row r = tid / 2, 16 halves at k + 16 * (tid & 1)
+    const int orow = tid >> 1;
+    const int ohalf = (tid & 1) * 16;
+    const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
+    const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
+
+    for (int k0 = 0; k0 < a.N; k0 += kOpK) {
+        // ---- stage data: d = raw - bg -> fp16 hi / lo
+        {
+            _Float16* hrow = dhi + tid * kOpPad;
+            _Float16* lrow = dlo + tid * kOpPad;
+            if (pvalid) {
+#pragma unroll
+                for (int g = 0; g < kOpK / 8; ++g) {
+                    float f[8];
+                    if constexpr (sizeof(RawT) == 2) {
+                        const uint4 u = *reinterpret_cast<const uint4*>(rrow + k0 + 8 * g);
+                        unpack16<RawT>(u, f);
+                    } else {
+                        const float4 u0 = *reinterpret_cast<const float4*>(rrow + k0 + 8 * g);
+                        const float4 u1 = *reinterpret_cast<const float4*>(rrow + k0 + 8 * g + 4);
+                        f[0] = u0.x; f[1] = u0.y; f[2] = u0.z; f[3] = u0.w;
+                        f[4] = u1.x; f[5] = u1.y; f[6] = u1.z; f[7] = u1.w;
+                    }
+                    const float4 b0 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g);
+                    const float4 b1 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g + 4);
+                    const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+                    half8_t h, l;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float d = f[e] - bb[e];
+                        const _Float16 dh = (_Float16)d;
+                        h[e] = dh;
+                        l[e] = (_Float16)(d - (float)dh);
+                    }
+                    *reinterpret_cast<half8_t*>(hrow + 8 * g) = h;
+                    *reinterpret_cast<half8_t*>(lrow + 8 * g) = l;
+                }
+            } else {
+                half8_t z = {};
+#pragma unroll
+                for (int g = 0; g < kOpK / 8; ++g) {
+                    *reinterpret_cast<half8_t*>(hrow + 8 * g) = z;
+                    *reinterpret_cast<half8_t*>(lrow + 8 * g) = z;
+                }
+            }
+        }
+        // ---- stage operator rows
+        {
+            const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
+            const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
+            half8_t* th = reinterpret_cast<half8_t*>(ahi + orow * kOpPad + ohalf);
+            half8_t* tl = reinterpret_cast<half8_t*>(alo + orow * kOpPad + ohalf);
+            th[0] = sh[0];
+            th[1] = sh[1];
+            tl[0] = sl[0];
+            tl[1] = sl[1];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kOpK; ks += 16) {
+            const int kk = ks + 8 * (lane >> 5);
+            half8_t bh[2], bl[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int col = wave * 64 + c * 32 + (lane & 31);
+                bh[c] = *reinterpret_cast<const half8_t*>(dhi + col * kOpPad + kk);
+                bl[c] = *reinterpret_cast<const half8_t*>(dlo + col * kOpPad + kk);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r * 32 + (lane & 31);
+                const half8_t ah = *reinterpret_cast<const half8_t*>(ahi + row * kOpPad + kk);
+                const half8_t al = *reinterpret_cast<const half8_t*>(alo + row * kOpPad + kk);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[c], acc[r][c], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- epilogue: |X| for 8 bins per lane per tile
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        if (pc >= a.M) continue;
+        float* orow_out = a.out + pc * a.out_stride;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);         // 0..15
+                const int zl = (row0 / 32 + r) * 16 + m;
+                if (zl < a.nzc) {
+                    const float re = acc[r][c][v], im = acc[r][c][v + 8];
+                    orow_out[zl] = __builtin_sqrtf(re * re + im * im) * a.inv_scale;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace oct

@@ -42,6 +42,7 @@ class ReconConfig:
     batch_frames: int = 50
     legacy_double_quantization: bool = False
     spectral_rows: str = "needed"           # "needed": only the depth rows stitching reads; "all"
+    form: str = "plain"                     # "plain": fused FP32 kernel; "mfma": fp16 3-product operator GEMM
 
 
 class Cancelled(Exception):
@@ -101,8 +102,16 @@ class Reconstructor:
         return self._mag
 
     # ------------------------------------------------------------------ core
+    def union_rows(self):
+        """[lo, hi): depth rows read by any stitcher (the operator form computes these for every tile)."""
+        rows = [st.needed_rows() for st in self.stitchers.values() if not st.empty]
+        return min(r[0] for r in rows), max(r[1] for r in rows)
+
     def _accumulate_row(self, yi, frames, num, den, b_off=0, cancel=None):
         cfg = self.cfg
+        if cfg.form not in ("plain", "mfma"):
+            raise ValueError(f"form must be plain|mfma, got {cfg.form!r}")
+        ulo, uhi = self.union_rows() if cfg.form == "mfma" else (0, 0)
         frames = np.asarray(frames)
         nF = len(frames)
         B = max(1, int(cfg.batch_frames))
@@ -111,7 +120,9 @@ class Reconstructor:
             st = self.stitchers[(xi, zi)]
             if st.empty:
                 continue
-            if cfg.spectral_rows == "needed":
+            if cfg.form == "mfma":
+                lo, hi = ulo, uhi
+            elif cfg.spectral_rows == "needed":
                 lo, hi = st.needed_rows()
             else:
                 lo, hi = 0, self.sp.nz
@@ -122,7 +133,10 @@ class Reconstructor:
                 raw = self.tile_frames(xi, yi, zi, fr)
                 nX = self.sp.n_scan_lines(raw.shape[1])
                 mbuf = self._mag_buffer(len(fr) * nX * (hi - lo)).narrow(0, 0, len(fr) * nX * (hi - lo))
-                mag = self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
+                if cfg.form == "mfma":
+                    mag = self.sp.run_operator(raw, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
+                else:
+                    mag = self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
                 st.contribute_into(mag, fr_dev[s:s + len(fr)], num, den, b0=b_off + s, mag_row0=lo)
 
     def process_row(self, yi: int, frames=None, cancel=None):

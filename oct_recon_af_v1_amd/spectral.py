@@ -19,7 +19,7 @@ _RAW_CODE = {np.dtype(np.int16): _lib.OCT_RAW_I16, np.dtype(np.uint16): _lib.OCT
              np.dtype(np.float32): _lib.OCT_RAW_F32}
 
 
-def _padded_band(sg: SpectralGeometry):
+def _padded_band(sg: SpectralGeometry, dtype=np.float32):
     """Band operator padded to the kernel's tap window (12 or 16): weights are appended as
     zeros and the window start is pulled left where it would run past N-1, with the leading
     positions weighted 0 (zero terms leave the tap sums unchanged)."""
@@ -34,7 +34,7 @@ def _padded_band(sg: SpectralGeometry):
     rows = np.arange(N)
     for t in range(T):
         w[rows, off + t] = sg.band_w[:, t]
-    return newstart.astype(np.int32), w.astype(np.float32), TP
+    return newstart.astype(np.int32), np.ascontiguousarray(w.astype(dtype)), TP
 
 
 class SpectralPlan:
@@ -64,6 +64,40 @@ class SpectralPlan:
             self.handle = None
 
 
+class OperatorPlan:
+    """Device copy of the fused spectral operator A for depth bins [z0, z0 + nzc) (owns an
+    oct_opplan): built on the device in float64, applied as a split-precision fp16 GEMM."""
+
+    def __init__(self, sg: SpectralGeometry, z0: int, nzc: int, device_index: int = 0, want_fp64: bool = False):
+        if sg.n != 2048:
+            raise ValueError(f"the operator form supports N = 2048 spectral pixels (got {sg.n})")
+        start, w, TP = _padded_band(sg, np.float64)
+        win = np.ascontiguousarray(np.stack([sg.window.real, sg.window.imag], -1).astype(np.float64))
+        self.z0, self.nzc, self.n = int(z0), int(nzc), sg.n
+        self.op64 = np.empty((nzc, sg.n, 2), np.float64) if want_fp64 else None
+        sa = ctypes.c_int(0)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().oct_opplan_create(ctypes.byref(h), device_index, sg.n, TP, start.ctypes.data,
+                                                w.ctypes.data, win.ctypes.data, self.z0, self.nzc, ctypes.byref(sa),
+                                                self.op64.ctypes.data if want_fp64 else None),
+                   "oct_opplan_create")
+        self.handle = h
+        self.scale_log2 = sa.value
+
+    def operator_fp64(self):
+        """Complex float64 operator (nzc, N) as built on the device (when requested)."""
+        return None if self.op64 is None else self.op64[..., 0] + 1j * self.op64[..., 1]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().oct_opplan_destroy(h)
+            except Exception:  # noqa: BLE001
+                pass
+            self.handle = None
+
+
 class HipSpectralProcessor:
     """Same interface/results as SpectralProcessor.magnitude, on the hip device (float32).
 
@@ -88,7 +122,9 @@ class HipSpectralProcessor:
         self.n = sg.n
         self.nz = self.n // 2
         dev = xp.device
+        self.sg = sg
         self.plan = SpectralPlan(sg, dev.index if dev.index is not None else 0)
+        self._ops = {}
         self._ws = None
         # optional list collecting (start, end) events around each fused-kernel launch on the
         # current stream (the stream the kernel is enqueued on)
@@ -161,6 +197,54 @@ class HipSpectralProcessor:
         _lib.check(L.oct_spectral(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size, kind,
                                   row0, rows, out.data_ptr(), out_stride, ws.data_ptr(), self.xp.stream()),
                    "oct_spectral")
+        if ev is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            ev.append((e0, e1))
+        return out
+
+    def operator_plan(self, z0: int, nzc: int, want_fp64: bool = False) -> OperatorPlan:
+        key = (int(z0), int(nzc))
+        op = self._ops.get(key)
+        if op is None or (want_fp64 and op.op64 is None):
+            dev = self.xp.device
+            op = OperatorPlan(self.sg, z0, nzc, dev.index if dev.index is not None else 0, want_fp64)
+            self._ops[key] = op
+        return op
+
+    def run_operator(self, raw, z0: int = 0, nzc: int | None = None, out=None, out_stride=None):
+        """Magnitudes of bins [z0, z0 + nzc) through the operator form (MFMA, fp16 3-product)."""
+        torch = self.torch
+        nzc = self.nz - z0 if nzc is None else int(nzc)
+        t = self._device_raw(raw)
+        if t.dim() != 3 or t.shape[-1] != self.n:
+            raise ValueError(f"raw must be (B, lines, {self.n}), got {tuple(t.shape)}")
+        if t.dtype == torch.float32:
+            raise ValueError("the operator form takes integer raw counts (int16 / uint16)")
+        B, lines, _ = t.shape
+        nX = self.n_scan_lines(lines)
+        if out is None:
+            out = torch.empty((B, nX, nzc), dtype=torch.float32, device=self.xp.device)
+            out_stride = nzc
+        elif out_stride is None:
+            out_stride = nzc
+        if B == 0 or nX <= 0:
+            if nX <= 0:
+                raise ValueError(f"raw has {lines} lines: no scan A-lines after {self.raw_apod_size} background lines")
+            return out
+        op = self.operator_plan(z0, nzc)
+        ws = self._workspace(B)
+        mode = _lib.OCT_APOD_LINES if self.apod_mode == "lines" else _lib.OCT_APOD_FRAME_MEAN
+        L = _lib.lib()
+        code = self._code(t)
+        _lib.check(L.oct_background(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size,
+                                    self.apod_group, ws.data_ptr(), self.xp.stream()), "oct_background")
+        ev = self.timing_events
+        if ev is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        _lib.check(L.oct_opspectral(op.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size,
+                                    out.data_ptr(), out_stride, ws.data_ptr(), self.xp.stream()), "oct_opspectral")
         if ev is not None:
             e1 = torch.cuda.Event(enable_timing=True)
             e1.record()

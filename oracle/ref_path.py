@@ -442,3 +442,18 @@ def lsb_agreement(mag_test, mag_ref, clim=None):
     b = db_to_bits(d_ref, clim).astype(np.int64)
     diff = np.abs(a - b)
     return float(np.mean(diff <= 1)), int(diff.max()), clim
+
+
+def operator_fp64(geo: OracleGeometry, z0: int, nzc: int):
+    """Dense float64 operator of the linear spectral map for bins [z0, z0 + nzc):
+    X = A d with d = raw - background, A = F_crop diag(window / N) (S + (1/N)(1 - S1) 1^T),
+    assembled from the dense restatement's sinc rows (octrecon/core/spectral.py:123-134, 151-153)."""
+    N = len(geo.lam)
+    S = np.zeros((N, N))
+    for t in range(geo.idx.shape[1]):
+        np.add.at(S, (np.arange(N), geo.idx[:, t]), geo.w[:, t])
+    u = 1.0 - S.sum(1)
+    Sp = S + np.outer(u, np.ones(N)) / N
+    z = np.arange(z0, z0 + nzc)
+    F = np.exp(2j * np.pi * ((np.outer(z, np.arange(N))) % N) / N) * (geo.window / N)[None, :]
+    return F @ Sp
