@@ -115,6 +115,89 @@ def cpu_baseline(rec, info, seconds: float):
                       f"stitching in {el:.1f}s; os.cpu_count()={os.cpu_count()}"}
 
 
+def time_steps(rec, vol, steps, warmup):
+    """Warm-up, then `steps` timed whole-job steps; returns (seconds, per-launch kernel ms)."""
+    import torch
+    events = []
+    rec.sp.timing_events = events
+    for _ in range(warmup):
+        rec.reconstruct_volume(vol)
+    torch.cuda.synchronize()
+    events.clear()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rec.reconstruct_volume(vol)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    rec.sp.timing_events = None
+    return el, [a.elapsed_time(b) for a, b in events]
+
+
+def copy_calibration(gib: float = 1.0):
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {n * 4} bytes read and written")
+    del a, b
+
+
+def config2(xp, args):
+    """Operator-form precision workload: 200 000 A-scans (200 B-scans x 1000 + 25 background
+    lines), reflectivity over 60 dB, all 1024 depth bins; MFMA form vs plain FP32 timing and
+    agreement with the float64 oracle on a sample of 2 B-scans."""
+    import torch
+
+    from oct_recon_af_v1_amd import synthetic as SY
+    from oct_recon_af_v1_amd.geometry import beta_from_index_units, build_spectral_geometry
+    from oct_recon_af_v1_amd.spectral import HipSpectralProcessor
+    from oracle import ref_path as R
+    spec = SY.SpectrumSpec(r_min=1e-7, r_max=1e-1)
+    lam = spec.lambda_nm()
+    beta = 3e-5
+    bref = beta_from_index_units(beta, lam)
+    sp = HipSpectralProcessor(build_spectral_geometry(lam, bref, 1.33), spec.apod, xp)
+    raw = SY.make_bscans(200, 1000, spec, beta=beta, seed=args.seed + 77, device=xp.device, chunk_files=10)
+    out = torch.empty((200, 1000, 1024), dtype=torch.float32, device=xp.device)
+    res = {}
+    for form in ("mfma", "plain"):
+        ev = []
+        sp.timing_events = ev
+        for it in range(args.warmup + args.steps):
+            if it == args.warmup:
+                torch.cuda.synchronize()
+                ev.clear()
+            if form == "mfma":
+                sp.run_operator(raw, 0, 1024, out=out)
+            else:
+                sp.run(raw, 0, 0, 1024, out=out)
+        torch.cuda.synchronize()
+        sp.timing_events = None
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        res[form] = {"ms": ms, "ascans_per_s": 200000 / (ms * 1e-3)}
+        if form == "mfma":
+            g = out[:2].cpu().numpy()
+    flops = 3 * 2 * 2048 * 2048 * 200000
+    res["mfma"]["tflops"] = flops / (res["mfma"]["ms"] * 1e-3) / 1e12
+    res["mfma"]["frac_of_fp16_dense_peak"] = res["mfma"]["tflops"] / 2500.0
+    h = raw[:2].cpu().numpy()
+    o64 = R.magnitude_fp64(h, spec.apod, R.spectral_geometry(lam, bref, 1.33))
+    frac, worst, _ = R.lsb_agreement(g, o64)
+    with np.errstate(divide="ignore"):
+        ddb = np.abs(20 * np.log10(g.astype(np.float64)) - 20 * np.log10(o64))
+    ddb = ddb[np.isfinite(ddb)]
+    res["mfma"].update({"lsb_within1_vs_fp64": frac, "lsb_max_vs_fp64": worst,
+                        "db_err_median": float(np.median(ddb)), "db_err_p999": float(np.quantile(ddb, 0.999)),
+                        "sample_ascans": int(g.shape[0] * g.shape[1])})
+    res["mfma_vs_plain"] = res["mfma"]["ascans_per_s"] / res["plain"]["ascans_per_s"]
+    del raw, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +209,8 @@ def main():
     ap.add_argument("--spectral-rows", default="needed", choices=["needed", "all"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the operator-form measurements")
+    ap.add_argument("--copy-calibration", action="store_true", help="run a 1 GiB device copy first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: multi-GPU runs are not part of this build "
@@ -136,27 +221,16 @@ def main():
 
     from oct_recon_af_v1_amd.backend import get_xp
     xp, _ = get_xp("hip")
+    if args.copy_calibration:
+        copy_calibration()
     rec, info = make_workload(xp, args)
     vol = rec.allocate_volume()
+    ascans = info["ascans"]
 
-    # per-launch HIP events around the fused spectral kernel, on the stream it runs on
-    spectral_events = []
-    rec.sp.timing_events = spectral_events
-
-    for _ in range(args.warmup):
-        rec.reconstruct_volume(vol)
-    torch.cuda.synchronize()
-    spectral_events.clear()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rec.reconstruct_volume(vol)
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    rec.sp.timing_events = None
-    ms = [a.elapsed_time(b) for a, b in spectral_events]
+    # ---- plain FP32 path (the headline value)
+    el, ms = time_steps(rec, vol, args.steps, args.warmup)
     launches = len(ms)
     avg_ms = float(np.mean(ms))
-    ascans = info["ascans"]
     per_launch = ascans / (launches / args.steps)
     lo, hi = rec.stitchers[(0, 0)].needed_rows() if args.spectral_rows == "needed" else (0, 1024)
     rows = hi - lo
@@ -164,7 +238,7 @@ def main():
     achieved = per_launch * bytes_per_ascan / (avg_ms * 1e-3) / 1e9
     value = ascans * args.steps / el
     clim = vol["clim"].cpu().numpy().tolist()
-    log(f"step {el / args.steps * 1e3:.2f} ms; spectral launch avg {avg_ms:.3f} ms over {launches} launches "
+    log(f"plain: step {el / args.steps * 1e3:.2f} ms; spectral launch avg {avg_ms:.3f} ms over {launches} launches "
         f"({per_launch:.0f} A-scans, {rows} rows); clim {clim}")
     res = {
         "metric": METRIC, "value": value, "unit": "A-scans/s", "n_gpus": 1, "steps": args.steps,
@@ -181,6 +255,37 @@ def main():
                      "bytes_per_ascan": bytes_per_ascan, "ascans_per_launch": per_launch,
                      "avg_launch_ms": avg_ms},
     }
+
+    if not args.no_mfma:
+        # ---- the same workload through the operator form (MFMA, fp16 3-product)
+        bits_plain = vol["bits"].clone()
+        clim_plain = vol["clim"].clone()
+        rec.cfg.form = "mfma"
+        el_m, ms_m = time_steps(rec, vol, args.steps, args.warmup)
+        rec.cfg.form = "plain"
+        ulo, uhi = rec.union_rows()
+        bits_m = rec.fin.quantize(vol["db"], clim_plain)
+        d = (bits_m.to(torch.int32) - bits_plain.to(torch.int32)).abs()
+        within = float((d <= 1).float().mean().item())
+        op = rec.sp.operator_plan(ulo, uhi - ulo)
+        rpad = ((2 * ((uhi - ulo + 15) // 16 * 16) + 127) // 128) * 128
+        per_m = ascans / (len(ms_m) / args.steps)
+        avg_m = float(np.mean(ms_m))
+        tfl = per_m * 3 * 2 * rpad * 2048 / (avg_m * 1e-3) / 1e12
+        res["mfma_form"] = {
+            "value": ascans * args.steps / el_m, "unit": "A-scans/s", "ms_per_step": el_m / args.steps * 1e3,
+            "ratio_vs_plain": (ascans * args.steps / el_m) / value,
+            "voxels_within_1lsb_of_plain": within, "max_lsb_diff_vs_plain": int(d.max().item()),
+            "rows": [ulo, uhi], "operator_scale_log2": op.scale_log2,
+            "roofline": {"bound": "mfma", "achieved": tfl, "peak": 2500.0, "unit": "TFLOP/s", "frac": tfl / 2500.0,
+                         "kernel": "opgemm_kernel<uint16_t>", "flops_per_ascan": 3 * 2 * rpad * 2048,
+                         "avg_launch_ms": avg_m, "ascans_per_launch": per_m,
+                         "hbm_GBps": per_m * (2 * 2048 + 4 * (uhi - ulo)) / (avg_m * 1e-3) / 1e9},
+        }
+        log(f"mfma: step {el_m / args.steps * 1e3:.2f} ms; opgemm avg {avg_m:.3f} ms; within 1 LSB of plain {within:.6f}")
+        del bits_plain
+        res["config2_precision_form"] = config2(xp, args)
+        log(f"config2: {res['config2_precision_form']}")
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(rec, info, args.cpu_seconds)
     print(json.dumps(res), flush=True)

@@ -23,6 +23,9 @@
  *   oct_stitch
  *       TileStitcher.contribute + the accumulation into num/den
  *       (octrecon/core/stitching.py:122-140; octrecon/pipeline.py:282-284)
+ *   oct_stitch_row + oct_volume_clim
+ *       the whole y-tile row loop of Reconstructor.process_row fused: every tile's contribute,
+ *       the num/den accumulation and the finalise (octrecon/pipeline.py:261-299)
  *   oct_finalize
  *       finalise of a y-tile row + plane/global clim (octrecon/pipeline.py:296-300, 376-388;
  *       octrecon/io/tiff_writer.py:48-50)
@@ -104,6 +107,31 @@ typedef struct oct_stitch_desc {
 
 int oct_stitch(const oct_stitch_desc* d, void* stream);
 
+/* ---- fused y-row stitching + finalise: every tile of one y-tile row in one output-owned
+ * pass (num/den accumulate in registers, never in memory), written as dB planes
+ * db[(plane0 + f) * n_out_z * n_out_x + zo * n_out_x + c] with the same per-block finite
+ * min/max partials as oct_finalize (partial: (planes, oct_stitch_row_blocks(), 2) ints).
+ * Equivalent to the per-tile oct_stitch accumulation followed by oct_finalize
+ * (octrecon/pipeline.py:261-299).  Each tile's magnitudes hold all n_frames frames. */
+typedef struct oct_row_tile {
+    const float* mag;         /* (n_frames, n_x, mag_stride): rows [mag_row0, mag_row0 + mag_stride) */
+    long long mag_stride;
+    int mag_row0;
+    int c0, C;                /* output columns [c0, c0 + C) */
+    const int32_t* z_rows;    /* (n_out_z, 2) */
+    const float* z_w;
+    const int32_t* x_cols;    /* (C, 2) */
+    const float* x_w;
+    const float* fz;          /* (n_z) */
+} oct_row_tile;
+int oct_stitch_row_blocks(int n_out_z, int n_out_x);
+int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x, int n_z, const int32_t* frame_idx,
+                   const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, int n_out_z, int n_out_x,
+                   float thr, float* db, int plane0, int* partial, void* stream);
+/* plane clims (n_planes, 2) and the global clim (2) from row partials */
+int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
+                    void* stream);
+
 /* num/den (n_planes, plane_elems) -> db (same layout) with den < thr -> NaN, 20 log10(num/den);
  * with den == NULL, db = 20 log10(num) (a magnitude volume without stitching);
  * plane_clim (n_planes, 2) and clim (2) = finite min / max (NaN if none). */
@@ -118,12 +146,14 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
                  const float* plane_clim, int legacy_double, void* stream);
 
 /* ---- dispersion sharpness search (octrecon/estimation/dispersion.py:46-107, 154-239)
- * One search evaluation is the spectral path with the candidate's window: replace the plan's
- * window (N, 2) unscaled (synchronises `stream`; no launch with this plan may be in flight on
- * another stream), then score magnitudes with the top-fraction metric:
- * out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over x and z >= z_min),
- * M = n_x (n_z - z_min), mag (n_bscans, n_x, n_z) float32, out float64. */
-int oct_plan_set_window(oct_plan* plan, const float* window, void* stream);
+ * One search evaluation is the spectral path with the candidate's window
+ * hann/rms * exp(-i beta (k - k0)^2).  oct_plan_use_window makes later oct_spectral launches
+ * with this plan read `dev_window` ((N, 2) float32 device array, already multiplied by 1/N)
+ * instead of the plan's own window; NULL restores it.  The pointer is captured at each launch,
+ * so a batch of candidates can be queued without host synchronisation.
+ * oct_topfrac_metric: out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over all
+ * x and z >= z_min), M = n_x (n_z - z_min); mag (n_bscans, n_x, n_z) float32, out float64. */
+int oct_plan_use_window(oct_plan* plan, const float* dev_window);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* stream);
 

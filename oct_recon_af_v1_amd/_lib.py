@@ -33,6 +33,12 @@ class StitchDesc(ctypes.Structure):
     ]
 
 
+class RowTile(ctypes.Structure):
+    """Mirror of oct_row_tile."""
+    _fields_ = [("mag", _vp), ("mag_stride", _ll), ("mag_row0", _i), ("c0", _i), ("C", _i),
+                ("z_rows", _vp), ("z_w", _vp), ("x_cols", _vp), ("x_w", _vp), ("fz", _vp)]
+
+
 # name -> (restype, argtypes); every symbol include/oct_hip.h declares
 SIGNATURES = {
     "oct_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp]),
@@ -41,13 +47,17 @@ SIGNATURES = {
     "oct_background": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
     "oct_spectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_stitch": (_i, [ctypes.POINTER(StitchDesc), _vp]),
+    "oct_stitch_row_blocks": (_i, [_i, _i]),
+    "oct_stitch_row": (_i, [ctypes.POINTER(RowTile), _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _f, _vp, _i, _vp,
+                            _vp]),
+    "oct_volume_clim": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     "oct_finalize_workspace_bytes": (_sz, [_i, _ll]),
     "oct_finalize": (_i, [_vp, _vp, _vp, _i, _ll, _f, _vp, _vp, _vp, _sz, _vp]),
     "oct_quantize": (_i, [_vp, _vp, _i, _ll, _vp, _vp, _i, _vp]),
     "oct_opplan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), _vp]),
     "oct_opplan_destroy": (_i, [_vp]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
-    "oct_plan_set_window": (_i, [_vp, _vp, _vp]),
+    "oct_plan_use_window": (_i, [_vp, _vp]),
     "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp]),
     "oct_last_error": (ctypes.c_char_p, []),
 }

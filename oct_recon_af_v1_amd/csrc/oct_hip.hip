@@ -12,6 +12,7 @@
 #include "estimate.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -43,24 +44,28 @@ struct oct_plan {
     int* band_start;
     float* band_w;
     float* win;   // window / N
+    const float* win_override;   // caller's device window / N (dispersion search), or null
+    float2* tw;   // (32, 64) transform twiddles [k1][n2]
+    int waves;    // workgroup shape of the fused kernel: 8 (LDS twiddles) or 4
 };
 
 namespace {
 
-template <typename RawT, int TP>
-int launch_spectral_t(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
-    const long long ngroups = (a.M + oct::kWaves - 1) / oct::kWaves;
-    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus);
+template <typename RawT, int TP, int W>
+int launch_spectral_w(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
+    const long long ngroups = (a.M + W - 1) / W;
+    const int per_cu = 8 / W;
+    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus * per_cu);
     if (grid <= 0) return OCT_OK;
     switch (kind) {
         case OCT_OUT_MAG:
-            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_MAG>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_MAG, W>), dim3(grid), dim3(64 * W), 0, s, a);
             break;
         case OCT_OUT_COMPLEX:
-            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_CPLX>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_CPLX, W>), dim3(grid), dim3(64 * W), 0, s, a);
             break;
         default:
-            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_EQ>), dim3(grid), dim3(oct::kThreads), 0, s, a);
+            hipLaunchKernelGGL((oct::spectral_kernel<RawT, TP, oct::KIND_EQ, W>), dim3(grid), dim3(64 * W), 0, s, a);
             break;
     }
     HIP_TRY(hipGetLastError());
@@ -69,8 +74,12 @@ int launch_spectral_t(const oct_plan* p, const oct::SpectralArgs& a, int kind, h
 
 template <typename RawT>
 int launch_spectral(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
-    if (p->taps == 12) return launch_spectral_t<RawT, 12>(p, a, kind, s);
-    return launch_spectral_t<RawT, 16>(p, a, kind, s);
+    if (p->waves == 4) {
+        if (p->taps == 12) return launch_spectral_w<RawT, 12, 4>(p, a, kind, s);
+        return launch_spectral_w<RawT, 16, 4>(p, a, kind, s);
+    }
+    if (p->taps == 12) return launch_spectral_w<RawT, 12, 8>(p, a, kind, s);
+    return launch_spectral_w<RawT, 16, 8>(p, a, kind, s);
 }
 
 template <typename RawT>
@@ -110,12 +119,23 @@ int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const i
     int cus = 0;
     hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     p->cus = (e == hipSuccess && cus > 0) ? cus : 256;
+    const char* ev = std::getenv("OCT_HIP_SPECTRAL_WAVES");
+    p->waves = (ev && std::atoi(ev) == 4) ? 4 : 8;
+    std::vector<float> twh(2 * 32 * 64);
+    for (int k1 = 0; k1 < 32; ++k1)
+        for (int n2 = 0; n2 < 64; ++n2) {
+            const double ang = 2.0 * M_PI * (double)(n2 * k1) / (double)n_lambda;
+            twh[2 * (k1 * 64 + n2)] = (float)std::cos(ang);
+            twh[2 * (k1 * 64 + n2) + 1] = (float)std::sin(ang);
+        }
     float* win = new float[2 * n_lambda];
     for (int i = 0; i < 2 * n_lambda; ++i) win[i] = window[i] * (1.0f / (float)n_lambda);   // exact: 2^-11
     int rc = OCT_OK;
     if (hipMalloc(&p->band_start, sizeof(int) * n_lambda) != hipSuccess ||
         hipMalloc(&p->band_w, sizeof(float) * n_lambda * taps) != hipSuccess ||
         hipMalloc(&p->win, sizeof(float) * 2 * n_lambda) != hipSuccess ||
+        hipMalloc(&p->tw, sizeof(float) * twh.size()) != hipSuccess ||
+        hipMemcpy(p->tw, twh.data(), sizeof(float) * twh.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->band_start, band_start, sizeof(int) * n_lambda, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->band_w, band_w, sizeof(float) * n_lambda * taps, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->win, win, sizeof(float) * 2 * n_lambda, hipMemcpyHostToDevice) != hipSuccess) {
@@ -139,6 +159,7 @@ int oct_plan_destroy(oct_plan* p) {
     if (p->band_start) hipFree(p->band_start);
     if (p->band_w) hipFree(p->band_w);
     if (p->win) hipFree(p->win);
+    if (p->tw) hipFree(p->tw);
     hipSetDevice(prev);
     delete p;
     return OCT_OK;
@@ -211,7 +232,8 @@ int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files,
     a.bg = bg;
     a.band_start = p->band_start;
     a.band_w = p->band_w;
-    a.win = p->win;
+    a.win = p->win_override ? p->win_override : p->win;
+    a.tw = p->tw;
     a.out = out;
     a.M = (long long)n_files * nX;
     a.lines = lines;
@@ -268,6 +290,69 @@ int oct_stitch(const oct_stitch_desc* d, void* stream) {
     return OCT_OK;
 }
 
+int oct_stitch_row_blocks(int n_out_z, int n_out_x) {
+    if (n_out_z <= 0 || n_out_x <= 0) return 0;
+    const int gx = (n_out_x + oct::kRowTX - 1) / oct::kRowTX;
+    const int gz = (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer);
+    return gx * gz;
+}
+
+int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x, int n_z, const int32_t* frame_idx,
+                   const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, int n_out_z, int n_out_x,
+                   float thr, float* db, int plane0, int* partial, void* stream) {
+    if (!tiles || !frame_idx || !db || !partial) return fail(OCT_EINVAL, "null argument");
+    if (n_tiles < 0 || n_tiles > oct::kRowMaxTiles) return fail(OCT_EINVAL, "%d tiles (at most %d per row)", n_tiles, oct::kRowMaxTiles);
+    if (shift && (!valid_lo || !valid_hi)) return fail(OCT_EINVAL, "shift without valid bounds");
+    if (n_frames <= 0 || n_out_z <= 0 || n_out_x <= 0) return OCT_OK;
+    if (n_x <= 0 || n_z <= 0 || plane0 < 0) return fail(OCT_EINVAL, "bad row shape");
+    oct::RowArgs a;
+    for (int i = 0; i < n_tiles; ++i) {
+        const oct_row_tile& t = tiles[i];
+        if (!t.mag || !t.z_rows || !t.z_w || !t.x_cols || !t.x_w || !t.fz) return fail(OCT_EINVAL, "tile %d: null pointer", i);
+        if (t.c0 < 0 || t.C <= 0 || t.c0 + t.C > n_out_x) return fail(OCT_EINVAL, "tile %d: columns outside the output", i);
+        a.t[i].mag = t.mag;
+        a.t[i].mstride = t.mag_stride;
+        a.t[i].mrow0 = t.mag_row0;
+        a.t[i].c0 = t.c0;
+        a.t[i].C = t.C;
+        a.t[i].zr = t.z_rows;
+        a.t[i].zw = t.z_w;
+        a.t[i].xr = t.x_cols;
+        a.t[i].xw = t.x_w;
+        a.t[i].fz = t.fz;
+    }
+    a.ntiles = n_tiles;
+    a.nX = n_x;
+    a.nZ = n_z;
+    a.frame_idx = frame_idx;
+    a.shift = shift;
+    a.vlo = valid_lo;
+    a.vhi = valid_hi;
+    a.nOutZ = n_out_z;
+    a.nOutX = n_out_x;
+    a.thr = thr;
+    a.db = db;
+    a.plane0 = plane0;
+    a.partial = partial;
+    a.blocks_per_plane = oct_stitch_row_blocks(n_out_z, n_out_x);
+    dim3 grid((n_out_x + oct::kRowTX - 1) / oct::kRowTX,
+              (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer), n_frames);
+    hipLaunchKernelGGL(oct::stitch_row_kernel, grid, dim3(oct::kRowTX, oct::kRowTZ), 0,
+                       reinterpret_cast<hipStream_t>(stream), a);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
+                    void* stream) {
+    if (!partial || !plane_clim || !clim) return fail(OCT_EINVAL, "null argument");
+    if (n_planes <= 0 || blocks_per_plane <= 0) return fail(OCT_EINVAL, "empty volume");
+    hipLaunchKernelGGL(oct::clim_kernel, dim3(1), dim3(oct::kFinThreads), 0, reinterpret_cast<hipStream_t>(stream),
+                       partial, blocks_per_plane, n_planes, plane_clim, clim);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 static int finalize_blocks(long long plane_elems) {
     long long b = (plane_elems + oct::kFinThreads * 8 - 1) / (oct::kFinThreads * 8);
     return (int)std::max<long long>(1, std::min<long long>(b, 256));
@@ -310,13 +395,9 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
 }
 
 // ---------------------------------------------------------------- dispersion search support
-int oct_plan_set_window(oct_plan* p, const float* window, void* stream) {
-    if (!p || !window) return fail(OCT_EINVAL, "null argument");
-    std::vector<float> w(2 * (size_t)p->n);
-    for (int i = 0; i < 2 * p->n; ++i) w[i] = window[i] * (1.0f / (float)p->n);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    HIP_TRY(hipMemcpyAsync(p->win, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
+int oct_plan_use_window(oct_plan* p, const float* dev_window) {
+    if (!p) return fail(OCT_EINVAL, "null plan");
+    p->win_override = dev_window;
     return OCT_OK;
 }
 

@@ -26,10 +26,7 @@ namespace oct {
 
 constexpr int kN = 2048;
 constexpr int kNZ = kN / 2;
-constexpr int kWaves = 8;
-constexpr int kThreads = 64 * kWaves;
 constexpr int kQ = kN / 64;            // 32 values per lane in the transform layout
-constexpr int kQW = kQ / kWaves;       // 4 sinc outputs per lane per A-scan for one wave
 constexpr int kStride = 66;            // padded transpose row (complex elements)
 constexpr int kBufCF = 32 * kStride;   // complex elements per A-scan buffer (>= kN)
 
@@ -41,6 +38,7 @@ struct SpectralArgs {
     const int* band_start;   // (N) first source sample of each output's tap window
     const float* band_w;     // (N, TP) tap weights
     const float* win;        // (N, 2) complex window scaled by 1/N
+    const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), [k1][n2] (W = 4 variant)
     void* out;
     long long M;             // A-scans = B * nX
     int lines;               // A-lines per raw file
@@ -79,16 +77,23 @@ __device__ __forceinline__ void unpack16<float>(const uint4& u, float (&f)[4]) {
     f[3] = __builtin_bit_cast(float, u.w);
 }
 
-template <typename RawT, int TP, int KIND>
-__global__ __launch_bounds__(kThreads, 1) void spectral_kernel(SpectralArgs a) {
+// W = waves per workgroup = A-scans per group.  W = 8: one 148 KB workgroup per CU with the
+// twiddle table in LDS; W = 4: two 68 KB workgroups per CU (one overlaps the other's
+// barriers and loads), twiddles read from an L1-resident [k1][lane] table.
+template <typename RawT, int TP, int KIND, int W>
+__global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a) {
+    constexpr int kWaves = W;
+    constexpr int kThreads = 64 * W;
+    constexpr int kQW = kQ / W;        // sinc outputs per lane per A-scan owned by one wave
+    constexpr bool kLdsTw = (W == 8);
     __shared__ __align__(16) cf buf[kWaves][kBufCF];
-    __shared__ __align__(16) cf tws[kQ][64];     // tws[k1][n2] = exp(+2 pi i n2 k1 / N)
+    __shared__ __align__(16) cf tws[kLdsTw ? kQ : 1][kLdsTw ? 64 : 1];
     __shared__ float lmean[kWaves];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 
-    if constexpr (KIND != KIND_EQ) {
+    if constexpr (KIND != KIND_EQ && kLdsTw) {
         for (int e = threadIdx.x; e < kQ * 64; e += kThreads) {
             const int k1 = e >> 6, n2 = e & 63;
             double s, c;
@@ -209,7 +214,12 @@ __global__ __launch_bounds__(kThreads, 1) void spectral_kernel(SpectralArgs a) {
             dft_inv_dif<kQ>(v);                          // Y[k1] in v[bitrev(k1)]
             static_for<1, kQ>([&](auto k1) {
                 constexpr int r = bitrev(decltype(k1)::value, 5);
-                v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
+                if constexpr (kLdsTw) {
+                    v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
+                } else {
+                    const float2 t = a.tw[decltype(k1)::value * 64 + lane];
+                    v[r] = cmul(v[r], cf{t.x, t.y});
+                }
             });
             wave_lds_sync();
             static_for<0, kQ>([&](auto k1) {
@@ -239,7 +249,7 @@ __global__ __launch_bounds__(kThreads, 1) void spectral_kernel(SpectralArgs a) {
                 if (k >= a.row0 && k < a.row0 + a.rows) {
                     if constexpr (KIND == KIND_MAG) {
                         reinterpret_cast<float*>(a.out)[p * a.out_stride + (k - a.row0)] =
-                            __builtin_sqrtf(o.x * o.x + o.y * o.y);
+                            __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
                     } else {
                         reinterpret_cast<cf*>(a.out)[p * a.out_stride + (k - a.row0)] = o;
                     }

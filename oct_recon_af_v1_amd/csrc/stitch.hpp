@@ -82,6 +82,126 @@ __global__ __launch_bounds__(256) void stitch_kernel(StitchArgs a) {
 }
 
 // Order-preserving map of a float to an int key (for min/max); NaN never reaches it.
+__device__ __forceinline__ int fkey(float f);
+
+// ---------------------------------------------------------------- fused y-row stitch + finalise
+// All tiles of one y-tile row in one pass, output-owned: thread (f, zo, c) accumulates
+//   num = sum_tiles sum_x Wx sum_r Wz fz valid mag,  den = the same without mag
+// in registers over every tile whose columns cover c, then finalises in place:
+// den < thr -> NaN, else 20 log10(num / den), and keeps per-block finite min/max partials
+// (same partial layout as finalize_kernel, so clim_kernel reduces a volume of rows).
+// Equivalent to the per-tile contributions + accumulation + finalise of
+// octrecon/pipeline.py:261-299, without num/den round trips through memory.
+constexpr int kRowMaxTiles = 40;   // keeps the by-value argument block under 4 KB
+struct RowTile {
+    const float* mag;       // (nF, nX, mstride) rows [mrow0, mrow0 + mstride) of the tile's frames
+    long long mstride;
+    int mrow0;
+    int c0, C;              // output columns [c0, c0 + C)
+    const int* zr;          // (nOutZ, 2)
+    const float* zw;
+    const int* xr;          // (C, 2)
+    const float* xw;
+    const float* fz;        // (nZ)
+};
+struct RowArgs {
+    RowTile t[kRowMaxTiles];
+    int ntiles;
+    int nX, nZ;
+    const int* frame_idx;   // (nF) tile-local y frame of each output plane (for the optical path)
+    const int* shift;       // (nY, nX) or null
+    const int* vlo;
+    const int* vhi;
+    int nOutZ, nOutX;
+    float thr;
+    float* db;              // plane p = plane0 + f: db[(p * nOutZ + zo) * nOutX + c]
+    int plane0;
+    int* partial;           // (planes, blocks_per_plane, 2)
+    int blocks_per_plane;
+};
+
+constexpr int kRowTX = 64, kRowTZ = 4, kRowZPer = 4;   // block 64 x 4 threads, 4 rows per thread
+
+__global__ __launch_bounds__(kRowTX * kRowTZ) void stitch_row_kernel(RowArgs a) {
+    const int c = blockIdx.x * kRowTX + threadIdx.x;
+    const int zbase = blockIdx.y * (kRowTZ * kRowZPer) + threadIdx.y;
+    const int f = blockIdx.z;
+    float num[kRowZPer], den[kRowZPer];
+#pragma unroll
+    for (int k = 0; k < kRowZPer; ++k) num[k] = den[k] = 0.f;
+    const int fb = a.frame_idx[f];
+    if (c < a.nOutX) {
+        for (int ti = 0; ti < a.ntiles; ++ti) {
+            const RowTile& t = a.t[ti];
+            const int cl = c - t.c0;
+            if (cl < 0 || cl >= t.C) continue;
+#pragma unroll
+            for (int tx = 0; tx < 2; ++tx) {
+                const float wx = t.xw[2 * cl + tx];
+                if (wx == 0.f) continue;
+                const int x = t.xr[2 * cl + tx];
+                int sh = 0, lo = 0, hi = a.nZ - 1;
+                if (a.shift) {
+                    const long long o = (long long)fb * a.nX + x;
+                    sh = a.shift[o];
+                    lo = a.vlo[o];
+                    hi = a.vhi[o];
+                }
+                const float* mrow = t.mag + ((long long)f * a.nX + x) * t.mstride - t.mrow0;
+#pragma unroll
+                for (int k = 0; k < kRowZPer; ++k) {
+                    const int zo = zbase + kRowTZ * k;
+                    if (zo >= a.nOutZ) continue;
+                    float sa = 0.f, sf = 0.f;
+#pragma unroll
+                    for (int tz = 0; tz < 2; ++tz) {
+                        const float wz = t.zw[2 * zo + tz];
+                        if (wz == 0.f) continue;
+                        const int r = t.zr[2 * zo + tz];
+                        const float fw = (r >= lo && r <= hi) ? t.fz[r] : 0.f;
+                        const int src = min(max(r + sh, 0), a.nZ - 1);
+                        sa = __builtin_fmaf(mrow[src] * fw, wz, sa);
+                        sf = __builtin_fmaf(fw, wz, sf);
+                    }
+                    num[k] = __builtin_fmaf(sa, wx, num[k]);
+                    den[k] = __builtin_fmaf(sf, wx, den[k]);
+                }
+            }
+        }
+    }
+    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+    if (c < a.nOutX) {
+#pragma unroll
+        for (int k = 0; k < kRowZPer; ++k) {
+            const int zo = zbase + kRowTZ * k;
+            if (zo >= a.nOutZ) continue;
+            const float v = (den[k] >= a.thr) ? 20.0f * log10f(num[k] / den[k]) : __builtin_nanf("");
+            a.db[((long long)(a.plane0 + f) * a.nOutZ + zo) * a.nOutX + c] = v;
+            if (__builtin_isfinite(v)) {
+                const int kk = fkey(v);
+                kmin = min(kmin, kk);
+                kmax = max(kmax, kk);
+            }
+        }
+    }
+    __shared__ int smin[kRowTX * kRowTZ], smax[kRowTX * kRowTZ];
+    const int tid = threadIdx.y * kRowTX + threadIdx.x;
+    smin[tid] = kmin;
+    smax[tid] = kmax;
+    __syncthreads();
+    for (int s = kRowTX * kRowTZ / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            smin[tid] = min(smin[tid], smin[tid + s]);
+            smax[tid] = max(smax[tid], smax[tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + blockIdx.y * gridDim.x + blockIdx.x) * 2;
+        a.partial[o] = smin[0];
+        a.partial[o + 1] = smax[0];
+    }
+}
 __device__ __forceinline__ int fkey(float f) {
     int i = __builtin_bit_cast(int, f);
     return i >= 0 ? i : (i ^ 0x7FFFFFFF);

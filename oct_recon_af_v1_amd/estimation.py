@@ -1,0 +1,196 @@
+"""Dispersion (beta) sharpness search and focus detection on the hip device.
+
+Reference: octrecon/estimation/dispersion.py (DispersionScorer :66-107, _metric :46-63,
+estimate_dispersion :154-267) and octrecon/estimation/focus.py (imgaussfilt_1d :44-52,
+findpeaks_matlab :55-72, detect_peak_near :191-206, interface_pix :209-218).
+
+A search evaluation is the spectral path with the candidate window hann/rms * exp(-i beta dk^2)
+(the reference's DispersionScorer window equals the reconstruction window of that beta:
+tests/test_estimation.py:98-106), so the device scorer runs the fused spectral kernel with a
+per-candidate window and the exact top-1 % metric kernel.  The grid of candidates is queued
+without host synchronisation; the bounded refinement (scipy minimize_scalar, as the reference)
+evaluates one candidate per call.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib
+from .geometry import SpectralGeometry, build_spectral_geometry
+
+Z_MIN_PIX = 20
+TOP_FRACTION = 0.01
+
+
+class HipDispersionScorer:
+    """Scores beta candidates on a fixed set of raw B-scans (B, lines, N) on the device."""
+
+    def __init__(self, lambda_nm, raw, apod_size: int, xp, n_medium: float = 1.33, apod_mode: str = "lines",
+                 metric: str = "top_percentile"):
+        if metric not in ("top_percentile", "auto"):
+            raise NotImplementedError("the device scorer implements the reference's default metric "
+                                      "'top_percentile' only")
+        from .spectral import HipSpectralProcessor
+        self.xp = xp
+        self.torch = xp.torch
+        self.sg0: SpectralGeometry = build_spectral_geometry(lambda_nm, 0.0, n_medium)
+        self.sp = HipSpectralProcessor(self.sg0, apod_size, xp, apod_mode=apod_mode)
+        self.raw = self.sp._device_raw(raw)
+        N = self.sg0.n
+        hann = 0.5 * (1 - np.cos(2 * np.pi * np.arange(N) / (N - 1)))
+        self.hann = hann / np.sqrt(np.mean(hann ** 2))
+        k_eq = 2 * np.pi / self.sg0.lambda_eq_nm
+        self.dk2 = (k_eq - k_eq.mean()) ** 2
+        self.B = self.raw.shape[0]
+        self.nX = self.sp.n_scan_lines(self.raw.shape[1])
+        self.nz = N // 2
+        self._mag = self.torch.empty((self.B, self.nX, self.nz), dtype=self.torch.float32, device=xp.device)
+        self.n_evals = 0
+
+    def windows(self, betas):
+        """(len(betas), N, 2) float32 windows scaled by 1/N (float64 phase)."""
+        betas = np.atleast_1d(np.asarray(betas, float))
+        w = self.hann[None, :] * np.exp(-1j * betas[:, None] * self.dk2[None, :]) / self.sg0.n
+        return np.ascontiguousarray(np.stack([w.real, w.imag], -1).astype(np.float32))
+
+    def scores_many(self, betas):
+        """(G, B) float64 scores for a batch of candidates, one host synchronisation."""
+        torch = self.torch
+        betas = np.atleast_1d(np.asarray(betas, float))
+        win = torch.from_numpy(self.windows(betas)).to(self.xp.device)
+        out = torch.empty((len(betas), self.B), dtype=torch.float64, device=self.xp.device)
+        L = _lib.lib()
+        try:
+            for g in range(len(betas)):
+                _lib.check(L.oct_plan_use_window(self.sp.plan.handle, win[g].data_ptr()), "oct_plan_use_window")
+                self.sp.run(self.raw, _lib.OCT_OUT_MAG, 0, self.nz, out=self._mag)
+                _lib.check(L.oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX,
+                                                TOP_FRACTION, out[g].data_ptr(), self.xp.stream()),
+                           "oct_topfrac_metric")
+        finally:
+            L.oct_plan_use_window(self.sp.plan.handle, None)
+        self.n_evals += len(betas)
+        res = out.cpu().numpy()          # synchronises; keeps `win` alive until the launches ran
+        del win
+        return res
+
+    def scores(self, beta, idx=None):
+        s = self.scores_many([beta])[0]
+        return s if idx is None else s[idx:idx + 1]
+
+
+def estimate_dispersion(scorer: HipDispersionScorer, initial: float, bounds=None, n_grid: int = 41, log=None):
+    """Coarse grid over [0.5, 1.5] * initial (or `bounds`), per-B-scan normalisation of the
+    score curves, bounded refinement of the mean normalised score and of each B-scan's own
+    curve (octrecon/estimation/dispersion.py:206-239)."""
+    from scipy.optimize import minimize_scalar
+    log = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    if bounds is None:
+        lo, hi = sorted((0.5 * initial, 1.5 * initial)) if initial != 0 else (-3e8, 3e8)
+    else:
+        lo, hi = sorted(float(b) for b in bounds)
+    grid = np.linspace(lo, hi, int(n_grid))
+    S = scorer.scores_many(grid)                        # (G, B)
+    smin, smax = S.min(0), S.max(0)
+    rngs = np.where(smax > smin, smax - smin, 1.0)
+    agg = ((S - smin) / rngs).mean(1)
+    gi = int(np.argmax(agg))
+
+    def bracket(i):
+        return grid[max(i - 1, 0)], grid[min(i + 1, len(grid) - 1)]
+
+    tol = max(abs(initial), abs(lo), abs(hi)) * 2e-5
+    a, b = bracket(gi)
+    r = minimize_scalar(lambda x: -float(np.mean((scorer.scores(x) - smin) / rngs)), bounds=(a, b),
+                        method="bounded", options={"xatol": tol})
+    value = float(r.x) if -r.fun >= agg[gi] else float(grid[gi])
+    per = []
+    for j in range(S.shape[1]):
+        i = int(np.argmax(S[:, j]))
+        a, b = bracket(i)
+        rj = minimize_scalar(lambda x: -float(scorer.scores(x, j)[0]), bounds=(a, b), method="bounded",
+                             options={"xatol": tol})
+        per.append(float(rj.x) if -rj.fun >= S[i, j] else float(grid[i]))
+    per = np.array(per)
+    el = time.perf_counter() - t0
+    at_bound = bool(value <= lo + (hi - lo) / (n_grid - 1) or value >= hi - (hi - lo) / (n_grid - 1))
+    log(f"dispersion estimate {value:.5g} ({scorer.n_evals} evaluations, {el * 1e3:.1f} ms)")
+    return {"value": value, "bounds": [lo, hi], "candidates": grid.tolist(), "score_curve": agg.tolist(),
+            "per_bscan_values": per.tolist(), "median_per_bscan": float(np.median(per)),
+            "spread_mad": float(1.4826 * np.median(np.abs(per - np.median(per)))), "at_bound": at_bound,
+            "n_evaluations": scorer.n_evals, "elapsed_s": el}
+
+
+# --------------------------------------------------------------------------- focus (host, 1-D)
+def matlab_round(x):
+    return np.floor(np.abs(x) + 0.5) * np.sign(x)
+
+
+def imgaussfilt_1d(v, sigma=4.0):
+    """imgaussfilt for a vector: kernel 2*ceil(2 sigma)+1 taps, replicate padding."""
+    v = np.asarray(v, float).ravel()
+    h = int(np.ceil(2 * sigma))
+    x = np.arange(-h, h + 1)
+    k = np.exp(-x ** 2 / (2 * sigma ** 2))
+    k /= k.sum()
+    p = np.concatenate([np.full(h, v[0]), v, np.full(h, v[-1])])
+    return np.convolve(p, k, mode="valid")
+
+
+def findpeaks_matlab(v):
+    """0-based indices of strict local maxima (flat peaks at their first sample, no endpoints)."""
+    v = np.asarray(v, float)
+    n = len(v)
+    out = []
+    i = 1
+    while i < n - 1:
+        if v[i] > v[i - 1]:
+            j = i
+            while j < n - 1 and v[j + 1] == v[i]:
+                j += 1
+            if j < n - 1 and v[j + 1] < v[i]:
+                out.append(i)
+            i = j + 1
+        else:
+            i += 1
+    return np.asarray(out, int)
+
+
+def detect_peak_near(profile, guess_pix, search_px=60, baseline_sigma=30.0, smooth_sigma=4.0):
+    """Confocal peak of a depth profile nearest a 1-based guess: dB, detrend (smooth minus
+    baseline), strict local maxima within +-search_px, the most prominent.  (pix or nan, dB)."""
+    prof = np.asarray(profile, float)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lp = 10 * np.log10(np.where(np.isfinite(prof) & (prof > 0), prof, np.nan))
+    ok = np.isfinite(lp)
+    if ok.sum() < 10:
+        return np.nan, 0.0
+    lp = np.interp(np.arange(len(lp)), np.nonzero(ok)[0], lp[ok])
+    detr = imgaussfilt_1d(lp, smooth_sigma) - imgaussfilt_1d(lp, baseline_sigma)
+    p = findpeaks_matlab(detr) + 1
+    p = p[np.abs(p - guess_pix) <= search_px]
+    if p.size == 0:
+        return np.nan, 0.0
+    best = p[np.argmax(detr[p - 1])]
+    return int(best), float(detr[best - 1])
+
+
+def depth_profile(mag):
+    """mean over frames of the median over x of |A|: (F, nX, nZ) -> (nZ,)."""
+    m = np.asarray(mag, float)
+    return np.nanmean(np.nanmedian(m, axis=1), axis=0)
+
+
+def interface_pix(mag):
+    """Brightest pixel of the x-median profile below the top third, median over frames (1-based)."""
+    m = np.asarray(mag, float)
+    nz = m.shape[-1]
+    vals = []
+    for f in m:
+        med = np.nanmedian(f, axis=0)
+        med[: int(matlab_round(nz / 3))] = np.nan
+        vals.append(int(np.nanargmax(med)) + 1)
+    return int(matlab_round(np.median(vals))), vals

@@ -14,8 +14,9 @@ finalise pass yields the global clim, and one quantise pass the uint16 codes.
 """
 from __future__ import annotations
 
+import ctypes
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -24,7 +25,7 @@ from .backend import get_xp, synchronize
 from .geometry import ScanGrid, build_spectral_geometry, build_tiled_geometry
 from .quantize import DeviceFinalizer
 from .spectral import HipSpectralProcessor
-from .stitching import HipTileStitcher, OpticalPathCorrection, TileStitcher
+from .stitching import HipTileStitcher, OpticalPathCorrection, TileStitcher, min_weight_threshold
 
 
 @dataclass
@@ -43,6 +44,7 @@ class ReconConfig:
     legacy_double_quantization: bool = False
     spectral_rows: str = "needed"           # "needed": only the depth rows stitching reads; "all"
     form: str = "plain"                     # "plain": fused FP32 kernel; "mfma": fp16 3-product operator GEMM
+    fused_rows: bool = True                 # one stitch+finalise pass per y row (False: per-tile contribute)
 
 
 class Cancelled(Exception):
@@ -83,6 +85,9 @@ class Reconstructor:
                 self.stitchers[(xi, zi)] = HipTileStitcher(base, self.xp, self.n_out[1])
         self.fin = DeviceFinalizer(self.xp)
         self._mag = None
+        self._tile_mag = {}
+        self._partial = None
+        self._keep = None
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
 
     # ------------------------------------------------------------------ data access
@@ -139,11 +144,113 @@ class Reconstructor:
                     mag = self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
                 st.contribute_into(mag, fr_dev[s:s + len(fr)], num, den, b0=b_off + s, mag_row0=lo)
 
+    # ------------------------------------------------------------------ fused row path
+    def _tile_rows(self, st):
+        if self.cfg.form == "mfma":
+            return self.union_rows()
+        if self.cfg.spectral_rows == "needed":
+            return st.needed_rows()
+        return 0, self.sp.nz
+
+    def _row_magnitudes(self, yi, frames, cancel=None):
+        """Spectral stage for every tile of row yi into per-tile buffers (nF, nX, rows)."""
+        cfg = self.cfg
+        frames = np.asarray(frames)
+        nF = len(frames)
+        B = max(1, int(cfg.batch_frames))
+        out = []
+        for (xi, zi) in self.scan.tiles_in_row(yi):
+            st = self.stitchers[(xi, zi)]
+            if st.empty:
+                continue
+            lo, hi = self._tile_rows(st)
+            key = (xi, zi)
+            buf = self._tile_mag.get(key)
+            nX = None
+            for s in range(0, nF, B):
+                if cancel is not None and cancel.is_set():
+                    raise Cancelled()
+                fr = frames[s:s + B]
+                raw = self.tile_frames(xi, yi, zi, fr)
+                nX = self.sp.n_scan_lines(raw.shape[1])
+                shape = (nF, nX, hi - lo)
+                if buf is None or tuple(buf.shape) != shape:
+                    buf = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
+                    self._tile_mag[key] = buf
+                dst = buf[s:s + len(fr)]
+                if cfg.form == "mfma":
+                    self.sp.run_operator(raw, lo, hi - lo, out=dst)
+                else:
+                    self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=dst)
+            out.append((st, buf, lo))
+        return out
+
+    def _stitch_row(self, tiles, frames, db, plane0, partial):
+        """One fused stitch + finalise launch for a row (oct_stitch_row)."""
+        _, nz_out, nx_out = self.n_out
+        if len(tiles) > 40:
+            raise ValueError("more than 40 tiles in one y row")
+        arr = (_lib.RowTile * max(1, len(tiles)))()
+        nX = None
+        for i, (st, buf, lo) in enumerate(tiles):
+            nX = buf.shape[1]
+            if nX < int(st.base.x_cols.max()) + 1:
+                raise ValueError("magnitude has fewer A-scans than the tile grid")
+            nlo, nhi = st.needed_rows()
+            if lo > nlo or lo + buf.shape[2] < nhi:
+                raise ValueError("tile magnitude rows do not cover the stitcher's rows")
+            t = arr[i]
+            t.mag = buf.data_ptr()
+            t.mag_stride = buf.shape[2]
+            t.mag_row0 = lo
+            t.c0 = st.c0
+            t.C = st.c1 - st.c0
+            t.z_rows = st.z_rows.data_ptr()
+            t.z_w = st.z_w.data_ptr()
+            t.x_cols = st.x_cols.data_ptr()
+            t.x_w = st.x_w.data_ptr()
+            t.fz = st.fz.data_ptr()
+        fr = self.torch.as_tensor(np.asarray(frames, np.int32), device=self.xp.device)
+        st0 = tiles[0][0] if tiles else None
+        shift = vlo = vhi = None
+        if st0 is not None and st0.shift is not None:
+            if nX != st0.n_x_opc:
+                raise ValueError("optical-path correction grid does not match the tile's A-scans")
+            if int(fr.min()) < 0 or int(fr.max()) >= st0.n_y:
+                raise ValueError("frame index outside the optical-path correction grid")
+            shift, vlo, vhi = st0.shift.data_ptr(), st0.vlo.data_ptr(), st0.vhi.data_ptr()
+        thr = float(np.float32(min_weight_threshold(np.float32)))
+        _lib.check(_lib.lib().oct_stitch_row(arr, len(tiles), len(frames), nX or 1, self.sp.nz, fr.data_ptr(),
+                                             shift, vlo, vhi, nz_out, nx_out, thr, db.data_ptr(), plane0,
+                                             partial.data_ptr(), self.xp.stream()), "oct_stitch_row")
+        self._keep = (arr, fr)
+
+    def _row_partials(self, n_planes):
+        _, nz_out, nx_out = self.n_out
+        nb = _lib.lib().oct_stitch_row_blocks(nz_out, nx_out)
+        need = n_planes * nb * 2
+        if self._partial is None or self._partial.numel() < need:
+            self._partial = self.torch.empty(need, dtype=self.torch.int32, device=self.xp.device)
+        return self._partial, nb
+
+    def _clim(self, partial, n_planes, nb):
+        pcl = self.torch.empty((n_planes, 2), dtype=self.torch.float32, device=self.xp.device)
+        cl = self.torch.empty(2, dtype=self.torch.float32, device=self.xp.device)
+        _lib.check(_lib.lib().oct_volume_clim(partial.data_ptr(), n_planes, nb, pcl.data_ptr(), cl.data_ptr(),
+                                              self.xp.stream()), "oct_volume_clim")
+        return pcl, cl
+
     def process_row(self, yi: int, frames=None, cancel=None):
         """Reconstruct the output planes of y-tile row `yi` (tile-local frames, default all).
         Returns (db, plane_clim) device tensors: (nF, nOutZ, nOutX) float32 and (nF, 2)."""
         frames = np.arange(self.scan.n_y_px) if frames is None else np.asarray(frames)
         _, nz_out, nx_out = self.n_out
+        if self.cfg.fused_rows:
+            db = self.xp.empty((len(frames), nz_out, nx_out))
+            partial, nb = self._row_partials(len(frames))
+            self._stitch_row(self._row_magnitudes(yi, frames, cancel), frames, db, 0, partial)
+            pcl, _ = self._clim(partial, len(frames), nb)
+            return db, pcl
         num = self.xp.zeros((len(frames), nz_out, nx_out))
         den = self.xp.zeros((len(frames), nz_out, nx_out))
         self._accumulate_row(yi, frames, num, den, cancel=cancel)
@@ -168,7 +275,7 @@ class Reconstructor:
         ny = self.scan.n_y_px
         _, nz_out, nx_out = self.n_out
         shape = (len(rows) * ny, nz_out, nx_out)
-        vol = {k: self.xp.zeros(shape) for k in ("num", "den")}
+        vol = {k: (self.xp.zeros(shape) if not self.cfg.fused_rows else None) for k in ("num", "den")}
         vol["db"] = self.xp.empty(shape)
         vol["bits"] = self.torch.empty(shape, dtype=self.torch.uint16, device=self.xp.device)
         vol["rows"] = rows
@@ -178,11 +285,20 @@ class Reconstructor:
         """All requested y-tile rows into one device volume: dB, plane clims, global clim and
         uint16 codes (single finalise and quantise passes).  No host synchronisation."""
         ny = self.scan.n_y_px
-        vol["num"].zero_()
-        vol["den"].zero_()
-        for k, r in enumerate(vol["rows"]):
-            self._accumulate_row(r, np.arange(ny), vol["num"], vol["den"], b_off=k * ny, cancel=cancel)
-        db, pcl, cl = self.fin.finalize(vol["num"], vol["den"], db=vol["db"])
+        if self.cfg.fused_rows:
+            n_planes = len(vol["rows"]) * ny
+            partial, nb = self._row_partials(n_planes)
+            frames = np.arange(ny)
+            for k, r in enumerate(vol["rows"]):
+                self._stitch_row(self._row_magnitudes(r, frames, cancel), frames, vol["db"], k * ny, partial)
+            pcl, cl = self._clim(partial, n_planes, nb)
+            db = vol["db"]
+        else:
+            vol["num"].zero_()
+            vol["den"].zero_()
+            for k, r in enumerate(vol["rows"]):
+                self._accumulate_row(r, np.arange(ny), vol["num"], vol["den"], b_off=k * ny, cancel=cancel)
+            db, pcl, cl = self.fin.finalize(vol["num"], vol["den"], db=vol["db"])
         self.fin.quantize(db, cl, pcl, legacy_double=self.cfg.legacy_double_quantization, out=vol["bits"])
         vol["plane_clim"], vol["clim"] = pcl, cl
         return vol
