@@ -33,6 +33,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(name: str):
+    """HBM bytes per A-scan measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated) for
+    this kernel on this workload, as recorded in profiles/<name>.md; None if absent."""
+    p = ROOT / "profiles" / f"{name}.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.strip().startswith("traffic_bytes_per_ascan:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def make_workload(xp, args):
     """Config: returns (Reconstructor, info)."""
     import torch
@@ -240,6 +252,7 @@ def main():
     clim = vol["clim"].cpu().numpy().tolist()
     log(f"plain: step {el / args.steps * 1e3:.2f} ms; spectral launch avg {avg_ms:.3f} ms over {launches} launches "
         f"({per_launch:.0f} A-scans, {rows} rows); clim {clim}")
+    tpa = profiled_traffic("spectral_kernel_tiled_fp32") if args.spectral_rows == "needed" else None
     res = {
         "metric": METRIC, "value": value, "unit": "A-scans/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
@@ -250,8 +263,12 @@ def main():
                                "depth 0-0.15 mm; spectral + stitch + finalise + quantise",
                    "ascans_per_step": ascans, "output_yzx": list(rec.n_out), "spectral_rows": [lo, hi]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "spectral_kernel<uint16_t,12,MAG>",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tpa * per_launch if tpa else None,
+                     "traffic_source": "profiles/spectral_kernel_tiled_fp32.md (rocprofv3 FETCH_SIZE x2.0 + "
+                                       "WRITE_SIZE x1.0, copy-calibrated)" if tpa else None,
+                     "kernel": "spectral_kernel<uint16_t,12,MAG,8>",
+                     "bound_note": "VALU-issue bound (non-packed FP32): the HBM fraction is reported as asked",
                      "bytes_per_ascan": bytes_per_ascan, "ascans_per_launch": per_launch,
                      "avg_launch_ms": avg_ms},
     }

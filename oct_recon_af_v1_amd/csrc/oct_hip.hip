@@ -335,10 +335,22 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
     a.plane0 = plane0;
     a.partial = partial;
     a.blocks_per_plane = oct_stitch_row_blocks(n_out_z, n_out_x);
-    dim3 grid((n_out_x + oct::kRowTX - 1) / oct::kRowTX,
-              (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer), n_frames);
-    hipLaunchKernelGGL(oct::stitch_row_kernel, grid, dim3(oct::kRowTX, oct::kRowTZ), 0,
-                       reinterpret_cast<hipStream_t>(stream), a);
+    // LDS-staged variant for narrow depth windows (OCT_HIP_ROW_STAGED=1; default: direct reads)
+    const char* ev = std::getenv("OCT_HIP_ROW_STAGED");
+    bool staged = ev && std::atoi(ev) == 1 && n_out_z <= oct::kStageZG * oct::kStageZPer;
+    for (int i = 0; i < n_tiles && staged; ++i) {
+        const oct_row_tile& t = tiles[i];
+        if (t.mag_stride > oct::kStageRows) staged = false;     // stage narrow depth windows only
+    }
+    const int gx = (n_out_x + oct::kRowTX - 1) / oct::kRowTX;
+    if (staged) {
+        hipLaunchKernelGGL(oct::stitch_row_staged_kernel, dim3(gx, n_frames), dim3(oct::kRowTX, oct::kStageZG), 0,
+                           reinterpret_cast<hipStream_t>(stream), a);
+    } else {
+        dim3 grid(gx, (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer), n_frames);
+        hipLaunchKernelGGL(oct::stitch_row_kernel, grid, dim3(oct::kRowTX, oct::kRowTZ), 0,
+                           reinterpret_cast<hipStream_t>(stream), a);
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

@@ -113,25 +113,6 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         const long long p = p0 + wave;
         float* D = reinterpret_cast<float*>(&buf[wave][0]);
 
-        // this wave's sinc taps and band starts (L2-resident tables, reloaded per group so
-        // that they do not occupy registers during the transform)
-        int bs[kQW];
-        float wt[kQW][TP];
-#pragma unroll
-        for (int qq = 0; qq < kQW; ++qq) {
-            const int i = lane + 64 * (wave * kQW + qq);
-            bs[qq] = a.band_start[i];
-            const float4* wp = reinterpret_cast<const float4*>(a.band_w + (long long)i * TP);
-#pragma unroll
-            for (int t4 = 0; t4 < TP / 4; ++t4) {
-                const float4 v = wp[t4];
-                wt[qq][4 * t4] = v.x;
-                wt[qq][4 * t4 + 1] = v.y;
-                wt[qq][4 * t4 + 2] = v.z;
-                wt[qq][4 * t4 + 3] = v.w;
-            }
-        }
-
         // ---- A: load, background, mean, d - m -> LDS
         if (wave < nvalid) {
             const long long b = p / a.nX;
@@ -173,25 +154,46 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         }
         __syncthreads();
 
-        // ---- B: banded sinc for all A-scans of the group, this wave's output samples;
-        //      the interferogram goes to the second half of each A-scan's LDS region
+        // ---- B: banded sinc for all A-scans of the group, this wave's output samples, in
+        //      chunks of 4 samples per lane whose taps and band starts sit in registers (each
+        //      table entry is read once per group); the interferogram goes to the second half
+        //      of each A-scan's LDS region
 #pragma unroll 1
-        for (int aa = 0; aa < nvalid; ++aa) {
-            const float* Da = reinterpret_cast<const float*>(&buf[aa][0]);
-            float* Ea = reinterpret_cast<float*>(&buf[aa][0]) + kN;
-            const float ma = lmean[aa];
+        for (int qc = 0; qc < kQW; qc += 4) {
+            int bs[4];
+            float wt[4][TP];
 #pragma unroll
-            for (int qq = 0; qq < kQW; ++qq) {
-                const int i = lane + 64 * (wave * kQW + qq);
-                const float* src = Da + bs[qq];
-                float acc = 0.f;
+            for (int qq = 0; qq < 4; ++qq) {
+                const int i = lane + 64 * (wave * kQW + qc + qq);
+                bs[qq] = a.band_start[i];
+                const float4* wp = reinterpret_cast<const float4*>(a.band_w + (long long)i * TP);
 #pragma unroll
-                for (int t = 0; t < TP; ++t) acc = __builtin_fmaf(wt[qq][t], src[t], acc);
-                const float v = acc + ma;
-                if constexpr (KIND == KIND_EQ) {
-                    reinterpret_cast<float*>(a.out)[(p0 + aa) * a.out_stride + i] = v;
-                } else {
-                    Ea[i] = v;
+                for (int t4 = 0; t4 < TP / 4; ++t4) {
+                    const float4 v = wp[t4];
+                    wt[qq][4 * t4] = v.x;
+                    wt[qq][4 * t4 + 1] = v.y;
+                    wt[qq][4 * t4 + 2] = v.z;
+                    wt[qq][4 * t4 + 3] = v.w;
+                }
+            }
+#pragma unroll 1
+            for (int aa = 0; aa < nvalid; ++aa) {
+                const float* Da = reinterpret_cast<const float*>(&buf[aa][0]);
+                float* Ea = reinterpret_cast<float*>(&buf[aa][0]) + kN;
+                const float ma = lmean[aa];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const int i = lane + 64 * (wave * kQW + qc + qq);
+                    const float* src = Da + bs[qq];
+                    float acc = 0.f;
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) acc = __builtin_fmaf(wt[qq][t], src[t], acc);
+                    const float v = acc + ma;
+                    if constexpr (KIND == KIND_EQ) {
+                        reinterpret_cast<float*>(a.out)[(p0 + aa) * a.out_stride + i] = v;
+                    } else {
+                        Ea[i] = v;
+                    }
                 }
             }
         }

@@ -295,6 +295,120 @@ __global__ __launch_bounds__(kFinThreads) void clim_kernel(const int* __restrict
     }
 }
 
+// Staged variant of stitch_row_kernel for narrow depth windows (every tile's magnitude stride
+// <= kStageRows, nOutZ <= kStageZG * kStageZPer): one block owns 64 output columns and ALL
+// output rows of one plane; for each tile covering those columns, the tile's contiguous
+// magnitude slice (the ~65 A-scans behind the columns x its depth rows) is loaded into LDS with
+// coalesced loads, and every output row is formed from LDS.  Same arithmetic per voxel as
+// stitch_row_kernel.
+constexpr int kStageRows = 64, kStageX = 80, kStageZG = 4, kStageZPer = 40;
+// staged rows are padded by one float so that lanes reading consecutive A-scans hit distinct banks
+
+__global__ __launch_bounds__(kRowTX * kStageZG) void stitch_row_staged_kernel(RowArgs a) {
+    __shared__ float slice[kStageX * (kStageRows + 1)];
+    __shared__ int smin[kRowTX * kStageZG], smax[kRowTX * kStageZG];
+    const int cb0 = blockIdx.x * kRowTX;
+    const int c = cb0 + threadIdx.x;
+    const int zg = threadIdx.y;
+    const int f = blockIdx.y;
+    const int tid = threadIdx.y * kRowTX + threadIdx.x;
+    float num[kStageZPer], den[kStageZPer];
+#pragma unroll
+    for (int k = 0; k < kStageZPer; ++k) num[k] = den[k] = 0.f;
+    const int fb = a.frame_idx[f];
+    for (int ti = 0; ti < a.ntiles; ++ti) {
+        const RowTile& t = a.t[ti];
+        const int clo = max(cb0, t.c0), chi = min(cb0 + kRowTX, t.c0 + t.C);   // block columns in the tile
+        if (clo >= chi) continue;                                           // uniform per block
+        const int xlo = t.xr[2 * (clo - t.c0)];
+        const int xhi = t.xr[2 * (chi - 1 - t.c0) + 1];
+        const int nxs = min(xhi, a.nX - 1) - xlo + 1;
+        const int rows = (int)t.mstride;
+        // stage nxs A-scans x rows (contiguous in memory) when they fit; otherwise read directly
+        const bool staged = nxs <= kStageX && rows <= kStageRows;      // uniform per block
+        const float* src = t.mag + ((long long)f * a.nX + xlo) * t.mstride;
+        const int prow = rows + 1;
+        if (staged)
+            for (int e = tid; e < nxs * rows; e += kRowTX * kStageZG) {
+                const int xi = e / rows;
+                slice[xi * prow + (e - xi * rows)] = src[e];
+            }
+        __syncthreads();
+        const int cl = c - t.c0;
+        if (c < a.nOutX && cl >= 0 && cl < t.C) {
+#pragma unroll
+            for (int tx = 0; tx < 2; ++tx) {
+                const float wx = t.xw[2 * cl + tx];
+                if (wx == 0.f) continue;
+                const int x = t.xr[2 * cl + tx];
+                int sh = 0, lo = 0, hi = a.nZ - 1;
+                if (a.shift) {
+                    const long long o = (long long)fb * a.nX + x;
+                    sh = a.shift[o];
+                    lo = a.vlo[o];
+                    hi = a.vhi[o];
+                }
+                const float* mrow = (staged ? slice + (x - xlo) * prow : src + (long long)(x - xlo) * rows) - t.mrow0;
+#pragma unroll
+                for (int k = 0; k < kStageZPer; ++k) {
+                    const int zo = zg + kStageZG * k;
+                    if (zo >= a.nOutZ) break;
+                    float sa = 0.f, sf = 0.f;
+#pragma unroll
+                    for (int tz = 0; tz < 2; ++tz) {
+                        const float wz = t.zw[2 * zo + tz];
+                        if (wz == 0.f) continue;
+                        const int r = t.zr[2 * zo + tz];
+                        const float fw = (r >= lo && r <= hi) ? t.fz[r] : 0.f;
+                        const int srow = min(max(r + sh, 0), a.nZ - 1);
+                        sa = __builtin_fmaf(mrow[srow] * fw, wz, sa);
+                        sf = __builtin_fmaf(fw, wz, sf);
+                    }
+                    num[k] = __builtin_fmaf(sa, wx, num[k]);
+                    den[k] = __builtin_fmaf(sf, wx, den[k]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+    if (c < a.nOutX) {
+#pragma unroll
+        for (int k = 0; k < kStageZPer; ++k) {
+            const int zo = zg + kStageZG * k;
+            if (zo >= a.nOutZ) break;
+            const float v = (den[k] >= a.thr) ? 20.0f * log10f(num[k] / den[k]) : __builtin_nanf("");
+            a.db[((long long)(a.plane0 + f) * a.nOutZ + zo) * a.nOutX + c] = v;
+            if (__builtin_isfinite(v)) {
+                const int kk = fkey(v);
+                kmin = min(kmin, kk);
+                kmax = max(kmax, kk);
+            }
+        }
+    }
+    smin[tid] = kmin;
+    smax[tid] = kmax;
+    __syncthreads();
+    for (int st = kRowTX * kStageZG / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+            smin[tid] = min(smin[tid], smin[tid + st]);
+            smax[tid] = max(smax[tid], smax[tid + st]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        // partial layout of the direct kernel: (plane, gz * gx) with this block in z-slot 0
+        const long long base = (long long)(a.plane0 + f) * a.blocks_per_plane;
+        a.partial[(base + blockIdx.x) * 2] = smin[0];
+        a.partial[(base + blockIdx.x) * 2 + 1] = smax[0];
+        for (int j = 1; j * (int)gridDim.x + (int)blockIdx.x < a.blocks_per_plane; ++j) {
+            const long long o = (base + j * gridDim.x + blockIdx.x) * 2;
+            a.partial[o] = 0x7FFFFFFF;
+            a.partial[o + 1] = (int)0x80000000;
+        }
+    }
+}
+
 // ---------------------------------------------------------------- legacy quantisation
 // All in float64 with no contraction, operation for operation as the host formulas:
 //   bits = min(uint16(round_half_away((db - c1) / (c2 - c1) * 65534)) + 1, 65535), NaN -> 0

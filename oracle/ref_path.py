@@ -457,3 +457,16 @@ def operator_fp64(geo: OracleGeometry, z0: int, nzc: int):
     z = np.arange(z0, z0 + nzc)
     F = np.exp(2j * np.pi * ((np.outer(z, np.arange(N))) % N) / N) * (geo.window / N)[None, :]
     return F @ Sp
+
+
+def top_percentile_metric(A, z_min=20, frac=0.01):
+    """Sharpness metric 'top_percentile' (octrecon/estimation/dispersion.py:46-63): per B-scan,
+    log of the mean of the k = max(1, int(M * frac)) largest |A|^2 (A float32, squares cast to
+    float64) over all x and z >= z_min."""
+    A = np.asarray(A, np.float32)
+    B = A.shape[0]
+    I = (A[:, :, z_min:] ** 2).reshape(B, -1).astype(np.float64)
+    M = I.shape[1]
+    k = max(1, int(M * frac))
+    top = np.partition(I, M - k, axis=1)[:, M - k:]
+    return np.log(top.mean(axis=1))
