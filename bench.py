@@ -223,6 +223,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the operator-form measurements")
     ap.add_argument("--copy-calibration", action="store_true", help="run a 1 GiB device copy first")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 200 000-A-scan all-bins measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: multi-GPU runs are not part of this build "
@@ -301,8 +302,9 @@ def main():
         }
         log(f"mfma: step {el_m / args.steps * 1e3:.2f} ms; opgemm avg {avg_m:.3f} ms; within 1 LSB of plain {within:.6f}")
         del bits_plain
-        res["config2_precision_form"] = config2(xp, args)
-        log(f"config2: {res['config2_precision_form']}")
+        if not args.no_config2:
+            res["config2_precision_form"] = config2(xp, args)
+            log(f"config2: {res['config2_precision_form']}")
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(rec, info, args.cpu_seconds)
     print(json.dumps(res), flush=True)
