@@ -292,8 +292,8 @@ int oct_stitch(const oct_stitch_desc* d, void* stream) {
 
 int oct_stitch_row_blocks(int n_out_z, int n_out_x) {
     if (n_out_z <= 0 || n_out_x <= 0) return 0;
-    const int gx = (n_out_x + oct::kRowTX - 1) / oct::kRowTX;
-    const int gz = (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer);
+    const int gx = (n_out_x + oct::kRowC - 1) / oct::kRowC;
+    const int gz = (n_out_z + oct::kRowZ - 1) / oct::kRowZ;
     return gx * gz;
 }
 
@@ -309,6 +309,9 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
     for (int i = 0; i < n_tiles; ++i) {
         const oct_row_tile& t = tiles[i];
         if (!t.mag || !t.z_rows || !t.z_w || !t.x_cols || !t.x_w || !t.fz) return fail(OCT_EINVAL, "tile %d: null pointer", i);
+        // tap pairs are read as 8-byte vectors
+        if (((uintptr_t)t.z_rows | (uintptr_t)t.z_w | (uintptr_t)t.x_cols | (uintptr_t)t.x_w) & 7)
+            return fail(OCT_EINVAL, "tile %d: tap arrays must be 8-byte aligned", i);
         if (t.c0 < 0 || t.C <= 0 || t.c0 + t.C > n_out_x) return fail(OCT_EINVAL, "tile %d: columns outside the output", i);
         a.t[i].mag = t.mag;
         a.t[i].mstride = t.mag_stride;
@@ -335,22 +338,14 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
     a.plane0 = plane0;
     a.partial = partial;
     a.blocks_per_plane = oct_stitch_row_blocks(n_out_z, n_out_x);
-    // LDS-staged variant for narrow depth windows (OCT_HIP_ROW_STAGED=1; default: direct reads)
-    const char* ev = std::getenv("OCT_HIP_ROW_STAGED");
-    bool staged = ev && std::atoi(ev) == 1 && n_out_z <= oct::kStageZG * oct::kStageZPer;
-    for (int i = 0; i < n_tiles && staged; ++i) {
-        const oct_row_tile& t = tiles[i];
-        if (t.mag_stride > oct::kStageRows) staged = false;     // stage narrow depth windows only
-    }
-    const int gx = (n_out_x + oct::kRowTX - 1) / oct::kRowTX;
-    if (staged) {
-        hipLaunchKernelGGL(oct::stitch_row_staged_kernel, dim3(gx, n_frames), dim3(oct::kRowTX, oct::kStageZG), 0,
+    if (n_frames > 65535) return fail(OCT_EINVAL, "%d planes in one call (at most 65535)", n_frames);
+    dim3 grid((n_out_x + oct::kRowC - 1) / oct::kRowC, (n_out_z + oct::kRowZ - 1) / oct::kRowZ, n_frames);
+    if (shift)
+        hipLaunchKernelGGL(oct::stitch_row_kernel<true>, grid, dim3(64 * oct::kRowWaves), 0,
                            reinterpret_cast<hipStream_t>(stream), a);
-    } else {
-        dim3 grid(gx, (n_out_z + oct::kRowTZ * oct::kRowZPer - 1) / (oct::kRowTZ * oct::kRowZPer), n_frames);
-        hipLaunchKernelGGL(oct::stitch_row_kernel, grid, dim3(oct::kRowTX, oct::kRowTZ), 0,
+    else
+        hipLaunchKernelGGL(oct::stitch_row_kernel<false>, grid, dim3(64 * oct::kRowWaves), 0,
                            reinterpret_cast<hipStream_t>(stream), a);
-    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

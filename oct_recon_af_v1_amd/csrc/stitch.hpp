@@ -120,63 +120,81 @@ struct RowArgs {
     int blocks_per_plane;
 };
 
-constexpr int kRowTX = 64, kRowTZ = 4, kRowZPer = 4;   // block 64 x 4 threads, 4 rows per thread
+// Block tile: kRowZ output rows x kRowC output columns of one plane.  Lane l of a wave owns row
+// zo = z0 + l % kRowZ and a column pair position l / kRowZ, so the magnitude reads of a wave hit
+// one or two consecutive A-scans (rows are contiguous in the tile's magnitude), and the z-weight
+// reads are contiguous.  The finalised values go through LDS so that the dB stores run along x.
+constexpr int kRowZ = 32, kRowC = 64, kRowWaves = 4;
+constexpr int kRowLanesC = 64 / kRowZ;                        // columns in flight per wave
+constexpr int kRowPasses = kRowC / (kRowWaves * kRowLanesC);  // column passes per wave
 
-__global__ __launch_bounds__(kRowTX * kRowTZ) void stitch_row_kernel(RowArgs a) {
-    const int c = blockIdx.x * kRowTX + threadIdx.x;
-    const int zbase = blockIdx.y * (kRowTZ * kRowZPer) + threadIdx.y;
+template <bool kShift>   // optical-path shift present: kernels without it carry no per-x lookups
+__global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
+    __shared__ float tile[kRowZ][kRowC + 1];
+    __shared__ int smin[kRowWaves], smax[kRowWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int zo = blockIdx.y * kRowZ + (lane % kRowZ);
+    const int cb0 = blockIdx.x * kRowC;
     const int f = blockIdx.z;
-    float num[kRowZPer], den[kRowZPer];
-#pragma unroll
-    for (int k = 0; k < kRowZPer; ++k) num[k] = den[k] = 0.f;
     const int fb = a.frame_idx[f];
-    if (c < a.nOutX) {
-        for (int ti = 0; ti < a.ntiles; ++ti) {
-            const RowTile& t = a.t[ti];
+    const bool zok = zo < a.nOutZ;
+    float num[kRowPasses], den[kRowPasses];
+#pragma unroll
+    for (int q = 0; q < kRowPasses; ++q) num[q] = den[q] = 0.f;
+    // Tiles outer (in order), columns inner: per voxel the same summation order as the
+    // per-tile path (tile, then x tap, then z tap).  The body is branch-free so that the loads
+    // of all columns issue together: taps outside a tile or of zero weight read an in-range
+    // element and contribute an exact +0 through their zero weight.
+    const int zc = zok ? zo : 0;
+    for (int ti = 0; ti < a.ntiles; ++ti) {
+        const RowTile& t = a.t[ti];
+        if (cb0 + kRowC <= t.c0 || cb0 >= t.c0 + t.C) continue;       // uniform per block
+        const int2 rr = reinterpret_cast<const int2*>(t.zr)[zc];
+        const float2 ww = reinterpret_cast<const float2*>(t.zw)[zc];
+        const int r0 = rr.x, r1 = rr.y;
+        const float wz0 = zok ? ww.x : 0.f, wz1 = zok ? ww.y : 0.f;
+        const float f0 = t.fz[min(max(r0, 0), a.nZ - 1)], f1 = t.fz[min(max(r1, 0), a.nZ - 1)];
+        const int slo = max(t.mrow0, 0), shi = min(t.mrow0 + (int)t.mstride, a.nZ) - 1;   // computed rows
+#pragma unroll
+        for (int q = 0; q < kRowPasses; ++q) {
+            const int c = cb0 + (q * kRowWaves + wave) * kRowLanesC + lane / kRowZ;
             const int cl = c - t.c0;
-            if (cl < 0 || cl >= t.C) continue;
+            const bool in = c < a.nOutX && cl >= 0 && cl < t.C;
+            const int cc = min(max(cl, 0), t.C - 1);
+            const int2 xr = reinterpret_cast<const int2*>(t.xr)[cc];
+            const float2 xw = reinterpret_cast<const float2*>(t.xw)[cc];
 #pragma unroll
             for (int tx = 0; tx < 2; ++tx) {
-                const float wx = t.xw[2 * cl + tx];
-                if (wx == 0.f) continue;
-                const int x = t.xr[2 * cl + tx];
+                const float wx = in ? (tx ? xw.y : xw.x) : 0.f;
+                const int x = tx ? xr.y : xr.x;
                 int sh = 0, lo = 0, hi = a.nZ - 1;
-                if (a.shift) {
+                if constexpr (kShift) {
                     const long long o = (long long)fb * a.nX + x;
                     sh = a.shift[o];
                     lo = a.vlo[o];
                     hi = a.vhi[o];
                 }
                 const float* mrow = t.mag + ((long long)f * a.nX + x) * t.mstride - t.mrow0;
-#pragma unroll
-                for (int k = 0; k < kRowZPer; ++k) {
-                    const int zo = zbase + kRowTZ * k;
-                    if (zo >= a.nOutZ) continue;
-                    float sa = 0.f, sf = 0.f;
-#pragma unroll
-                    for (int tz = 0; tz < 2; ++tz) {
-                        const float wz = t.zw[2 * zo + tz];
-                        if (wz == 0.f) continue;
-                        const int r = t.zr[2 * zo + tz];
-                        const float fw = (r >= lo && r <= hi) ? t.fz[r] : 0.f;
-                        const int src = min(max(r + sh, 0), a.nZ - 1);
-                        sa = __builtin_fmaf(mrow[src] * fw, wz, sa);
-                        sf = __builtin_fmaf(fw, wz, sf);
-                    }
-                    num[k] = __builtin_fmaf(sa, wx, num[k]);
-                    den[k] = __builtin_fmaf(sf, wx, den[k]);
-                }
+                const float fw0 = (r0 >= lo && r0 <= hi) ? f0 : 0.f;
+                const float fw1 = (r1 >= lo && r1 <= hi) ? f1 : 0.f;
+                const int s0 = min(max(min(max(r0 + sh, 0), a.nZ - 1), slo), shi);
+                const int s1 = min(max(min(max(r1 + sh, 0), a.nZ - 1), slo), shi);
+                float sa = __builtin_fmaf(mrow[s0] * fw0, wz0, 0.f);
+                float sf = __builtin_fmaf(fw0, wz0, 0.f);
+                sa = __builtin_fmaf(mrow[s1] * fw1, wz1, sa);
+                sf = __builtin_fmaf(fw1, wz1, sf);
+                num[q] = __builtin_fmaf(sa, wx, num[q]);
+                den[q] = __builtin_fmaf(sf, wx, den[q]);
             }
         }
     }
     int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
-    if (c < a.nOutX) {
 #pragma unroll
-        for (int k = 0; k < kRowZPer; ++k) {
-            const int zo = zbase + kRowTZ * k;
-            if (zo >= a.nOutZ) continue;
-            const float v = (den[k] >= a.thr) ? 20.0f * log10f(num[k] / den[k]) : __builtin_nanf("");
-            a.db[((long long)(a.plane0 + f) * a.nOutZ + zo) * a.nOutX + c] = v;
+    for (int q = 0; q < kRowPasses; ++q) {
+        const int cl0 = (q * kRowWaves + wave) * kRowLanesC + lane / kRowZ;
+        if (zok && cb0 + cl0 < a.nOutX) {
+            const float v = (den[q] >= a.thr) ? 20.0f * log10f(num[q] / den[q]) : __builtin_nanf("");
+            tile[lane % kRowZ][cl0] = v;
             if (__builtin_isfinite(v)) {
                 const int kk = fkey(v);
                 kmin = min(kmin, kk);
@@ -184,22 +202,30 @@ __global__ __launch_bounds__(kRowTX * kRowTZ) void stitch_row_kernel(RowArgs a) 
             }
         }
     }
-    __shared__ int smin[kRowTX * kRowTZ], smax[kRowTX * kRowTZ];
-    const int tid = threadIdx.y * kRowTX + threadIdx.x;
-    smin[tid] = kmin;
-    smax[tid] = kmax;
-    __syncthreads();
-    for (int s = kRowTX * kRowTZ / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            smin[tid] = min(smin[tid], smin[tid + s]);
-            smax[tid] = max(smax[tid], smax[tid + s]);
-        }
-        __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, off));
+        kmax = max(kmax, __shfl_xor(kmax, off));
     }
-    if (tid == 0) {
+    if (lane == 0) {
+        smin[wave] = kmin;
+        smax[wave] = kmax;
+    }
+    __syncthreads();
+    // coalesced stores: consecutive threads write consecutive columns of one output row
+    for (int e = threadIdx.x; e < kRowZ * kRowC; e += 64 * kRowWaves) {
+        const int zr = e / kRowC, cc = e - zr * kRowC;
+        const int z = blockIdx.y * kRowZ + zr, c = cb0 + cc;
+        if (z < a.nOutZ && c < a.nOutX) a.db[((long long)(a.plane0 + f) * a.nOutZ + z) * a.nOutX + c] = tile[zr][cc];
+    }
+    if (threadIdx.x == 0) {
+        int mn = smin[0], mx = smax[0];
+        for (int w = 1; w < kRowWaves; ++w) {
+            mn = min(mn, smin[w]);
+            mx = max(mx, smax[w]);
+        }
         const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + blockIdx.y * gridDim.x + blockIdx.x) * 2;
-        a.partial[o] = smin[0];
-        a.partial[o + 1] = smax[0];
+        a.partial[o] = mn;
+        a.partial[o + 1] = mx;
     }
 }
 __device__ __forceinline__ int fkey(float f) {
@@ -295,119 +321,6 @@ __global__ __launch_bounds__(kFinThreads) void clim_kernel(const int* __restrict
     }
 }
 
-// Staged variant of stitch_row_kernel for narrow depth windows (every tile's magnitude stride
-// <= kStageRows, nOutZ <= kStageZG * kStageZPer): one block owns 64 output columns and ALL
-// output rows of one plane; for each tile covering those columns, the tile's contiguous
-// magnitude slice (the ~65 A-scans behind the columns x its depth rows) is loaded into LDS with
-// coalesced loads, and every output row is formed from LDS.  Same arithmetic per voxel as
-// stitch_row_kernel.
-constexpr int kStageRows = 64, kStageX = 80, kStageZG = 4, kStageZPer = 40;
-// staged rows are padded by one float so that lanes reading consecutive A-scans hit distinct banks
-
-__global__ __launch_bounds__(kRowTX * kStageZG) void stitch_row_staged_kernel(RowArgs a) {
-    __shared__ float slice[kStageX * (kStageRows + 1)];
-    __shared__ int smin[kRowTX * kStageZG], smax[kRowTX * kStageZG];
-    const int cb0 = blockIdx.x * kRowTX;
-    const int c = cb0 + threadIdx.x;
-    const int zg = threadIdx.y;
-    const int f = blockIdx.y;
-    const int tid = threadIdx.y * kRowTX + threadIdx.x;
-    float num[kStageZPer], den[kStageZPer];
-#pragma unroll
-    for (int k = 0; k < kStageZPer; ++k) num[k] = den[k] = 0.f;
-    const int fb = a.frame_idx[f];
-    for (int ti = 0; ti < a.ntiles; ++ti) {
-        const RowTile& t = a.t[ti];
-        const int clo = max(cb0, t.c0), chi = min(cb0 + kRowTX, t.c0 + t.C);   // block columns in the tile
-        if (clo >= chi) continue;                                           // uniform per block
-        const int xlo = t.xr[2 * (clo - t.c0)];
-        const int xhi = t.xr[2 * (chi - 1 - t.c0) + 1];
-        const int nxs = min(xhi, a.nX - 1) - xlo + 1;
-        const int rows = (int)t.mstride;
-        // stage nxs A-scans x rows (contiguous in memory) when they fit; otherwise read directly
-        const bool staged = nxs <= kStageX && rows <= kStageRows;      // uniform per block
-        const float* src = t.mag + ((long long)f * a.nX + xlo) * t.mstride;
-        const int prow = rows + 1;
-        if (staged)
-            for (int e = tid; e < nxs * rows; e += kRowTX * kStageZG) {
-                const int xi = e / rows;
-                slice[xi * prow + (e - xi * rows)] = src[e];
-            }
-        __syncthreads();
-        const int cl = c - t.c0;
-        if (c < a.nOutX && cl >= 0 && cl < t.C) {
-#pragma unroll
-            for (int tx = 0; tx < 2; ++tx) {
-                const float wx = t.xw[2 * cl + tx];
-                if (wx == 0.f) continue;
-                const int x = t.xr[2 * cl + tx];
-                int sh = 0, lo = 0, hi = a.nZ - 1;
-                if (a.shift) {
-                    const long long o = (long long)fb * a.nX + x;
-                    sh = a.shift[o];
-                    lo = a.vlo[o];
-                    hi = a.vhi[o];
-                }
-                const float* mrow = (staged ? slice + (x - xlo) * prow : src + (long long)(x - xlo) * rows) - t.mrow0;
-#pragma unroll
-                for (int k = 0; k < kStageZPer; ++k) {
-                    const int zo = zg + kStageZG * k;
-                    if (zo >= a.nOutZ) break;
-                    float sa = 0.f, sf = 0.f;
-#pragma unroll
-                    for (int tz = 0; tz < 2; ++tz) {
-                        const float wz = t.zw[2 * zo + tz];
-                        if (wz == 0.f) continue;
-                        const int r = t.zr[2 * zo + tz];
-                        const float fw = (r >= lo && r <= hi) ? t.fz[r] : 0.f;
-                        const int srow = min(max(r + sh, 0), a.nZ - 1);
-                        sa = __builtin_fmaf(mrow[srow] * fw, wz, sa);
-                        sf = __builtin_fmaf(fw, wz, sf);
-                    }
-                    num[k] = __builtin_fmaf(sa, wx, num[k]);
-                    den[k] = __builtin_fmaf(sf, wx, den[k]);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
-    if (c < a.nOutX) {
-#pragma unroll
-        for (int k = 0; k < kStageZPer; ++k) {
-            const int zo = zg + kStageZG * k;
-            if (zo >= a.nOutZ) break;
-            const float v = (den[k] >= a.thr) ? 20.0f * log10f(num[k] / den[k]) : __builtin_nanf("");
-            a.db[((long long)(a.plane0 + f) * a.nOutZ + zo) * a.nOutX + c] = v;
-            if (__builtin_isfinite(v)) {
-                const int kk = fkey(v);
-                kmin = min(kmin, kk);
-                kmax = max(kmax, kk);
-            }
-        }
-    }
-    smin[tid] = kmin;
-    smax[tid] = kmax;
-    __syncthreads();
-    for (int st = kRowTX * kStageZG / 2; st > 0; st >>= 1) {
-        if (tid < st) {
-            smin[tid] = min(smin[tid], smin[tid + st]);
-            smax[tid] = max(smax[tid], smax[tid + st]);
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        // partial layout of the direct kernel: (plane, gz * gx) with this block in z-slot 0
-        const long long base = (long long)(a.plane0 + f) * a.blocks_per_plane;
-        a.partial[(base + blockIdx.x) * 2] = smin[0];
-        a.partial[(base + blockIdx.x) * 2 + 1] = smax[0];
-        for (int j = 1; j * (int)gridDim.x + (int)blockIdx.x < a.blocks_per_plane; ++j) {
-            const long long o = (base + j * gridDim.x + blockIdx.x) * 2;
-            a.partial[o] = 0x7FFFFFFF;
-            a.partial[o + 1] = (int)0x80000000;
-        }
-    }
-}
 
 // ---------------------------------------------------------------- legacy quantisation
 // All in float64 with no contraction, operation for operation as the host formulas:
