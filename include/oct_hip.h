@@ -154,6 +154,11 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
  * oct_topfrac_metric: out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over all
  * x and z >= z_min), M = n_x (n_z - z_min); mag (n_bscans, n_x, n_z) float32, out float64. */
 int oct_plan_use_window(oct_plan* plan, const float* dev_window);
+/* Per-file candidates: file b of later oct_spectral launches uses the window at
+ * dev_windows + b * file_stride floats (file_stride 0: one window for every file, >= 2N otherwise).
+ * Used by the batched per-B-scan search (each B-scan its own bracket), which the reference runs
+ * one B-scan at a time (octrecon/estimation/dispersion.py:226-239). */
+int oct_plan_use_windows(oct_plan* plan, const float* dev_windows, long long file_stride);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* stream);
 
@@ -171,6 +176,46 @@ int oct_opplan_destroy(oct_opplan* plan);
 /* magnitudes |A (raw - bg)| of bins [z0, z0+nzc) -> out (M, out_stride) float32 */
 int oct_opspectral(const oct_opplan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                    int apod_size, float* out, long long out_stride, const float* bg, void* stream);
+
+/* ---- focus detector input (octrecon/estimation/focus.py:209-218, 221-279: the depth profile
+ * np.median over x of |A| per B-scan, float32 magnitudes compared in float64)
+ * out[b, z] = median over x of mag[b, x, z] (float64 mean of the two middle values for even n_x),
+ * mag rows z in [0, n_z) of A-scans spaced x_stride floats; n_x <= 1024. */
+int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, double* out, void* stream);
+
+/* ---- learned beta / focus estimator (no counterpart in the reference: it replaces the searches
+ * of octrecon/estimation/dispersion.py:154-267 and focus.py:191-218, which stay the labels, the
+ * judge and the fallback).  bfloat16 activations and weights (uint16_t bit patterns, round to
+ * nearest even), float32 master weights, gradients and accumulation.
+ * Network input: out[b, channel, h, w] (n_bscans, channels, height, width) bf16 =
+ *   clamp((20 log10(max_{j<zpool} mag[b, x0 + w, z0 + h zpool + j]) - offset_db) / scale_db, -8, 8). */
+int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride, int n_rows, int x0, int z0, int zpool,
+                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, int channel,
+                    void* stream);
+/* conv 3x3 / stride 2 / pad 1 + ReLU: x (n, ci, h, w) -> y (n, co, h/2, w/2); w (co, ci, 3, 3); co % 16 == 0 */
+int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
+                     int wd, int co, void* stream);
+/* dx = conv^T(dpre) (n, ci, h, w) float32, times (xmask > 0) when xmask is given; ci % 16 == 0 */
+int oct_cnn_conv_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci, int h,
+                          int wd, int co, void* stream);
+size_t oct_cnn_conv_bwd_weight_workspace_bytes(int n, int ci, int h, int wd, int co);
+/* dw (co, ci, 3, 3) and db (co) float32 from dpre (n, co, h/2, w/2) and x (n, ci, h, w) */
+int oct_cnn_conv_bwd_weight(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int ci, int h, int wd,
+                            int co, void* workspace, size_t workspace_bytes, void* stream);
+/* fully connected y = x w^T + b: exactly one of y_relu_bf16 (ReLU, bf16) or y_linear (float32) */
+int oct_cnn_fc_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y_relu_bf16, float* y_linear,
+                   int n, int k, int o, void* stream);
+int oct_cnn_fc_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int k, int o,
+                        void* stream);
+int oct_cnn_fc_bwd_weight(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int k, int o,
+                          void* stream);
+/* heteroscedastic Gaussian NLL over head (n, 2T) = (means, log variances), targets (n, T), T <= 4;
+ * loss[0] = mean loss, loss[1 + j] = mean squared error of target j; dhead (n, 2T) */
+int oct_cnn_gauss_nll(const float* head, const float* target, const float* weights, float* loss, float* dhead, int n,
+                      int n_targets, void* stream);
+/* Adam with bias correction and decoupled weight decay; params_bf16 receives the rounded weights */
+int oct_cnn_adam(float* params, const float* grads, float* m, float* v, uint16_t* params_bf16, long long count,
+                 float lr, float beta1, float beta2, float eps, float weight_decay, int step, void* stream);
 
 const char* oct_last_error(void);
 

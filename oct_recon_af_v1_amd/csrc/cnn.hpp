@@ -1,0 +1,381 @@
+// Learned beta / focus estimator: BF16 convolutional network kernels (training and inference).
+//
+// Storage: activations and weights in bfloat16 (round to nearest even), master weights, gradients
+// and every accumulation in float32.  Layers:
+//   conv 3x3, stride 2, zero padding 1, ReLU      x (N, Ci, H, W) -> y (N, Co, H/2, W/2)
+//   fully connected (+ ReLU or linear head)        x (N, K)        -> y (N, O)
+// Backward kernels take the gradient of the layer's pre-activation output (dpre) and produce the
+// gradient of its input already multiplied by the ReLU mask of the layer below (x > 0), i.e. the
+// dpre of the layer below.  Every index below is checked against the shapes on the host.
+//
+// There is no counterpart in the reference (octrecon/estimation/dispersion.py:154-267 and
+// focus.py:191-218 are the searches this network replaces; they remain the labels and the judge).
+#pragma once
+#include "oct_common.hpp"
+
+namespace oct {
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)0x7FC0u;       // NaN stays NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);                                      // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+constexpr int kCnnThreads = 256;
+constexpr int kConvCoB = 16;   // output channels per thread (forward)
+constexpr int kConvCiB = 16;   // input channels per thread (backward data)
+
+// y[n, co, oy, ox] = relu(b[co] + sum_ci sum_{ky,kx} w[co, ci, ky, kx] x[n, ci, 2oy-1+ky, 2ox-1+kx])
+// Grid (ceil(N Ho Wo / 256), Co / 16); summation order: ci ascending, then ky, kx.
+__global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* __restrict__ x,
+                                                               const uint16_t* __restrict__ w,
+                                                               const float* __restrict__ bias,
+                                                               uint16_t* __restrict__ y, int N, int Ci, int H,
+                                                               int W, int Co) {
+    const int Ho = H >> 1, Wo = W >> 1;
+    const long long P = (long long)N * Ho * Wo;
+    const long long p = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (p >= P) return;
+    const int co0 = blockIdx.y * kConvCoB;
+    const int ox = (int)(p % Wo);
+    const long long t = p / Wo;
+    const int oy = (int)(t % Ho);
+    const int n = (int)(t / Ho);
+    float acc[kConvCoB];
+#pragma unroll
+    for (int j = 0; j < kConvCoB; ++j) acc[j] = bias[co0 + j];
+    const int iy0 = 2 * oy - 1, ix0 = 2 * ox - 1;     // iy0 + 2 <= H - 1 and ix0 + 2 <= W - 1 always
+    for (int ci = 0; ci < Ci; ++ci) {
+        const uint16_t* xp = x + ((long long)n * Ci + ci) * H * W;
+        float v[9];
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int iy = iy0 + ky, ix = ix0 + kx;
+                v[3 * ky + kx] = (iy >= 0 && ix >= 0) ? bf2f(xp[(long long)iy * W + ix]) : 0.f;
+            }
+        const uint16_t* wp = w + ((long long)co0 * Ci + ci) * 9;
+#pragma unroll
+        for (int j = 0; j < kConvCoB; ++j)
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[j] = __builtin_fmaf(bf2f(wp[(long long)j * Ci * 9 + k]), v[k], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kConvCoB; ++j)
+        y[((long long)n * Co + co0 + j) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(acc[j], 0.f));
+}
+
+// dx[n, ci, iy, ix] = sum_co sum_taps w[co, ci, ky, kx] dpre[n, co, oy, ox] over iy = 2oy-1+ky,
+// ix = 2ox-1+kx, times (xmask[n, ci, iy, ix] > 0) when xmask is given.  One thread owns the 2 x 2
+// input positions (2a + {0,1}, 2b + {0,1}) for 16 input channels, so every thread uses all 9 taps
+// (uniform weight reads).  Grid (ceil(N Ho Wo / 256), Ci / 16).
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_data_kernel(const float* __restrict__ dpre,
+                                                                    const uint16_t* __restrict__ w,
+                                                                    const uint16_t* __restrict__ xmask,
+                                                                    float* __restrict__ dx, int N, int Ci, int H,
+                                                                    int W, int Co) {
+    const int Ho = H >> 1, Wo = W >> 1;
+    const long long P = (long long)N * Ho * Wo;
+    const long long p = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (p >= P) return;
+    const int ci0 = blockIdx.y * kConvCiB;
+    const int b = (int)(p % Wo);
+    const long long t = p / Wo;
+    const int a = (int)(t % Ho);
+    const int n = (int)(t / Ho);
+    const bool ra = a + 1 < Ho, rb = b + 1 < Wo;
+    float acc[4][kConvCiB];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < kConvCiB; ++j) acc[q][j] = 0.f;
+    for (int co = 0; co < Co; ++co) {
+        const float* g = dpre + ((long long)n * Co + co) * Ho * Wo;
+        const float g00 = g[(long long)a * Wo + b];
+        const float g01 = rb ? g[(long long)a * Wo + b + 1] : 0.f;
+        const float g10 = ra ? g[(long long)(a + 1) * Wo + b] : 0.f;
+        const float g11 = (ra && rb) ? g[(long long)(a + 1) * Wo + b + 1] : 0.f;
+        const uint16_t* wp = w + ((long long)co * Ci + ci0) * 9;
+#pragma unroll
+        for (int j = 0; j < kConvCiB; ++j) {
+            float wk[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) wk[k] = bf2f(wp[j * 9 + k]);
+            acc[0][j] = __builtin_fmaf(wk[4], g00, acc[0][j]);                         // (2a,   2b)
+            acc[1][j] = __builtin_fmaf(wk[5], g00, __builtin_fmaf(wk[3], g01, acc[1][j]));   // (2a,   2b+1)
+            acc[2][j] = __builtin_fmaf(wk[7], g00, __builtin_fmaf(wk[1], g10, acc[2][j]));   // (2a+1, 2b)
+            acc[3][j] = __builtin_fmaf(
+                wk[8], g00, __builtin_fmaf(wk[6], g01, __builtin_fmaf(wk[2], g10, __builtin_fmaf(wk[0], g11, acc[3][j]))));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kConvCiB; ++j) {
+        const long long base = ((long long)n * Ci + ci0 + j) * H * W;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long o = base + (long long)(2 * a + (q >> 1)) * W + 2 * b + (q & 1);
+            float v = acc[q][j];
+            if (xmask && !(bf2f(xmask[o]) > 0.f)) v = 0.f;
+            dx[o] = v;
+        }
+    }
+}
+
+// Per (co, ci, split): partial sums over a slice of the N Ho Wo output positions of
+//   dw[co, ci, ky, kx] = sum dpre[n, co, oy, ox] x[n, ci, 2oy-1+ky, 2ox-1+kx],  db[co] = sum dpre
+// into part[((s Co + co) Ci + ci) 10 + k] (k = 9: the bias sum).  Grid (Co Ci, S).
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_kernel(const float* __restrict__ dpre,
+                                                                      const uint16_t* __restrict__ x,
+                                                                      float* __restrict__ part, int N, int Ci,
+                                                                      int H, int W, int Co) {
+    const int co = blockIdx.x / Ci, ci = blockIdx.x - (blockIdx.x / Ci) * Ci;
+    const int s = blockIdx.y, S = gridDim.y;
+    const int Ho = H >> 1, Wo = W >> 1;
+    const long long P = (long long)N * Ho * Wo;
+    float acc[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) acc[k] = 0.f;
+    for (long long p = (long long)s * kCnnThreads + threadIdx.x; p < P; p += (long long)S * kCnnThreads) {
+        const int ox = (int)(p % Wo);
+        const long long t = p / Wo;
+        const int oy = (int)(t % Ho);
+        const int n = (int)(t / Ho);
+        const float g = dpre[((long long)n * Co + co) * Ho * Wo + (long long)oy * Wo + ox];
+        const uint16_t* xp = x + ((long long)n * Ci + ci) * H * W;
+        const int iy0 = 2 * oy - 1, ix0 = 2 * ox - 1;
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const int iy = iy0 + ky, ix = ix0 + kx;
+                const float v = (iy >= 0 && ix >= 0) ? bf2f(xp[(long long)iy * W + ix]) : 0.f;
+                acc[3 * ky + kx] = __builtin_fmaf(g, v, acc[3 * ky + kx]);
+            }
+        acc[9] += g;
+    }
+    __shared__ float red[kCnnThreads / 64][10];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        float v = acc[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 10) {
+        float v = red[0][threadIdx.x];
+        for (int wv = 1; wv < kCnnThreads / 64; ++wv) v += red[wv][threadIdx.x];
+        part[(((long long)s * Co + co) * Ci + ci) * 10 + threadIdx.x] = v;
+    }
+}
+
+// dw[(co Ci + ci) 9 + k] = sum_s part[.., k]; db[co] = sum_s part[(s, co, ci = 0), 9].
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_reduce_kernel(const float* __restrict__ part,
+                                                                             float* __restrict__ dw,
+                                                                             float* __restrict__ db, int Ci, int Co,
+                                                                             int S) {
+    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    const long long nw = (long long)Co * Ci * 9;
+    if (i < nw) {
+        const long long cc = i / 9;
+        const int k = (int)(i - cc * 9);
+        float v = 0.f;
+        for (int s = 0; s < S; ++s) v += part[((long long)s * Co * Ci + cc) * 10 + k];
+        dw[i] = v;
+    } else if (i < nw + Co) {
+        const int co = (int)(i - nw);
+        float v = 0.f;
+        for (int s = 0; s < S; ++s) v += part[((long long)s * Co * Ci + (long long)co * Ci) * 10 + 9];
+        db[co] = v;
+    }
+}
+
+// y[n, o] = b[o] + sum_k w[o, k] x[n, k] (k ascending); ReLU + bf16 store, or a float32 store.
+__global__ __launch_bounds__(kCnnThreads) void fc_fwd_kernel(const uint16_t* __restrict__ x,
+                                                             const uint16_t* __restrict__ w,
+                                                             const float* __restrict__ bias,
+                                                             uint16_t* __restrict__ ybf, float* __restrict__ yf,
+                                                             int N, int K, int O) {
+    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (i >= (long long)N * O) return;
+    const int n = (int)(i / O), o = (int)(i - (long long)(i / O) * O);
+    const uint16_t* xr = x + (long long)n * K;
+    const uint16_t* wr = w + (long long)o * K;
+    float acc = bias[o];
+    for (int k = 0; k < K; ++k) acc = __builtin_fmaf(bf2f(wr[k]), bf2f(xr[k]), acc);
+    if (ybf) ybf[i] = f2bf(fmaxf(acc, 0.f));
+    else yf[i] = acc;
+}
+
+// dx[n, k] = sum_o dpre[n, o] w[o, k] (o ascending), times (xmask[n, k] > 0) when given.
+__global__ __launch_bounds__(kCnnThreads) void fc_bwd_data_kernel(const float* __restrict__ dpre,
+                                                                  const uint16_t* __restrict__ w,
+                                                                  const uint16_t* __restrict__ xmask,
+                                                                  float* __restrict__ dx, int N, int K, int O) {
+    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (i >= (long long)N * K) return;
+    const int n = (int)(i / K), k = (int)(i - (long long)(i / K) * K);
+    float acc = 0.f;
+    for (int o = 0; o < O; ++o) acc = __builtin_fmaf(dpre[(long long)n * O + o], bf2f(w[(long long)o * K + k]), acc);
+    if (xmask && !(bf2f(xmask[i]) > 0.f)) acc = 0.f;
+    dx[i] = acc;
+}
+
+// dw[o, k] = sum_n dpre[n, o] x[n, k];  db[o] = sum_n dpre[n, o]  (n ascending).
+__global__ __launch_bounds__(kCnnThreads) void fc_bwd_weight_kernel(const float* __restrict__ dpre,
+                                                                    const uint16_t* __restrict__ x,
+                                                                    float* __restrict__ dw, float* __restrict__ db,
+                                                                    int N, int K, int O) {
+    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    const long long nw = (long long)O * K;
+    if (i < nw) {
+        const int o = (int)(i / K), k = (int)(i - (long long)(i / K) * K);
+        float acc = 0.f;
+        for (int n = 0; n < N; ++n) acc = __builtin_fmaf(dpre[(long long)n * O + o], bf2f(x[(long long)n * K + k]), acc);
+        dw[i] = acc;
+    } else if (i < nw + O) {
+        const int o = (int)(i - nw);
+        float acc = 0.f;
+        for (int n = 0; n < N; ++n) acc += dpre[(long long)n * O + o];
+        db[o] = acc;
+    }
+}
+
+// Heteroscedastic Gaussian negative log-likelihood, head (N, 2T) = (mu_0..mu_T-1, s_0..s_T-1),
+// s = log variance, targets (N, T), per-target weights wt (T), T <= 4:
+//   L = (1/N) sum_n sum_j wt_j 0.5 ((t - mu)^2 exp(-s) + s)
+// dhead = dL/dhead; loss[0] = L, loss[1 + j] = (1/N) sum_n (t - mu_j)^2.  One block.
+constexpr int kNllMaxT = 4;
+__global__ __launch_bounds__(kCnnThreads) void gauss_nll_kernel(const float* __restrict__ head,
+                                                                const float* __restrict__ target,
+                                                                const float* __restrict__ wt, float* __restrict__ loss,
+                                                                float* __restrict__ dhead, int N, int T) {
+    __shared__ float red[kNllMaxT + 1][kCnnThreads];
+    float acc[kNllMaxT + 1];
+#pragma unroll
+    for (int j = 0; j <= kNllMaxT; ++j) acc[j] = 0.f;
+    const float invN = 1.0f / (float)N;
+    for (int n = threadIdx.x; n < N; n += kCnnThreads) {
+        for (int j = 0; j < T; ++j) {
+            const float mu = head[(long long)n * 2 * T + j];
+            const float s = head[(long long)n * 2 * T + T + j];
+            const float e = expf(-s);
+            const float r = target[(long long)n * T + j] - mu;
+            acc[kNllMaxT] += wt[j] * 0.5f * (r * r * e + s);
+            acc[j] += r * r;
+            dhead[(long long)n * 2 * T + j] = -wt[j] * r * e * invN;
+            dhead[(long long)n * 2 * T + T + j] = wt[j] * 0.5f * (1.f - r * r * e) * invN;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j <= kNllMaxT; ++j) red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int st = kCnnThreads / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+#pragma unroll
+            for (int j = 0; j <= kNllMaxT; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        loss[0] = red[kNllMaxT][0] * invN;
+        for (int j = 0; j < T; ++j) loss[1 + j] = red[j][0] * invN;
+    }
+}
+
+// Adam (bias-corrected, decoupled weight decay):  m = b1 m + (1-b1) g;  v = b2 v + (1-b2) g^2;
+// p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps) + lr wd p;  pbf = bf16(p).
+__global__ __launch_bounds__(kCnnThreads) void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                                                           float* __restrict__ m, float* __restrict__ v,
+                                                           uint16_t* __restrict__ pbf, long long count, float lr,
+                                                           float b1, float b2, float eps, float wd, float c1,
+                                                           float c2) {
+    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float pi = p[i];
+    pi -= lr * wd * pi;
+    pi -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    pbf[i] = f2bf(pi);
+}
+
+// Network input from a magnitude volume (B, nX, rows) (row stride `xstride` floats per A-scan):
+//   out[b, c, h, w] = bf16(clamp((20 log10(max_{j < zpool} mag[b, x0 + w, z0 + h zpool + j]) - offset)
+//                                 * inv_scale, -8, 8))
+// with magnitudes below 1e-30 taken as 1e-30.  Grid (ceil(H W / 256), B); h runs fastest.
+__global__ __launch_bounds__(kCnnThreads) void logmag_crop_kernel(const float* __restrict__ mag, long long xstride,
+                                                                  int nX, int x0, int z0, int zpool, int H, int W,
+                                                                  float offset, float inv_scale,
+                                                                  uint16_t* __restrict__ out, int C, int c) {
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * kCnnThreads + threadIdx.x;
+    if (e >= H * W) return;
+    const int h = e % H, w = e / H;
+    const float* src = mag + ((long long)b * nX + x0 + w) * xstride + z0 + (long long)h * zpool;
+    float m = src[0];
+    for (int j = 1; j < zpool; ++j) m = fmaxf(m, src[j]);
+    const float db = 20.0f * log10f(fmaxf(m, 1e-30f));
+    const float y = fminf(fmaxf((db - offset) * inv_scale, -8.f), 8.f);
+    out[(((long long)b * C + c) * H + h) * W + w] = f2bf(y);
+}
+
+// Exact median over x of mag[b, x, z] for every (b, z): float64 mean of the two middle order
+// statistics (the single middle one for odd nX), as numpy.median of the float32 values cast to
+// float64.  Block (b, 8 depth rows), the columns staged in LDS, order statistics by bisection on
+// the order-preserving integer keys.  Grid (ceil(nz / 8), B), LDS 8 (nX + 1) floats, nX <= 1024.
+constexpr int kMedZ = 8;
+constexpr int kMedMaxX = 1024;
+
+__device__ __forceinline__ int med_key(float f) {
+    const int i = __builtin_bit_cast(int, f);
+    return i >= 0 ? i : (i ^ 0x7FFFFFFF);
+}
+
+__device__ float wave_select(const float* col, int nX, int k) {
+    // smallest key u with #(key <= u) >= k + 1
+    long long lo = -2147483648LL, hi = 2147483647LL;
+    const int lane = threadIdx.x & 63;
+    while (lo < hi) {
+        const long long mid = lo + ((hi - lo) >> 1);
+        int cnt = 0;
+        for (int x = lane; x < nX; x += 64) cnt += (med_key(col[x]) <= mid) ? 1 : 0;
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+        if (cnt >= k + 1) hi = mid;
+        else lo = mid + 1;
+    }
+    const int key = (int)lo;
+    return __builtin_bit_cast(float, key >= 0 ? key : (key ^ 0x7FFFFFFF));
+}
+
+__global__ __launch_bounds__(kCnnThreads) void depth_median_kernel(const float* __restrict__ mag, long long xstride,
+                                                                   int nX, int nz, double* __restrict__ out) {
+    extern __shared__ float cols[];   // [kMedZ][nX + 1]
+    const int b = blockIdx.y;
+    const int z0 = blockIdx.x * kMedZ;
+    const int ld = nX + 1;
+    for (int e = threadIdx.x; e < nX * kMedZ; e += kCnnThreads) {
+        const int x = e / kMedZ, zz = e - x * kMedZ;
+        const int z = z0 + zz;
+        cols[zz * ld + x] = (z < nz) ? mag[((long long)b * nX + x) * xstride + z] : 0.f;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    for (int zz = wave; zz < kMedZ; zz += kCnnThreads / 64) {
+        const int z = z0 + zz;
+        if (z >= nz) break;
+        const float* col = cols + zz * ld;
+        const float v1 = wave_select(col, nX, (nX - 1) / 2);
+        const float v2 = (nX & 1) ? v1 : wave_select(col, nX, nX / 2);
+        if ((threadIdx.x & 63) == 0) out[(long long)b * nz + z] = 0.5 * ((double)v1 + (double)v2);
+    }
+}
+
+}  // namespace oct

@@ -10,6 +10,7 @@
 #include "stitch.hpp"
 #include "opform.hpp"
 #include "estimate.hpp"
+#include "cnn.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -45,6 +46,7 @@ struct oct_plan {
     float* band_w;
     float* win;   // window / N
     const float* win_override;   // caller's device window / N (dispersion search), or null
+    long long win_fstride;       // floats between per-file override windows (0: one window)
     float2* tw;   // (32, 64) transform twiddles [k1][n2]
     int waves;    // workgroup shape of the fused kernel: 8 (LDS twiddles) or 4
 };
@@ -233,6 +235,7 @@ int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files,
     a.band_start = p->band_start;
     a.band_w = p->band_w;
     a.win = p->win_override ? p->win_override : p->win;
+    a.win_fstride = p->win_override ? p->win_fstride : 0;
     a.tw = p->tw;
     a.out = out;
     a.M = (long long)n_files * nX;
@@ -405,6 +408,18 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
 int oct_plan_use_window(oct_plan* p, const float* dev_window) {
     if (!p) return fail(OCT_EINVAL, "null plan");
     p->win_override = dev_window;
+    p->win_fstride = 0;
+    return OCT_OK;
+}
+
+int oct_plan_use_windows(oct_plan* p, const float* dev_windows, long long file_stride) {
+    if (!p) return fail(OCT_EINVAL, "null plan");
+    if (file_stride < 0 || (file_stride > 0 && file_stride < 2LL * p->n))
+        return fail(OCT_EINVAL, "window stride %lld (0 or >= 2N)", file_stride);
+    if (reinterpret_cast<uintptr_t>(dev_windows) % 8 || file_stride % 2)
+        return fail(OCT_EINVAL, "windows must be 8-byte aligned complex pairs");
+    p->win_override = dev_windows;
+    p->win_fstride = dev_windows ? file_stride : 0;
     return OCT_OK;
 }
 
@@ -584,6 +599,164 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
         case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
         default: hipLaunchKernelGGL(oct::opgemm_kernel<float>, dim3(grid), dim3(256), 0, s, a); break;
     }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+
+// ---------------------------------------------------------------- learned estimator
+static int conv_shape_ok(int n, int ci, int h, int w, int co) {
+    return n > 0 && ci > 0 && co > 0 && h >= 2 && w >= 2 && (h % 2) == 0 && (w % 2) == 0;
+}
+
+int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
+                     int wd, int co, void* stream) {
+    if (!x || !w || !bias || !y) return fail(OCT_EINVAL, "null argument");
+    if (!conv_shape_ok(n, ci, h, wd, co)) return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d", n, ci, h, wd, co);
+    if (co % oct::kConvCoB) return fail(OCT_EINVAL, "output channels %d not a multiple of %d", co, oct::kConvCoB);
+    const long long P = (long long)n * (h / 2) * (wd / 2);
+    const long long g = (P + oct::kCnnThreads - 1) / oct::kCnnThreads;
+    if (g > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(oct::conv_fwd_kernel, dim3((unsigned)g, co / oct::kConvCoB), dim3(oct::kCnnThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), x, w, bias, y, n, ci, h, wd, co);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_conv_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci, int h,
+                          int wd, int co, void* stream) {
+    if (!dpre || !w || !dx) return fail(OCT_EINVAL, "null argument");
+    if (!conv_shape_ok(n, ci, h, wd, co)) return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d", n, ci, h, wd, co);
+    if (ci % oct::kConvCiB) return fail(OCT_EINVAL, "input channels %d not a multiple of %d", ci, oct::kConvCiB);
+    const long long P = (long long)n * (h / 2) * (wd / 2);
+    const long long g = (P + oct::kCnnThreads - 1) / oct::kCnnThreads;
+    if (g > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(oct::conv_bwd_data_kernel, dim3((unsigned)g, ci / oct::kConvCiB), dim3(oct::kCnnThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), dpre, w, xmask, dx, n, ci, h, wd, co);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+static int conv_splits(int n, int ci, int h, int wd, int co) {
+    const long long P = (long long)n * (h / 2) * (wd / 2);
+    long long s = (2048 + (long long)co * ci - 1) / ((long long)co * ci);
+    s = std::min<long long>(s, std::max<long long>(1, P / 1024));
+    return (int)std::max<long long>(1, std::min<long long>(s, 1024));
+}
+
+size_t oct_cnn_conv_bwd_weight_workspace_bytes(int n, int ci, int h, int wd, int co) {
+    if (!conv_shape_ok(n, ci, h, wd, co)) return 0;
+    return sizeof(float) * 10 * (size_t)conv_splits(n, ci, h, wd, co) * (size_t)co * (size_t)ci;
+}
+
+int oct_cnn_conv_bwd_weight(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int ci, int h, int wd,
+                            int co, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!dpre || !x || !dw || !db || !workspace) return fail(OCT_EINVAL, "null argument");
+    if (!conv_shape_ok(n, ci, h, wd, co)) return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d", n, ci, h, wd, co);
+    if ((long long)co * ci > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "too many channels");
+    const int S = conv_splits(n, ci, h, wd, co);
+    if (workspace_bytes < oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, wd, co))
+        return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes,
+                    oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, wd, co));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    float* part = static_cast<float*>(workspace);
+    hipLaunchKernelGGL(oct::conv_bwd_weight_kernel, dim3(co * ci, S), dim3(oct::kCnnThreads), 0, s, dpre, x, part, n,
+                       ci, h, wd, co);
+    HIP_TRY(hipGetLastError());
+    const long long tot = (long long)co * ci * 9 + co;
+    hipLaunchKernelGGL(oct::conv_bwd_weight_reduce_kernel, dim3((unsigned)((tot + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, s, part, dw, db, ci, co, S);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_fc_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y_relu_bf16, float* y_linear,
+                   int n, int k, int o, void* stream) {
+    if (!x || !w || !bias) return fail(OCT_EINVAL, "null argument");
+    if ((y_relu_bf16 != nullptr) == (y_linear != nullptr)) return fail(OCT_EINVAL, "exactly one output (ReLU bf16 or linear f32)");
+    if (n <= 0 || k <= 0 || o <= 0) return fail(OCT_EINVAL, "bad fc shape (%d, %d) -> %d", n, k, o);
+    const long long tot = (long long)n * o;
+    hipLaunchKernelGGL(oct::fc_fwd_kernel, dim3((unsigned)((tot + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), x, w, bias, y_relu_bf16, y_linear,
+                       n, k, o);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_fc_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int k, int o,
+                        void* stream) {
+    if (!dpre || !w || !dx) return fail(OCT_EINVAL, "null argument");
+    if (n <= 0 || k <= 0 || o <= 0) return fail(OCT_EINVAL, "bad fc shape (%d, %d) -> %d", n, k, o);
+    const long long tot = (long long)n * k;
+    hipLaunchKernelGGL(oct::fc_bwd_data_kernel, dim3((unsigned)((tot + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), dpre, w, xmask, dx, n, k, o);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_fc_bwd_weight(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int k, int o,
+                          void* stream) {
+    if (!dpre || !x || !dw || !db) return fail(OCT_EINVAL, "null argument");
+    if (n <= 0 || k <= 0 || o <= 0) return fail(OCT_EINVAL, "bad fc shape (%d, %d) -> %d", n, k, o);
+    const long long tot = (long long)o * k + o;
+    hipLaunchKernelGGL(oct::fc_bwd_weight_kernel, dim3((unsigned)((tot + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), dpre, x, dw, db, n, k, o);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_gauss_nll(const float* head, const float* target, const float* weights, float* loss, float* dhead, int n,
+                      int n_targets, void* stream) {
+    if (!head || !target || !weights || !loss || !dhead) return fail(OCT_EINVAL, "null argument");
+    if (n <= 0 || n_targets <= 0 || n_targets > oct::kNllMaxT) return fail(OCT_EINVAL, "bad loss shape (%d, %d)", n, n_targets);
+    hipLaunchKernelGGL(oct::gauss_nll_kernel, dim3(1), dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream),
+                       head, target, weights, loss, dhead, n, n_targets);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_cnn_adam(float* params, const float* grads, float* m, float* v, uint16_t* params_bf16, long long count,
+                 float lr, float beta1, float beta2, float eps, float weight_decay, int step, void* stream) {
+    if (!params || !grads || !m || !v || !params_bf16) return fail(OCT_EINVAL, "null argument");
+    if (count <= 0 || step < 1) return fail(OCT_EINVAL, "bad count %lld / step %d", count, step);
+    const float c1 = (float)(1.0 - std::pow((double)beta1, step));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, step));
+    hipLaunchKernelGGL(oct::adam_kernel, dim3((unsigned)((count + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), params, grads, m, v, params_bf16,
+                       count, lr, beta1, beta2, eps, weight_decay, c1, c2);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride, int n_rows, int x0, int z0, int zpool,
+                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, int channel,
+                    void* stream) {
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_bscans <= 0) return OCT_OK;
+    if (height <= 0 || width <= 0 || zpool <= 0 || x0 < 0 || z0 < 0 || x0 + width > n_x ||
+        z0 + (long long)height * zpool > n_rows || n_rows > x_stride)
+        return fail(OCT_EINVAL, "crop x [%d, %d) z [%d, %lld) outside (%d, %d)", x0, x0 + width, z0,
+                    z0 + (long long)height * zpool, n_x, n_rows);
+    if (channel < 0 || channel >= channels) return fail(OCT_EINVAL, "channel %d of %d", channel, channels);
+    if (!(scale_db > 0.f)) return fail(OCT_EINVAL, "scale must be positive");
+    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    const long long hw = (long long)height * width;
+    hipLaunchKernelGGL(oct::logmag_crop_kernel, dim3((unsigned)((hw + oct::kCnnThreads - 1) / oct::kCnnThreads), n_bscans),
+                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, x0, z0, zpool,
+                       height, width, offset_db, 1.0f / scale_db, out, channels, channel);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, double* out, void* stream) {
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_bscans <= 0) return OCT_OK;
+    if (n_x <= 0 || n_x > oct::kMedMaxX || n_z <= 0 || n_z > x_stride)
+        return fail(OCT_EINVAL, "median shape (%d x %d, stride %lld; at most %d A-scans)", n_x, n_z, x_stride, oct::kMedMaxX);
+    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    const size_t lds = sizeof(float) * oct::kMedZ * (size_t)(n_x + 1);
+    hipLaunchKernelGGL(oct::depth_median_kernel, dim3((n_z + oct::kMedZ - 1) / oct::kMedZ, n_bscans),
+                       dim3(oct::kCnnThreads), lds, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, n_z, out);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

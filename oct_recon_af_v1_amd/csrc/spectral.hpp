@@ -38,6 +38,7 @@ struct SpectralArgs {
     const int* band_start;   // (N) first source sample of each output's tap window
     const float* band_w;     // (N, TP) tap weights
     const float* win;        // (N, 2) complex window scaled by 1/N
+    long long win_fstride;   // floats between the windows of consecutive raw files (0: one window)
     const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), [k1][n2] (W = 4 variant)
     void* out;
     long long M;             // A-scans = B * nX
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         if (wave < nvalid) {
             cf* B = &buf[wave][0];
             const float* E = reinterpret_cast<const float*>(B) + kN;
-            const float2* wg = reinterpret_cast<const float2*>(a.win);
+            const float2* wg = reinterpret_cast<const float2*>(a.win + (p / a.nX) * a.win_fstride);
             cf v[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {

@@ -37,23 +37,59 @@ class HipDispersionScorer:
         self.torch = xp.torch
         self.sg0: SpectralGeometry = build_spectral_geometry(lambda_nm, 0.0, n_medium)
         self.sp = HipSpectralProcessor(self.sg0, apod_size, xp, apod_mode=apod_mode)
-        self.raw = self.sp._device_raw(raw)
         N = self.sg0.n
         hann = 0.5 * (1 - np.cos(2 * np.pi * np.arange(N) / (N - 1)))
         self.hann = hann / np.sqrt(np.mean(hann ** 2))
         k_eq = 2 * np.pi / self.sg0.lambda_eq_nm
         self.dk2 = (k_eq - k_eq.mean()) ** 2
+        self.nz = N // 2
+        self._mag = None
+        self.n_evals = 0
+        self.set_raw(raw)
+
+    def set_raw(self, raw):
+        """Score another set of B-scans (same spectrometer)."""
+        self.raw = self.sp._device_raw(raw)
         self.B = self.raw.shape[0]
         self.nX = self.sp.n_scan_lines(self.raw.shape[1])
-        self.nz = N // 2
-        self._mag = self.torch.empty((self.B, self.nX, self.nz), dtype=self.torch.float32, device=xp.device)
-        self.n_evals = 0
+        shape = (self.B, self.nX, self.nz)
+        if self._mag is None or tuple(self._mag.shape) != shape:
+            self._mag = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
 
     def windows(self, betas):
         """(len(betas), N, 2) float32 windows scaled by 1/N (float64 phase)."""
         betas = np.atleast_1d(np.asarray(betas, float))
         w = self.hann[None, :] * np.exp(-1j * betas[:, None] * self.dk2[None, :]) / self.sg0.n
         return np.ascontiguousarray(np.stack([w.real, w.imag], -1).astype(np.float32))
+
+    def _metric(self, out_row):
+        _lib.check(_lib.lib().oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX,
+                                                 TOP_FRACTION, out_row.data_ptr(), self.xp.stream()),
+                   "oct_topfrac_metric")
+
+    def magnitude_at(self, betas, out=None):
+        """|A| (B, nX, N/2) with one candidate for all B-scans (scalar) or one per B-scan."""
+        betas = np.atleast_1d(np.asarray(betas, float))
+        if betas.size not in (1, self.B):
+            raise ValueError(f"{betas.size} candidates for {self.B} B-scans")
+        win = self.torch.from_numpy(self.windows(betas)).to(self.xp.device)
+        return self.magnitude_with(win, out)
+
+    def magnitude_with(self, win, out=None):
+        """|A| with device windows win: (N, 2) for every B-scan or (B, N, 2) one per B-scan."""
+        per_file = win.dim() == 3
+        if per_file and win.shape[0] != self.B:
+            raise ValueError(f"{win.shape[0]} windows for {self.B} B-scans")
+        out = self._mag if out is None else out
+        L = _lib.lib()
+        try:
+            _lib.check(L.oct_plan_use_windows(self.sp.plan.handle, win.data_ptr(), 2 * self.sg0.n if per_file else 0),
+                       "oct_plan_use_windows")
+            self.sp.run(self.raw, _lib.OCT_OUT_MAG, 0, self.nz, out=out)
+        finally:
+            L.oct_plan_use_window(self.sp.plan.handle, None)
+        self._keep = win                  # alive until the next call (the launch is asynchronous)
+        return out
 
     def scores_many(self, betas):
         """(G, B) float64 scores for a batch of candidates, one host synchronisation."""
@@ -66,9 +102,7 @@ class HipDispersionScorer:
             for g in range(len(betas)):
                 _lib.check(L.oct_plan_use_window(self.sp.plan.handle, win[g].data_ptr()), "oct_plan_use_window")
                 self.sp.run(self.raw, _lib.OCT_OUT_MAG, 0, self.nz, out=self._mag)
-                _lib.check(L.oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX,
-                                                TOP_FRACTION, out[g].data_ptr(), self.xp.stream()),
-                           "oct_topfrac_metric")
+                self._metric(out[g])
         finally:
             L.oct_plan_use_window(self.sp.plan.handle, None)
         self.n_evals += len(betas)
@@ -76,9 +110,61 @@ class HipDispersionScorer:
         del win
         return res
 
+    def scores_per_file(self, betas):
+        """(B,) float64: B-scan b scored at its own candidate betas[b] (one launch)."""
+        out = self.torch.empty((self.B,), dtype=self.torch.float64, device=self.xp.device)
+        self.magnitude_at(np.asarray(betas, float).reshape(self.B))
+        self._metric(out)
+        self.n_evals += 1
+        return out.cpu().numpy()
+
     def scores(self, beta, idx=None):
         s = self.scores_many([beta])[0]
         return s if idx is None else s[idx:idx + 1]
+
+
+def depth_median(mag, xp, out=None):
+    """(B, nX, nZ) float32 device magnitudes -> (B, nZ) float64 device median over x (exact, as
+    numpy.median of the float32 values in float64; octrecon/estimation/focus.py:213-216)."""
+    torch = xp.torch
+    B, nX, nZ = mag.shape
+    if mag.stride(2) != 1 or mag.stride(1) < nZ or mag.stride(0) != nX * mag.stride(1):
+        raise ValueError("magnitudes must be (B, nX, rows) with contiguous rows")
+    out = torch.empty((B, nZ), dtype=torch.float64, device=xp.device) if out is None else out
+    _lib.check(_lib.lib().oct_depth_median(mag.data_ptr(), B, nX, mag.stride(1), nZ, out.data_ptr(), xp.stream()),
+               "oct_depth_median")
+    return out
+
+
+def search_per_bscan(scorer: HipDispersionScorer, lo: float, hi: float, n_grid: int = 41, iters: int = 24):
+    """Each B-scan's own sharpness maximum over [lo, hi]: the reference's coarse grid
+    (octrecon/estimation/dispersion.py:206-214), then the bracket around each B-scan's best grid
+    point refined by golden-section search, all B-scans advanced together (one launch per step).
+    Returns (beta (B,), score (B,), grid scores (G, B))."""
+    grid = np.linspace(lo, hi, int(n_grid))
+    S = scorer.scores_many(grid)
+    gi = np.argmax(S, 0)
+    a = grid[np.maximum(gi - 1, 0)].astype(float)
+    b = grid[np.minimum(gi + 1, len(grid) - 1)].astype(float)
+    r = (np.sqrt(5.0) - 1) / 2
+    c = b - r * (b - a)
+    d = a + r * (b - a)
+    fc = scorer.scores_per_file(c)
+    fd = scorer.scores_per_file(d)
+    for _ in range(iters):
+        left = fc >= fd                           # maximum in [a, d]
+        b = np.where(left, d, b)
+        a = np.where(left, a, c)
+        nc = np.where(left, b - r * (b - a), d)
+        nd = np.where(left, c, a + r * (b - a))
+        fnew = scorer.scores_per_file(np.where(left, nc, nd))
+        fc, fd = np.where(left, fnew, fd), np.where(left, fc, fnew)
+        c, d = nc, nd
+    x = 0.5 * (c + d)
+    fx = np.maximum(fc, fd)
+    best_grid = S[gi, np.arange(S.shape[1])]
+    use_grid = best_grid > fx
+    return np.where(use_grid, grid[gi], x), np.where(use_grid, best_grid, fx), S
 
 
 def estimate_dispersion(scorer: HipDispersionScorer, initial: float, bounds=None, n_grid: int = 41, log=None):

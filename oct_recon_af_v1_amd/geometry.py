@@ -214,13 +214,15 @@ def build_spectral_geometry(lambda_nm, dispersion_quadratic_term: float, n_mediu
                             bstart, band_w, float(dispersion_quadratic_term), float(n_medium))
 
 
-def beta_from_index_units(beta_idx: float, lambda_nm) -> float:
+def beta_from_index_units(beta_idx, lambda_nm):
     """Dispersion given as phase per squared equispaced-k sample index,
     phi = beta_idx * ((k - k0)/dk)^2, converted to the reference's nm^2/rad term
-    beta = beta_idx / dk^2 with dk the equispaced-k step."""
+    beta = beta_idx / dk^2 with dk the equispaced-k step (float in, float out; arrays elementwise)."""
     k = 2 * np.pi / np.asarray(lambda_nm, float)
     dk = (k.max() - k.min()) / (len(k) - 1)
-    return float(beta_idx) / dk ** 2
+    if np.ndim(beta_idx) == 0:
+        return float(beta_idx) / dk ** 2
+    return np.asarray(beta_idx, float) / dk ** 2
 
 
 # --------------------------------------------------------------------------------------

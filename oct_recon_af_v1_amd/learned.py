@@ -1,0 +1,376 @@
+"""Learned dispersion (beta) and focus-depth estimator: a small BF16 CNN trained and run by this
+package's own HIP kernels (oct_cnn_* in liboct_hip).
+
+It stands in for the dispersion sharpness search (octrecon/estimation/dispersion.py:154-267) and
+the confocal focus detector (octrecon/estimation/focus.py:191-218, 221-279).  Both stay: they make
+the training labels, judge the network's error on held-out data, and are the fallback for
+B-scans where the network's own predicted uncertainty is above a threshold.
+
+Input: two 256 x 256 log-magnitude images of the central 256 A-scans of a B-scan, reconstructed
+with two probe dispersions (beta = -1e-4 and +1e-4 in index units; the residual blur of the two
+differs in size, which fixes the sign of beta that a single image leaves open), depth rows
+0..1023 max-pooled by 4.  Output: beta, focus depth and the log variance of each (a
+heteroscedastic Gaussian head), the variance being the confidence.
+
+Network: 6 x (conv 3x3 stride 2 + ReLU) 2-16-32-32-64-64-64 channels (256 -> 4 pixels), then
+fully connected 1024 -> 128 (ReLU) -> 4; 238 k parameters, bf16 weights and activations, float32
+master weights, gradients and accumulation, Adam.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .geometry import beta_from_index_units
+
+ARCH_CONV = ((2, 16), (16, 32), (32, 32), (32, 64), (64, 64), (64, 64))
+IMG = 256
+FC_HIDDEN = 128
+N_TARGETS = 2                   # beta, focus
+HEAD = 2 * N_TARGETS            # means, log variances
+FLAT = ARCH_CONV[-1][1] * (IMG >> len(ARCH_CONV)) ** 2
+
+
+@dataclass
+class InputSpec:
+    n_x_crop: int = IMG
+    zpool: int = 4
+    n_z: int = 1024
+    probes_idx: tuple = (-1e-4, 1e-4)     # probe dispersions, index units (rad per squared k sample)
+    offset_db: float = 10.0
+    scale_db: float = 20.0
+    beta_scale: float = 2e-4              # target t_beta = beta_idx / beta_scale
+    focus_center: float = 512.0           # target t_focus = (z_bin - center) / scale
+    focus_scale: float = 256.0
+
+
+class ParamLayout:
+    """Flat parameter vector: per layer weight then bias, each slice aligned to 64 elements."""
+
+    def __init__(self):
+        self.entries = {}
+        off = 0
+
+        def add(name, shape):
+            nonlocal off
+            self.entries[name] = (tuple(shape), off)
+            off += (int(np.prod(shape)) + 63) // 64 * 64
+
+        for i, (ci, co) in enumerate(ARCH_CONV):
+            add(f"conv{i}.w", (co, ci, 3, 3))
+            add(f"conv{i}.b", (co,))
+        add("fc0.w", (FC_HIDDEN, FLAT))
+        add("fc0.b", (FC_HIDDEN,))
+        add("fc1.w", (HEAD, FC_HIDDEN))
+        add("fc1.b", (HEAD,))
+        self.total = off
+        self.n_params = sum(int(np.prod(s)) for s, _ in self.entries.values())
+
+    def init(self, seed: int = 0) -> np.ndarray:
+        """He-uniform weights, zero biases, a small head."""
+        rng = np.random.default_rng(seed)
+        p = np.zeros(self.total, np.float32)
+        for name, (shape, off) in self.entries.items():
+            if name.endswith(".w"):
+                fan_in = int(np.prod(shape[1:]))
+                lim = math.sqrt(6.0 / fan_in) * (0.1 if name == "fc1.w" else 1.0)
+                p[off:off + int(np.prod(shape))] = rng.uniform(-lim, lim, int(np.prod(shape)))
+        return p
+
+    def unflatten(self, flat) -> dict:
+        return {n: np.asarray(flat[o:o + int(np.prod(s))]).reshape(s) for n, (s, o) in self.entries.items()}
+
+
+def _need(t, numel, what):
+    """Host-side extent check before a launch: the kernel addresses exactly `numel` elements."""
+    if t.numel() != numel or not t.is_contiguous():
+        raise ValueError(f"{what}: tensor {tuple(t.shape)} ({t.numel()} elements), kernel needs {numel}")
+
+
+class EstimatorNet:
+    """Device network state (master float32 weights, bf16 copy, Adam moments) and the forward /
+    backward sequences over the oct_cnn_* kernels."""
+
+    def __init__(self, xp, params: np.ndarray | None = None, seed: int = 0):
+        self.xp = xp
+        torch = self.torch = xp.torch
+        self.layout = ParamLayout()
+        p = self.layout.init(seed) if params is None else np.asarray(params, np.float32)
+        if p.shape != (self.layout.total,):
+            raise ValueError(f"parameter vector of {p.shape}, expected ({self.layout.total},)")
+        dev = xp.device
+        self.p32 = torch.tensor(p, dtype=torch.float32, device=dev)
+        self.pbf = self.p32.to(torch.bfloat16)
+        self.grad = torch.zeros_like(self.p32)
+        self.m = torch.zeros_like(self.p32)
+        self.v = torch.zeros_like(self.p32)
+        self.steps = 0
+        self._bufs = {}
+
+    # -- slices
+    def _slice(self, flat, name):
+        shape, off = self.layout.entries[name]
+        return flat[off:off + int(np.prod(shape))]
+
+    def _ptr(self, flat, name):
+        return self._slice(flat, name).data_ptr()
+
+    def params_numpy(self) -> np.ndarray:
+        return self.p32.cpu().numpy()
+
+    # -- buffers per batch size
+    def _buffers(self, n):
+        if n in self._bufs:
+            return self._bufs[n]
+        torch = self.torch
+        dev = self.xp.device
+        acts, dpre, ws = [], [], []
+        h = IMG
+        L = _lib.lib()
+        for ci, co in ARCH_CONV:
+            acts.append(torch.empty((n, co, h // 2, h // 2), dtype=torch.bfloat16, device=dev))
+            dpre.append(torch.empty((n, co, h // 2, h // 2), dtype=torch.float32, device=dev))
+            nb = L.oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, h, co)
+            ws.append(torch.empty((max(1, nb // 4),), dtype=torch.float32, device=dev))
+            h //= 2
+        b = dict(acts=acts, dpre=dpre, ws=ws,
+                 hid=torch.empty((n, FC_HIDDEN), dtype=torch.bfloat16, device=dev),
+                 dhid=torch.empty((n, FC_HIDDEN), dtype=torch.float32, device=dev),
+                 head=torch.empty((n, HEAD), dtype=torch.float32, device=dev),
+                 dhead=torch.empty((n, HEAD), dtype=torch.float32, device=dev),
+                 loss=torch.empty((1 + N_TARGETS,), dtype=torch.float32, device=dev))
+        self._bufs[n] = b
+        return b
+
+    # -- passes
+    def forward(self, x):
+        """x (n, 2, 256, 256) bf16 device tensor -> head (n, 4) float32 (kept for backward)."""
+        torch = self.torch
+        if x.dtype != torch.bfloat16 or tuple(x.shape[1:]) != (ARCH_CONV[0][0], IMG, IMG) or not x.is_contiguous():
+            raise ValueError(f"input must be contiguous bf16 (n, {ARCH_CONV[0][0]}, {IMG}, {IMG}), got "
+                             f"{x.dtype} {tuple(x.shape)}")
+        n = x.shape[0]
+        b = self._buffers(n)
+        L = _lib.lib()
+        s = self.xp.stream()
+        cur, h = x, IMG
+        for i, (ci, co) in enumerate(ARCH_CONV):
+            _need(cur, n * ci * h * h, f"conv{i} input")
+            _need(b["acts"][i], n * co * (h // 2) ** 2, f"conv{i} output")
+            _lib.check(L.oct_cnn_conv_fwd(cur.data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
+                                          self._ptr(self.p32, f"conv{i}.b"), b["acts"][i].data_ptr(), n, ci, h, h, co,
+                                          s), f"conv{i} forward")
+            cur, h = b["acts"][i], h // 2
+        _lib.check(L.oct_cnn_fc_fwd(cur.data_ptr(), self._ptr(self.pbf, "fc0.w"), self._ptr(self.p32, "fc0.b"),
+                                    b["hid"].data_ptr(), None, n, FLAT, FC_HIDDEN, s), "fc0 forward")
+        _lib.check(L.oct_cnn_fc_fwd(b["hid"].data_ptr(), self._ptr(self.pbf, "fc1.w"), self._ptr(self.p32, "fc1.b"),
+                                    None, b["head"].data_ptr(), n, FC_HIDDEN, HEAD, s), "fc1 forward")
+        b["x"] = x
+        return b["head"]
+
+    def loss(self, n, target, weights):
+        """Gaussian NLL of the last forward's head against target (n, 2) float32 device; fills
+        dhead; returns the (3,) device tensor (loss, mse_beta, mse_focus)."""
+        b = self._buffers(n)
+        _lib.check(_lib.lib().oct_cnn_gauss_nll(b["head"].data_ptr(), target.data_ptr(), weights.data_ptr(),
+                                                b["loss"].data_ptr(), b["dhead"].data_ptr(), n, N_TARGETS,
+                                                self.xp.stream()), "loss")
+        return b["loss"]
+
+    def backward(self, n, dhead=None):
+        """Gradients of every parameter into self.grad from dhead (default: the loss's)."""
+        b = self._buffers(n)
+        L = _lib.lib()
+        s = self.xp.stream()
+        dh = b["dhead"] if dhead is None else dhead
+        g = self.grad
+        _lib.check(L.oct_cnn_fc_bwd_weight(dh.data_ptr(), b["hid"].data_ptr(), self._ptr(g, "fc1.w"),
+                                           self._ptr(g, "fc1.b"), n, FC_HIDDEN, HEAD, s), "fc1 backward (weights)")
+        _lib.check(L.oct_cnn_fc_bwd_data(dh.data_ptr(), self._ptr(self.pbf, "fc1.w"), b["hid"].data_ptr(),
+                                         b["dhid"].data_ptr(), n, FC_HIDDEN, HEAD, s), "fc1 backward (data)")
+        last = len(ARCH_CONV) - 1
+        _lib.check(L.oct_cnn_fc_bwd_weight(b["dhid"].data_ptr(), b["acts"][last].data_ptr(), self._ptr(g, "fc0.w"),
+                                           self._ptr(g, "fc0.b"), n, FLAT, FC_HIDDEN, s), "fc0 backward (weights)")
+        _lib.check(L.oct_cnn_fc_bwd_data(b["dhid"].data_ptr(), self._ptr(self.pbf, "fc0.w"),
+                                         b["acts"][last].data_ptr(), b["dpre"][last].data_ptr(), n, FLAT, FC_HIDDEN,
+                                         s), "fc0 backward (data)")
+        for i in range(last, -1, -1):
+            ci, co = ARCH_CONV[i]
+            hin = IMG >> i                          # input side of conv i
+            xin = b["x"] if i == 0 else b["acts"][i - 1]
+            ws = b["ws"][i]
+            _need(xin, n * ci * hin * hin, f"conv{i} input")
+            _need(b["dpre"][i], n * co * (hin // 2) ** 2, f"conv{i} output gradient")
+            if i > 0:
+                _need(b["dpre"][i - 1], n * ci * hin * hin, f"conv{i} input gradient")
+            _lib.check(L.oct_cnn_conv_bwd_weight(b["dpre"][i].data_ptr(), xin.data_ptr(), self._ptr(g, f"conv{i}.w"),
+                                                 self._ptr(g, f"conv{i}.b"), n, ci, hin, hin, co, ws.data_ptr(),
+                                                 ws.numel() * 4, s), f"conv{i} backward (weights)")
+            if i > 0:
+                _lib.check(L.oct_cnn_conv_bwd_data(b["dpre"][i].data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
+                                                   xin.data_ptr(), b["dpre"][i - 1].data_ptr(), n, ci, hin, hin, co,
+                                                   s), f"conv{i} backward (data)")
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        self.steps += 1
+        _lib.check(_lib.lib().oct_cnn_adam(self.p32.data_ptr(), self.grad.data_ptr(), self.m.data_ptr(),
+                                           self.v.data_ptr(), self.pbf.data_ptr(), self.layout.total, lr, beta1,
+                                           beta2, eps, weight_decay, self.steps, self.xp.stream()), "adam")
+
+
+# --------------------------------------------------------------------------- inputs and labels
+class InputBuilder:
+    """Network inputs of raw B-scans: the two probe reconstructions, cropped and log-scaled on the
+    device (oct_logmag_crop)."""
+
+    def __init__(self, scorer, spec: InputSpec = InputSpec()):
+        self.scorer = scorer
+        self.spec = spec
+        lam = scorer.sg0.lambda_nm
+        self.probe_ref = [beta_from_index_units(b, lam) for b in spec.probes_idx]
+        self.probe_win = scorer.torch.from_numpy(scorer.windows(self.probe_ref)).to(scorer.xp.device)
+
+    def build(self, raw=None, out=None):
+        """(B, 2, 256, 256) bf16 for the scorer's B-scans (or `raw`, which becomes its set)."""
+        sc, sp = self.scorer, self.spec
+        torch = sc.torch
+        if raw is not None:
+            sc.set_raw(raw)
+        if sc.nX < sp.n_x_crop or sp.n_z * 1 > sc.nz or sp.n_z // sp.zpool != IMG:
+            raise ValueError(f"B-scans of {sc.nX} A-scans x {sc.nz} rows cannot give the {IMG} x {IMG} crop")
+        B = sc.B
+        out = torch.empty((B, 2, IMG, IMG), dtype=torch.bfloat16, device=sc.xp.device) if out is None else out
+        x0 = (sc.nX - sp.n_x_crop) // 2
+        L = _lib.lib()
+        for c in range(len(self.probe_ref)):
+            mag = sc.magnitude_with(self.probe_win[c])
+            _lib.check(L.oct_logmag_crop(mag.data_ptr(), B, sc.nX, mag.stride(1), sc.nz, x0, 0, sp.zpool, IMG,
+                                         sp.n_x_crop, sp.offset_db, sp.scale_db, out.data_ptr(), 2, c,
+                                         sc.xp.stream()), "oct_logmag_crop")
+        return out
+
+
+def focus_from_profiles(profiles, search_px=None, guess=None):
+    """Focus (0-based depth bin) and prominence (dB) of each depth profile by the reference's
+    confocal detector (octrecon/estimation/focus.py:191-206); nan where no peak is found.
+    Without a guess the whole depth range is searched."""
+    from .estimation import detect_peak_near
+    P = np.asarray(profiles, float)
+    n = P.shape[-1]
+    out = np.full(P.shape[0], np.nan)
+    prom = np.zeros(P.shape[0])
+    for i, prof in enumerate(P):
+        g = n / 2 if guess is None else float(np.atleast_1d(guess)[min(i, np.size(guess) - 1)]) + 1
+        pix, pr = detect_peak_near(prof, g, search_px=n if search_px is None else search_px)
+        if np.isfinite(pix):
+            out[i] = pix - 1
+            prom[i] = pr
+    return out, prom
+
+
+def make_labels(scorer, lo_idx=-2.5e-4, hi_idx=2.5e-4, n_grid=41, iters=24):
+    """Search labels for the scorer's B-scans: per-B-scan sharpness maximum (beta, index units),
+    its score, and the focus detector on the depth profile reconstructed at that beta."""
+    from .estimation import depth_median, search_per_bscan
+    lam = scorer.sg0.lambda_nm
+    to_ref = beta_from_index_units(1.0, lam)
+    beta_ref, score, _ = search_per_bscan(scorer, lo_idx * to_ref, hi_idx * to_ref, n_grid, iters)
+    mag = scorer.magnitude_at(beta_ref)
+    prof = depth_median(mag, scorer.xp).cpu().numpy()
+    focus, prom = focus_from_profiles(prof)
+    return dict(beta_idx=beta_ref / to_ref, beta_ref=beta_ref, score=score, focus=focus, prominence=prom,
+                profiles=prof)
+
+
+def normalise_targets(beta_idx, focus, spec: InputSpec = InputSpec()):
+    return np.stack([np.asarray(beta_idx) / spec.beta_scale,
+                     (np.asarray(focus) - spec.focus_center) / spec.focus_scale], -1).astype(np.float32)
+
+
+def denormalise(head, spec: InputSpec = InputSpec()):
+    """head (n, 4) numpy -> beta_idx, focus bin, sigma_beta_idx, sigma_focus (bins)."""
+    h = np.asarray(head, float)
+    beta = h[:, 0] * spec.beta_scale
+    focus = h[:, 1] * spec.focus_scale + spec.focus_center
+    sb = np.exp(0.5 * h[:, 2]) * spec.beta_scale
+    sf = np.exp(0.5 * h[:, 3]) * spec.focus_scale
+    return beta, focus, sb, sf
+
+
+# --------------------------------------------------------------------------- training
+def train(net: EstimatorNet, X, Y, epochs: int = 20, batch: int = 64, lr: float = 2e-3, weight_decay: float = 1e-4,
+          weights=(1.0, 1.0), warmup: int = 200, seed: int = 0, log=None, val=None):
+    """Adam with linear warm-up and cosine decay over `epochs` passes of (X, Y) device tensors
+    (X (M, 2, 256, 256) bf16, Y (M, 2) float32 normalised targets); drops the last partial
+    batch.  Returns the per-epoch history (train loss, validation MSEs)."""
+    torch = net.torch
+    dev = net.xp.device
+    M = X.shape[0]
+    steps_per_epoch = M // batch
+    if steps_per_epoch < 1:
+        raise ValueError(f"{M} samples for batches of {batch}")
+    total = epochs * steps_per_epoch
+    w = torch.tensor(weights, dtype=torch.float32, device=dev)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    xb = torch.empty((batch,) + tuple(X.shape[1:]), dtype=X.dtype, device=dev)
+    yb = torch.empty((batch, Y.shape[1]), dtype=Y.dtype, device=dev)
+    hist = []
+    t0 = time.perf_counter()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(M, generator=gen, device=dev)
+        acc = torch.zeros((1 + N_TARGETS,), dtype=torch.float32, device=dev)
+        for k in range(steps_per_epoch):
+            idx = perm[k * batch:(k + 1) * batch]
+            torch.index_select(X, 0, idx, out=xb)
+            torch.index_select(Y, 0, idx, out=yb)
+            net.forward(xb)
+            acc += net.loss(batch, yb, w)
+            net.backward(batch)
+            f = min(1.0, (step + 1) / warmup) * 0.5 * (1 + math.cos(math.pi * step / total))
+            net.adam(lr * f, weight_decay=weight_decay)
+            step += 1
+        rec = dict(epoch=ep, train=(acc / steps_per_epoch).cpu().numpy().tolist(),
+                   elapsed_s=time.perf_counter() - t0)
+        if val is not None:
+            rec["val_mse"] = evaluate(net, val[0], val[1], batch)
+        hist.append(rec)
+        if log:
+            log(f"epoch {ep}: {rec}")
+    return hist
+
+
+def predict(net: EstimatorNet, X, batch: int = 256):
+    """head (M, 4) numpy for X (M, 2, 256, 256) bf16 device."""
+    torch = net.torch
+    out = []
+    for i in range(0, X.shape[0], batch):
+        xb = X[i:i + batch].contiguous()
+        out.append(net.forward(xb).clone())
+    return torch.cat(out).cpu().numpy()
+
+
+def evaluate(net, X, Y, batch=256):
+    h = predict(net, X, batch)
+    y = Y.cpu().numpy() if hasattr(Y, "cpu") else np.asarray(Y)
+    return np.mean((h[:, :N_TARGETS] - y) ** 2, 0).tolist()
+
+
+def save_params(path, net: EstimatorNet, spec: InputSpec = InputSpec(), meta=None):
+    """Trained weights as an .npz of plain arrays (bf16 weights stored as their uint16 bits and
+    the float32 master copy)."""
+    import json
+    p = net.params_numpy()
+    np.savez_compressed(path, params=p, meta=np.array(json.dumps(dict(meta or {}, spec=spec.__dict__,
+                                                                       n_params=net.layout.n_params))))
+
+
+def load_params(path):
+    import json
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32), json.loads(str(z["meta"]))

@@ -90,3 +90,91 @@ def make_bscans(n_files: int, n_x: int, spec: SpectrumSpec = SpectrumSpec(), bet
         blk = torch.cat([bg, sig], dim=1).clamp_(0, 4095)
         out[f0:f0 + nf] = blk.to(torch.int32).to(torch.uint16)
     return out
+
+
+# --------------------------------------------------------------------------- estimator workload
+@dataclass
+class FocusSpec:
+    """Scattering sample under a confocal gate, for the beta / focus estimator workload.
+
+    Per B-scan: a tissue surface at depth surf (uniform [surf_lo, surf_hi) bins), below it a
+    scattering medium with intensity reflectivity scat_r per depth bin attenuated as
+    exp(-(z - surf) / att_bins); every reflector (the 5 config-0 reflectors and the medium) is
+    weighted by the confocal intensity gain floor + exp(-(z - focus)^2 / (2 width^2)).  The medium
+    is circular complex Gaussian speckle on the integer depth bins, entered as
+    Re(G(k) exp(i beta dq)) with G(k) = sum_z A(z) exp(i 2 pi z kn / N) evaluated exactly at the
+    spectrometer's non-uniform k by a float32 GEMM.
+    """
+    scat_r: float = 2e-5
+    att_bins: float = 1000.0
+    surf_lo: float = 8.0
+    surf_hi: float = 48.0
+    gate_floor: float = 0.1
+    gate_width_bins: float = 8.8
+    n_depth: int = 1024
+
+
+def make_focus_bscans(n_files: int, n_x: int, betas, focus_bins, spec: SpectrumSpec = SpectrumSpec(),
+                      fspec: FocusSpec = FocusSpec(), seed: int = 0, device="cpu", chunk_files: int = 8, out=None):
+    """(n_files, apod + n_x, N) uint16: config-0 B-scans (5 reflectors per A-scan) plus the
+    scattering medium, all under a confocal gate at `focus_bins[f]`, with dispersion `betas[f]`
+    (index units, as make_bscans)."""
+    import torch
+    dev = torch.device(device)
+    lam = spec.lambda_nm()
+    N = spec.n
+    kn, k0n = _k_axes(lam)
+    S = spec.peak * np.exp(-4 * np.log(2) * (lam - spec.center_nm) ** 2 / spec.fwhm_nm ** 2)
+    kn_t = torch.tensor(kn, dtype=torch.float32, device=dev)
+    dq_t = torch.tensor((kn - k0n) ** 2, dtype=torch.float32, device=dev)
+    S_t = torch.tensor(S, dtype=torch.float32, device=dev)
+    nd = fspec.n_depth
+    ph = 2 * np.pi * np.outer(np.arange(nd), kn) / N                       # (nd, N) float64
+    E = torch.tensor(np.concatenate([np.concatenate([np.cos(ph), np.sin(ph)], 1),
+                                     np.concatenate([-np.sin(ph), np.cos(ph)], 1)], 0),
+                     dtype=torch.float32, device=dev)                         # (2 nd, 2 N)
+    lines = spec.apod + n_x
+    if out is None:
+        out = torch.empty((n_files, lines, N), dtype=torch.uint16, device=dev)
+    betas = np.broadcast_to(np.asarray(betas, float), (n_files,))
+    focus_bins = np.broadcast_to(np.asarray(focus_bins, float), (n_files,))
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    R = spec.n_reflectors
+    zgrid = torch.arange(nd, dtype=torch.float32, device=dev)
+    two_pi_over_n = 2 * np.pi / N
+    for f0 in range(0, n_files, chunk_files):
+        nf = min(chunk_files, n_files - f0)
+        bt = torch.tensor(betas[f0:f0 + nf], dtype=torch.float32, device=dev).view(nf, 1, 1)
+        fz = torch.tensor(focus_bins[f0:f0 + nf], dtype=torch.float32, device=dev).view(nf, 1, 1)
+        surf = fspec.surf_lo + (fspec.surf_hi - fspec.surf_lo) * torch.rand((nf, 1, 1), generator=g, device=dev)
+
+        def gate(z):
+            return fspec.gate_floor + torch.exp(-(z - fz) ** 2 / (2 * fspec.gate_width_bins ** 2))
+
+        # medium: amplitude profile (nf, 1, nd) x complex Gaussian speckle (nf, n_x, nd)
+        zz = zgrid.view(1, 1, nd)
+        inten = fspec.scat_r * torch.exp(-(zz - surf) / fspec.att_bins) * (zz >= surf) * gate(zz)
+        amp = torch.sqrt(inten * 0.5)
+        re = torch.randn((nf, n_x, nd), generator=g, device=dev) * amp
+        im = torch.randn((nf, n_x, nd), generator=g, device=dev) * amp
+        G = torch.cat([re, im], -1).view(nf * n_x, 2 * nd) @ E                 # (nf n_x, 2N): Re, Im
+        G = G.view(nf, n_x, 2 * N)
+        cb, sb = torch.cos(bt * dq_t), torch.sin(bt * dq_t)
+        acc = 1.0 + G[..., :N] * cb - G[..., N:] * sb
+        del G, re, im
+        # config-0 reflectors under the gate
+        z = spec.z_lo + (spec.z_hi - spec.z_lo) * torch.rand((nf, n_x, R, 1), generator=g, device=dev)
+        lr = np.log(spec.r_min) + (np.log(spec.r_max) - np.log(spec.r_min)) * torch.rand(
+            (nf, n_x, R, 1), generator=g, device=dev)
+        phi = 2 * np.pi * torch.rand((nf, n_x, R, 1), generator=g, device=dev)
+        a_r = torch.exp(0.5 * lr) * torch.sqrt(gate(z.view(nf, n_x * R, 1)).view(nf, n_x, R, 1))
+        for r in range(R):
+            phr = (two_pi_over_n * z[:, :, r]) * kn_t + phi[:, :, r] + bt * dq_t
+            acc += a_r[:, :, r] * torch.cos(phr)
+        rate = torch.clamp(S_t * acc, min=0.0)
+        sig = torch.poisson(rate, generator=g)
+        bg = torch.poisson(S_t.expand(nf, spec.apod, N).contiguous(), generator=g)
+        blk = torch.cat([bg, sig], dim=1).clamp_(0, 4095)
+        out[f0:f0 + nf] = blk.to(torch.int32).to(torch.uint16)
+    return out

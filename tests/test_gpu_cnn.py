@@ -1,0 +1,256 @@
+"""Estimator network kernels (oct_cnn_*, oct_logmag_crop, oct_depth_median, per-file windows)
+against the CPU oracle (oracle/cnn_ref.py, pinned in test_cnn_oracle.py).
+
+Tolerances: forward outputs are bf16: at least 99.5 % bit-identical to the oracle's rounding of
+its float64 result and none more than 1 bf16 ulp away (float32 accumulation order); gradients are
+float32 from identical bf16 inputs: max error <= 2e-5 of the largest magnitude per tensor; the
+whole network: head within 2 % of the oracle's (bf16 emulated), and the chained backward from the
+device's stored activations within 1e-4 relative L2 per tensor; the depth median is exact.
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn_ref as C
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def xp():
+    from oct_recon_af_v1_amd.backend import get_xp
+    return get_xp("hip")[0]
+
+
+def _bf(xp, a):
+    return xp.torch.tensor(C.bf16_round(a)).to(xp.torch.bfloat16).to(xp.device)
+
+
+def _f32(xp, a):
+    return xp.torch.tensor(np.asarray(a, np.float32)).to(xp.device)
+
+
+def _np(t):
+    return t.float().cpu().numpy()
+
+
+def _bf_close(got, ref):
+    """got: bf16 device result as float32; ref: float64 oracle value -> rounded to bf16."""
+    r = C.bf16_round(ref.astype(np.float32))
+    eq = np.mean(got == r)
+    ulp = np.abs(r).astype(np.float64) * 2.0 ** -7 + 1e-30
+    return eq, float(np.max(np.abs(got - r) / ulp))
+
+
+def _rel(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
+
+
+LAYERS = [(2, 16, 256), (16, 32, 128), (32, 32, 64), (32, 64, 32), (64, 64, 16), (64, 64, 8)]
+
+
+@pytest.mark.parametrize("ci,co,h", LAYERS)
+def test_conv_layer_kernels(xp, ci, co, h):
+    from oct_recon_af_v1_amd import _lib
+    print(f"case: conv {ci}->{co} at {h}x{h}", flush=True)
+    torch = xp.torch
+    L = _lib.lib()
+    rng = np.random.default_rng(ci * 1000 + co + h)
+    n = 3
+    x = C.bf16_round(rng.standard_normal((n, ci, h, h)).astype(np.float32))
+    w = C.bf16_round((rng.standard_normal((co, ci, 3, 3)) / np.sqrt(9 * ci)).astype(np.float32))
+    b = (rng.standard_normal(co) * 0.1).astype(np.float32)
+    xd, wd, bd = _bf(xp, x), _bf(xp, w), _f32(xp, b)
+    y = torch.empty((n, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
+    _lib.check(L.oct_cnn_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), n, ci, h, h, co,
+                                  xp.stream()), "conv fwd")
+    pre = C.conv_fwd(x, w, b)
+    eq, worst = _bf_close(_np(y), np.maximum(pre, 0))
+    print(f"forward: {eq:.4%} bit-identical, worst {worst:.2f} ulp", flush=True)
+    assert eq >= 0.995 and worst <= 1.0
+    dpre = rng.standard_normal(pre.shape).astype(np.float32)
+    dd = _f32(xp, dpre)
+    ws_bytes = L.oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, h, co)
+    ws = torch.empty((ws_bytes // 4,), dtype=torch.float32, device=xp.device)
+    dw = torch.empty((co, ci, 3, 3), dtype=torch.float32, device=xp.device)
+    db = torch.empty((co,), dtype=torch.float32, device=xp.device)
+    _lib.check(L.oct_cnn_conv_bwd_weight(dd.data_ptr(), xd.data_ptr(), dw.data_ptr(), db.data_ptr(), n, ci, h, h, co,
+                                         ws.data_ptr(), ws_bytes, xp.stream()), "conv bwd weight")
+    dx_ref, dw_ref, db_ref = C.conv_bwd(x, w, dpre)
+    assert _rel(dw.cpu().numpy(), dw_ref) <= 2e-5 and _rel(db.cpu().numpy(), db_ref) <= 2e-5
+    if ci % 16 == 0:
+        dx = torch.empty((n, ci, h, h), dtype=torch.float32, device=xp.device)
+        _lib.check(L.oct_cnn_conv_bwd_data(dd.data_ptr(), wd.data_ptr(), xd.data_ptr(), dx.data_ptr(), n, ci, h, h,
+                                           co, xp.stream()), "conv bwd data")
+        assert _rel(dx.cpu().numpy(), dx_ref * (x > 0)) <= 2e-5
+        _lib.check(L.oct_cnn_conv_bwd_data(dd.data_ptr(), wd.data_ptr(), None, dx.data_ptr(), n, ci, h, h, co,
+                                           xp.stream()), "conv bwd data (no mask)")
+        assert _rel(dx.cpu().numpy(), dx_ref) <= 2e-5
+
+
+def test_fc_loss_adam_kernels(xp):
+    from oct_recon_af_v1_amd import _lib
+    print("case: fc / loss / adam", flush=True)
+    torch = xp.torch
+    L = _lib.lib()
+    rng = np.random.default_rng(11)
+    n, k, o = 5, 1024, 128
+    x = C.bf16_round(rng.standard_normal((n, k)).astype(np.float32))
+    w = C.bf16_round((rng.standard_normal((o, k)) / 32).astype(np.float32))
+    b = (rng.standard_normal(o) * 0.1).astype(np.float32)
+    xd, wd, bd = _bf(xp, x), _bf(xp, w), _f32(xp, b)
+    yb = torch.empty((n, o), dtype=torch.bfloat16, device=xp.device)
+    yl = torch.empty((n, o), dtype=torch.float32, device=xp.device)
+    _lib.check(L.oct_cnn_fc_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), yb.data_ptr(), None, n, k, o,
+                                xp.stream()), "fc relu")
+    _lib.check(L.oct_cnn_fc_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), None, yl.data_ptr(), n, k, o,
+                                xp.stream()), "fc linear")
+    ref = C.fc_fwd(x, w, b)
+    eq, worst = _bf_close(_np(yb), np.maximum(ref, 0))
+    assert eq >= 0.995 and worst <= 1.0
+    assert _rel(yl.cpu().numpy(), ref) <= 2e-6
+    dpre = rng.standard_normal((n, o)).astype(np.float32)
+    dd = _f32(xp, dpre)
+    dx = torch.empty((n, k), dtype=torch.float32, device=xp.device)
+    dw = torch.empty((o, k), dtype=torch.float32, device=xp.device)
+    db = torch.empty((o,), dtype=torch.float32, device=xp.device)
+    _lib.check(L.oct_cnn_fc_bwd_data(dd.data_ptr(), wd.data_ptr(), xd.data_ptr(), dx.data_ptr(), n, k, o,
+                                     xp.stream()), "fc bwd data")
+    _lib.check(L.oct_cnn_fc_bwd_weight(dd.data_ptr(), xd.data_ptr(), dw.data_ptr(), db.data_ptr(), n, k, o,
+                                       xp.stream()), "fc bwd weight")
+    rdx, rdw, rdb = C.fc_bwd(x, w, dpre)
+    assert _rel(dx.cpu().numpy(), rdx * (x > 0)) <= 2e-5
+    assert _rel(dw.cpu().numpy(), rdw) <= 2e-5 and _rel(db.cpu().numpy(), rdb) <= 2e-5
+    # loss
+    head = rng.standard_normal((n, 4)).astype(np.float32)
+    tgt = rng.standard_normal((n, 2)).astype(np.float32)
+    wt = np.array([1.0, 0.5], np.float32)
+    loss = torch.empty(3, dtype=torch.float32, device=xp.device)
+    dh = torch.empty((n, 4), dtype=torch.float32, device=xp.device)
+    hd, td, wtd = _f32(xp, head), _f32(xp, tgt), _f32(xp, wt)        # held: the launch is asynchronous
+    _lib.check(L.oct_cnn_gauss_nll(hd.data_ptr(), td.data_ptr(), wtd.data_ptr(), loss.data_ptr(), dh.data_ptr(), n,
+                                   2, xp.stream()), "loss")
+    rl, rmse, rdh = C.gauss_nll(head.astype(np.float64), tgt.astype(np.float64), wt)
+    assert abs(loss[0].item() - rl) <= 1e-5 * max(1, abs(rl))
+    assert np.allclose(loss[1:].cpu().numpy(), rmse, rtol=1e-5)
+    assert _rel(dh.cpu().numpy(), rdh) <= 1e-5
+    # adam, 3 steps
+    p0 = rng.standard_normal(1000).astype(np.float32)
+    P, M, V = _f32(xp, p0), torch.zeros(1000, device=xp.device), torch.zeros(1000, device=xp.device)
+    Pb = torch.empty(1000, dtype=torch.bfloat16, device=xp.device)
+    p, m, v = p0.astype(np.float64), np.zeros(1000), np.zeros(1000)
+    for step in (1, 2, 3):
+        g = rng.standard_normal(1000).astype(np.float32)
+        gd = _f32(xp, g)
+        _lib.check(L.oct_cnn_adam(P.data_ptr(), gd.data_ptr(), M.data_ptr(), V.data_ptr(), Pb.data_ptr(),
+                                  1000, 1e-2, 0.9, 0.999, 1e-8, 1e-3, step, xp.stream()), "adam")
+        xp.synchronize()
+        p, m, v = C.adam(p, g.astype(np.float64), m, v, step, 1e-2, wd=1e-3)
+    assert np.max(np.abs(P.cpu().numpy() - p)) <= 1e-5
+    assert np.array_equal(Pb.float().cpu().numpy(), C.bf16_round(P.cpu().numpy()))
+
+
+def test_network_gradients_vs_oracle(xp):
+    """Forward and backward of the whole network on the device against the oracle (bf16 emulated):
+    the same head gradient is propagated through both, and the loss kernel is checked on the
+    device's own head."""
+    from oct_recon_af_v1_amd import learned as LN
+    print("case: whole-network gradients, batch 4", flush=True)
+    torch = xp.torch
+    lay = LN.ParamLayout()
+    p = lay.init(21)
+    p[lay.entries["fc1.w"][1]:lay.entries["fc1.w"][1] + 4 * LN.FC_HIDDEN] *= 10
+    net = LN.EstimatorNet(xp, p)
+    rng = np.random.default_rng(22)
+    x = C.bf16_round(rng.standard_normal((4, 2, LN.IMG, LN.IMG)).astype(np.float32) + 0.3)
+    tgt = rng.standard_normal((4, 2)).astype(np.float32) * 0.5
+    dh = (rng.standard_normal((4, 4)) * 0.1).astype(np.float32)
+    xd, td, wd, dhd = _bf(xp, x), _f32(xp, tgt), _f32(xp, [1.0, 1.0]), _f32(xp, dh)
+    head = net.forward(xd)
+    loss = net.loss(4, td, wd)
+    net.backward(4, dhead=dhd)
+    xp.synchronize()
+    P = lay.unflatten(p.astype(np.float64))
+    ref = C.NetRef(LN.ARCH_CONV, bf16=True)
+    rhead = ref.forward(P, x)
+    # backward from the device's own stored activations (bf16 rounding and ReLU masks of values at
+    # rounding boundaries are then identical): the chain of backward kernels to float32 accuracy
+    b = net._bufs[4]
+    acts = [x.astype(np.float64)] + [a.float().cpu().numpy().astype(np.float64) for a in b["acts"]]
+    hid = b["hid"].float().cpu().numpy().astype(np.float64)
+    ref.cache = (acts, acts[-1].reshape(4, -1), hid)
+    G = ref.backward(P, dh.astype(np.float64))
+    h = head.cpu().numpy()
+    print(f"head max |diff| {np.abs(h - rhead).max():.2e} of max |head| {np.abs(rhead).max():.2e}", flush=True)
+    assert np.abs(h - rhead).max() <= 2e-2 * np.abs(rhead).max()
+    rl, _, _ = C.gauss_nll(h.astype(np.float64), tgt.astype(np.float64), [1.0, 1.0])
+    assert abs(loss[0].item() - rl) <= 1e-5 * max(1.0, abs(rl))
+    g = lay.unflatten(net.grad.cpu().numpy())
+    errs = {k: np.linalg.norm(g[k] - G[k]) / max(np.linalg.norm(G[k]), 1e-30) for k in P}
+    print("relative L2 gradient errors:", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-4, (k, e)
+
+
+def test_training_reduces_loss(xp):
+    """A few hundred Adam steps on a fixed synthetic regression (a planted bright row whose depth
+    encodes one target, a brightness offset of the second channel the other) must bring both
+    targets' mean squared error below 0.1 (their variance is 1/3)."""
+    from oct_recon_af_v1_amd import learned as LN
+    print("case: training smoke", flush=True)
+    torch = xp.torch
+    rng = np.random.default_rng(31)
+    M = 256
+    t = rng.uniform(-1, 1, (M, 2)).astype(np.float32)
+    X = rng.standard_normal((M, 2, LN.IMG, LN.IMG)).astype(np.float32) * 0.2
+    rows = ((t[:, 1] + 1) * 100 + 20).astype(int)
+    for i in range(M):
+        X[i, :, rows[i]:rows[i] + 4, :] += 2.0
+        X[i, 1] += 0.5 + 0.5 * t[i, 0]
+    Xd = _bf(xp, X)
+    Yd = _f32(xp, t)
+    net = LN.EstimatorNet(xp, seed=1)
+    hist = LN.train(net, Xd, Yd, epochs=25, batch=32, lr=2e-3, warmup=20)
+    first, last = hist[0]["train"][0], hist[-1]["train"][0]
+    mse = hist[-1]["train"][1:]
+    print(f"loss {first:.3f} -> {last:.3f}, mse {mse}", flush=True)
+    assert np.isfinite(last) and last < first - 1.0 and max(mse) < 0.1
+
+
+def test_depth_median_and_crop(xp):
+    from oct_recon_af_v1_amd import _lib
+    from oct_recon_af_v1_amd.estimation import depth_median
+    print("case: depth median / crop", flush=True)
+    torch = xp.torch
+    rng = np.random.default_rng(41)
+    for nX in (512, 257, 1):
+        mag = (np.abs(rng.standard_normal((3, nX, 1024))) * 50).astype(np.float32)
+        mag[0, :, 7] = 2.5                                       # ties
+        med = depth_median(_f32(xp, mag), xp).cpu().numpy()
+        assert np.array_equal(med, np.median(mag.astype(np.float64), axis=1)), nX
+    mag = (np.abs(rng.standard_normal((2, 300, 1024))) * 40).astype(np.float32)
+    md = _f32(xp, mag)
+    out = torch.zeros((2, 2, 256, 256), dtype=torch.bfloat16, device=xp.device)
+    for c in (0, 1):
+        _lib.check(_lib.lib().oct_logmag_crop(md.data_ptr(), 2, 300, 1024, 1024, 22, 0, 4, 256, 256, 10.0 * (c + 1),
+                                              20.0, out.data_ptr(), 2, c, xp.stream()), "crop")
+    got = out.float().cpu().numpy()
+    for c in (0, 1):
+        ref = C.logmag_crop(mag, 22, 0, 4, 256, 256, 10.0 * (c + 1), 20.0)
+        eq, worst = _bf_close(got[:, c], ref)
+        assert eq >= 0.999 and worst <= 1.0
+
+
+def test_per_file_windows_equal_single(xp):
+    """scores_per_file with a different beta per B-scan equals scoring each beta for all."""
+    from oct_recon_af_v1_amd import synthetic as SY
+    from oct_recon_af_v1_amd.estimation import HipDispersionScorer
+    from oct_recon_af_v1_amd.geometry import beta_from_index_units
+    print("case: per-file windows", flush=True)
+    spec = SY.SpectrumSpec()
+    lam = spec.lambda_nm()
+    raw = SY.make_bscans(4, 128, spec, beta=[1e-4, -5e-5, 0.0, 2e-4], seed=5, device=xp.device)
+    sc = HipDispersionScorer(lam, raw, spec.apod, xp)
+    betas = beta_from_index_units(np.array([1e-4, -5e-5, 3e-5, 1.5e-4]), lam)
+    per = sc.scores_per_file(betas)
+    single = np.array([sc.scores_many([b])[0][i] for i, b in enumerate(betas)])
+    assert np.array_equal(per, single)
