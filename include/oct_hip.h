@@ -161,6 +161,12 @@ int oct_plan_use_window(oct_plan* plan, const float* dev_window);
 int oct_plan_use_windows(oct_plan* plan, const float* dev_windows, long long file_stride);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* stream);
+/* Candidate windows out (count, n, 2) float32: hann_over_n[j] exp(-i beta_b dk2[j]) in float64, with
+ * beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref (device
+ * arrays; `head` lets a network's output set the candidates without a host round trip). */
+int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,
+                           double to_ref, int count, const double* hann_over_n, const double* dk2, int n, float* out,
+                           void* stream);
 
 /* ---- operator (MFMA) form: the whole linear spectral map as one complex operator
  * A = F_crop diag(window / N) S' (N/2-point crop [z0, z0+nzc), S' = banded sinc + rank-1 mean

@@ -71,6 +71,8 @@ SIGNATURES = {
     "oct_cnn_fc_bwd_weight": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "oct_cnn_gauss_nll": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]),
     "oct_cnn_adam": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _f, _i, _vp]),
+    "oct_dispersion_windows": (_i, [_vp, _vp, _i, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _vp, _vp,
+                                    _i, _vp, _vp]),
     "oct_last_error": (ctypes.c_char_p, []),
 }
 

@@ -83,4 +83,26 @@ __global__ __launch_bounds__(kSelThreads) void topfrac_metric_kernel(const float
     }
 }
 
+// Dispersion search windows: out[b, j] = hann_n[j] exp(-i beta_b dk2[j]) (hann_n = Hann/rms/N,
+// dk2 = (k - mean k)^2 of the equispaced k grid, octrecon/estimation/dispersion.py:66-107), float64
+// phase, stored as float32 pairs, with
+//   beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref.
+// Grid (ceil(n / 256), count).
+__global__ __launch_bounds__(256) void windows_kernel(const double* __restrict__ betas, const float* __restrict__ head,
+                                                      int head_stride, double head_scale, double offset,
+                                                      double to_ref, const double* __restrict__ hann_n,
+                                                      const double* __restrict__ dk2, int n,
+                                                      float* __restrict__ out) {
+    const int b = blockIdx.y;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const double beta = ((head ? (double)head[(long long)b * head_stride] * head_scale : betas[b]) + offset) * to_ref;
+    double s, c;
+    sincos(beta * dk2[j], &s, &c);
+    float2 w;
+    w.x = (float)(hann_n[j] * c);
+    w.y = (float)(-hann_n[j] * s);
+    reinterpret_cast<float2*>(out)[(long long)b * n + j] = w;
+}
+
 }  // namespace oct

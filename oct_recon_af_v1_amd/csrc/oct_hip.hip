@@ -438,6 +438,19 @@ int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_m
     return OCT_OK;
 }
 
+int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,
+                           double to_ref, int count, const double* hann_over_n, const double* dk2, int n, float* out,
+                           void* stream) {
+    if ((!betas && !head) || !hann_over_n || !dk2 || !out) return fail(OCT_EINVAL, "null argument");
+    if (count <= 0) return OCT_OK;
+    if (n <= 0 || count > 65535 || (head && head_stride <= 0)) return fail(OCT_EINVAL, "bad window shape (%d x %d)", count, n);
+    if (reinterpret_cast<uintptr_t>(out) % 8) return fail(OCT_EINVAL, "out must be 8-byte aligned");
+    hipLaunchKernelGGL(oct::windows_kernel, dim3((n + 255) / 256, count), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       head ? nullptr : betas, head, head_stride, head_scale, offset, to_ref, hann_over_n, dk2, n, out);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 // ---------------------------------------------------------------- operator (MFMA) form
 struct oct_opplan {
     int device;

@@ -42,6 +42,9 @@ class HipDispersionScorer:
         self.hann = hann / np.sqrt(np.mean(hann ** 2))
         k_eq = 2 * np.pi / self.sg0.lambda_eq_nm
         self.dk2 = (k_eq - k_eq.mean()) ** 2
+        torch = self.torch
+        self._hann_n = torch.tensor(self.hann / N, dtype=torch.float64, device=xp.device)
+        self._dk2 = torch.tensor(self.dk2, dtype=torch.float64, device=xp.device)
         self.nz = N // 2
         self._mag = None
         self.n_evals = 0
@@ -56,11 +59,24 @@ class HipDispersionScorer:
         if self._mag is None or tuple(self._mag.shape) != shape:
             self._mag = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
 
-    def windows(self, betas):
-        """(len(betas), N, 2) float32 windows scaled by 1/N (float64 phase)."""
-        betas = np.atleast_1d(np.asarray(betas, float))
-        w = self.hann[None, :] * np.exp(-1j * betas[:, None] * self.dk2[None, :]) / self.sg0.n
-        return np.ascontiguousarray(np.stack([w.real, w.imag], -1).astype(np.float32))
+    def windows(self, betas=None, head=None, head_scale=1.0, offset=0.0, to_ref=1.0, count=None):
+        """(count, N, 2) float32 device windows hann/rms/N exp(-i beta dk2), float64 phase
+        (oct_dispersion_windows): beta = betas (reference units), or per B-scan from a device
+        network head: (head[b, 0] * head_scale + offset) * to_ref."""
+        torch = self.torch
+        if head is None:
+            b = torch.tensor(np.atleast_1d(np.asarray(betas, float)), dtype=torch.float64, device=self.xp.device)
+            count, hp, hs = b.numel(), None, 1
+        else:
+            b, hp, hs = None, head.data_ptr(), head.stride(0)
+            count = head.shape[0] if count is None else count
+        out = torch.empty((count, self.sg0.n, 2), dtype=torch.float32, device=self.xp.device)
+        _lib.check(_lib.lib().oct_dispersion_windows(None if b is None else b.data_ptr(), hp, hs, head_scale, offset,
+                                                     to_ref, count, self._hann_n.data_ptr(), self._dk2.data_ptr(),
+                                                     self.sg0.n, out.data_ptr(), self.xp.stream()),
+                   "oct_dispersion_windows")
+        self._keep_b = b                  # the launch is asynchronous
+        return out
 
     def _metric(self, out_row):
         _lib.check(_lib.lib().oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX,
@@ -72,8 +88,8 @@ class HipDispersionScorer:
         betas = np.atleast_1d(np.asarray(betas, float))
         if betas.size not in (1, self.B):
             raise ValueError(f"{betas.size} candidates for {self.B} B-scans")
-        win = self.torch.from_numpy(self.windows(betas)).to(self.xp.device)
-        return self.magnitude_with(win, out)
+        win = self.windows(betas)
+        return self.magnitude_with(win[0] if betas.size == 1 else win, out)
 
     def magnitude_with(self, win, out=None):
         """|A| with device windows win: (N, 2) for every B-scan or (B, N, 2) one per B-scan."""
@@ -95,7 +111,7 @@ class HipDispersionScorer:
         """(G, B) float64 scores for a batch of candidates, one host synchronisation."""
         torch = self.torch
         betas = np.atleast_1d(np.asarray(betas, float))
-        win = torch.from_numpy(self.windows(betas)).to(self.xp.device)
+        win = self.windows(betas)
         out = torch.empty((len(betas), self.B), dtype=torch.float64, device=self.xp.device)
         L = _lib.lib()
         try:

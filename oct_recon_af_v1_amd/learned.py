@@ -41,11 +41,17 @@ class InputSpec:
     zpool: int = 4
     n_z: int = 1024
     probes_idx: tuple = (-1e-4, 1e-4)     # probe dispersions, index units (rad per squared k sample)
+    relative: bool = False                # probes are offsets from a per-B-scan centre (second stage)
     offset_db: float = 10.0
     scale_db: float = 20.0
-    beta_scale: float = 2e-4              # target t_beta = beta_idx / beta_scale
+    beta_scale: float = 2e-4              # target t_beta = (beta_idx - centre) / beta_scale
     focus_center: float = 512.0           # target t_focus = (z_bin - center) / scale
     focus_scale: float = 256.0
+
+
+STAGE1 = InputSpec()
+# second stage: probes at the first stage's beta +-2e-5, residual within +-3e-5
+STAGE2 = InputSpec(probes_idx=(-2e-5, 2e-5), relative=True, beta_scale=3e-5)
 
 
 class ParamLayout:
@@ -117,7 +123,7 @@ class EstimatorNet:
         return flat[off:off + int(np.prod(shape))]
 
     def _ptr(self, flat, name):
-        return self._slice(flat, name).data_ptr()
+        return flat.data_ptr() + self.layout.entries[name][1] * flat.element_size()
 
     def params_numpy(self) -> np.ndarray:
         return self.p32.cpu().numpy()
@@ -225,31 +231,48 @@ class EstimatorNet:
 # --------------------------------------------------------------------------- inputs and labels
 class InputBuilder:
     """Network inputs of raw B-scans: the two probe reconstructions, cropped and log-scaled on the
-    device (oct_logmag_crop)."""
+    device (oct_logmag_crop).  Probes are fixed dispersions, or offsets from a per-B-scan centre
+    (spec.relative), reconstructed with per-B-scan windows in one launch per probe."""
 
-    def __init__(self, scorer, spec: InputSpec = InputSpec()):
+    def __init__(self, scorer, spec: InputSpec = STAGE1):
         self.scorer = scorer
         self.spec = spec
-        lam = scorer.sg0.lambda_nm
-        self.probe_ref = [beta_from_index_units(b, lam) for b in spec.probes_idx]
-        self.probe_win = scorer.torch.from_numpy(scorer.windows(self.probe_ref)).to(scorer.xp.device)
+        self.lam = scorer.sg0.lambda_nm
+        self.to_ref = beta_from_index_units(1.0, self.lam)
+        if not spec.relative:
+            self.probe_win = scorer.windows([b * self.to_ref for b in spec.probes_idx])
 
-    def build(self, raw=None, out=None):
-        """(B, 2, 256, 256) bf16 for the scorer's B-scans (or `raw`, which becomes its set)."""
+    def build(self, raw=None, out=None, centers_idx=None, centers_head=None):
+        """(B, 2, 256, 256) bf16 for the scorer's B-scans (or `raw`, which becomes its set).  A
+        relative spec takes the centres (index units) as a host array `centers_idx`, or as
+        `centers_head` = (device head (B, 4) float32, beta scale): centre = head[b, 0] * scale,
+        turned into windows on the device without a host round trip."""
         sc, sp = self.scorer, self.spec
         torch = sc.torch
         if raw is not None:
             sc.set_raw(raw)
-        if sc.nX < sp.n_x_crop or sp.n_z * 1 > sc.nz or sp.n_z // sp.zpool != IMG:
+        if sc.nX < sp.n_x_crop or sp.n_z > sc.nz or sp.n_z // sp.zpool != IMG:
             raise ValueError(f"B-scans of {sc.nX} A-scans x {sc.nz} rows cannot give the {IMG} x {IMG} crop")
         B = sc.B
+        if sp.relative and centers_head is None:
+            c = np.asarray(centers_idx, float).reshape(-1)
+            if c.size != B:
+                raise ValueError(f"{c.size} centres for {B} B-scans")
+        if centers_head is not None and centers_head[0].shape[0] != B:
+            raise ValueError(f"{centers_head[0].shape[0]} heads for {B} B-scans")
         out = torch.empty((B, 2, IMG, IMG), dtype=torch.bfloat16, device=sc.xp.device) if out is None else out
         x0 = (sc.nX - sp.n_x_crop) // 2
         L = _lib.lib()
-        for c in range(len(self.probe_ref)):
-            mag = sc.magnitude_with(self.probe_win[c])
+        for k, probe in enumerate(sp.probes_idx):
+            if sp.relative and centers_head is not None:
+                mag = sc.magnitude_with(sc.windows(head=centers_head[0], head_scale=centers_head[1], offset=probe,
+                                                   to_ref=self.to_ref))
+            elif sp.relative:
+                mag = sc.magnitude_with(sc.windows((c + probe) * self.to_ref))
+            else:
+                mag = sc.magnitude_with(self.probe_win[k])
             _lib.check(L.oct_logmag_crop(mag.data_ptr(), B, sc.nX, mag.stride(1), sc.nz, x0, 0, sp.zpool, IMG,
-                                         sp.n_x_crop, sp.offset_db, sp.scale_db, out.data_ptr(), 2, c,
+                                         sp.n_x_crop, sp.offset_db, sp.scale_db, out.data_ptr(), 2, k,
                                          sc.xp.stream()), "oct_logmag_crop")
         return out
 
@@ -286,15 +309,15 @@ def make_labels(scorer, lo_idx=-2.5e-4, hi_idx=2.5e-4, n_grid=41, iters=24):
                 profiles=prof)
 
 
-def normalise_targets(beta_idx, focus, spec: InputSpec = InputSpec()):
-    return np.stack([np.asarray(beta_idx) / spec.beta_scale,
+def normalise_targets(beta_idx, focus, spec: InputSpec = STAGE1, center_idx=0.0):
+    return np.stack([(np.asarray(beta_idx) - np.asarray(center_idx)) / spec.beta_scale,
                      (np.asarray(focus) - spec.focus_center) / spec.focus_scale], -1).astype(np.float32)
 
 
-def denormalise(head, spec: InputSpec = InputSpec()):
+def denormalise(head, spec: InputSpec = STAGE1, center_idx=0.0):
     """head (n, 4) numpy -> beta_idx, focus bin, sigma_beta_idx, sigma_focus (bins)."""
     h = np.asarray(head, float)
-    beta = h[:, 0] * spec.beta_scale
+    beta = np.asarray(center_idx, float) + h[:, 0] * spec.beta_scale
     focus = h[:, 1] * spec.focus_scale + spec.focus_center
     sb = np.exp(0.5 * h[:, 2]) * spec.beta_scale
     sf = np.exp(0.5 * h[:, 3]) * spec.focus_scale
@@ -361,16 +384,117 @@ def evaluate(net, X, Y, batch=256):
     return np.mean((h[:, :N_TARGETS] - y) ** 2, 0).tolist()
 
 
-def save_params(path, net: EstimatorNet, spec: InputSpec = InputSpec(), meta=None):
-    """Trained weights as an .npz of plain arrays (bf16 weights stored as their uint16 bits and
-    the float32 master copy)."""
+def save_params(path, stages, meta=None):
+    """Trained weights of the stages (EstimatorNet or flat float32 vectors) as an .npz of plain
+    float32 arrays plus JSON metadata."""
     import json
-    p = net.params_numpy()
-    np.savez_compressed(path, params=p, meta=np.array(json.dumps(dict(meta or {}, spec=spec.__dict__,
-                                                                       n_params=net.layout.n_params))))
+    stages = stages if isinstance(stages, (list, tuple)) else [stages]
+    arrays = {f"params{i}": (st.params_numpy() if isinstance(st, EstimatorNet) else np.asarray(st, np.float32))
+              for i, st in enumerate(stages)}
+    np.savez_compressed(path, meta=np.array(json.dumps(dict(meta or {}, n_params=ParamLayout().n_params,
+                                                            stages=len(stages)))), **arrays)
 
 
 def load_params(path):
+    """([stage parameter vectors], meta)."""
     import json
     with np.load(path, allow_pickle=False) as z:
-        return z["params"].astype(np.float32), json.loads(str(z["meta"]))
+        meta = json.loads(str(z["meta"]))
+        return [z[f"params{i}"].astype(np.float32) for i in range(meta.get("stages", 1))], meta
+
+
+def calibrate_gate(err, sigma, tol, target=0.99):
+    """Largest predicted-sigma gate such that, of the samples it accepts, at least `target` have
+    |err| <= tol.  Returns (gate, accepted fraction, precision)."""
+    err, sigma = np.abs(np.asarray(err, float)), np.asarray(sigma, float)
+    order = np.argsort(sigma, kind="stable")
+    ok = err[order] <= tol
+    prec = np.cumsum(ok) / np.arange(1, len(ok) + 1)
+    good = np.nonzero(prec >= target)[0]
+    if good.size == 0:
+        return float(sigma.min() * 0.5), 0.0, float("nan")
+    k = int(good.max())
+    return float(sigma[order][k]), (k + 1) / len(ok), float(prec[k])
+
+
+# --------------------------------------------------------------------------- estimator with fallback
+DEFAULT_PARAMS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "estimator_bf16.npz"
+SCORE_TOL = 0.05          # judge tolerance: sharpness (log top-1 % intensity) within 0.05 of the optimum
+BETA_TOL_IDX = 5.5e-6     # |beta error| (index units) at which that drop is reached on this spectrometer
+FOCUS_TOL_BINS = 8.8      # judge tolerance: the stitching focus weight's standard deviation (sqrt(2) x 30 um / dz)
+
+
+def take_bscans(raw, idx):
+    """raw[idx] along the first axis for a device uint16 tensor (gathered through an int16 view)."""
+    torch = __import__("torch")
+    i = torch.as_tensor(np.asarray(idx, np.int64), device=raw.device)
+    if raw.dtype == torch.uint16:
+        return torch.index_select(raw.view(torch.int16), 0, i).view(torch.uint16)
+    return torch.index_select(raw, 0, i)
+
+
+class LearnedEstimator:
+    """beta (index and reference units) and focus depth of B-scans from the two-stage network, with
+    the sharpness search / confocal detector as fallback where the predicted sigma exceeds the gate.
+
+    Stage 1 sees probes at beta = -+1e-4 and predicts beta over +-2e-4; stage 2 sees probes at the
+    stage-1 beta -+2e-5 and predicts the residual (and the focus, from the sharper images).
+    gate_beta / gate_focus: largest accepted stage-2 sigma (index units / depth bins); defaults
+    come from the calibration stored with the weights."""
+
+    def __init__(self, xp, lambda_nm, apod_size: int, params=None, gate_beta=None, gate_focus=None,
+                 n_medium: float = 1.33):
+        from .estimation import HipDispersionScorer
+        meta = {}
+        if params is None or isinstance(params, (str, bytes, __import__("os").PathLike)):
+            params, meta = load_params(params or DEFAULT_PARAMS)
+        if len(params) != 2:
+            raise ValueError(f"{len(params)} parameter sets, the estimator has 2 stages")
+        self.xp = xp
+        self.nets = [EstimatorNet(xp, p) for p in params]
+        self.specs = (STAGE1, STAGE2)
+        self.lambda_nm = np.asarray(lambda_nm, float)
+        self.apod = apod_size
+        self.n_medium = n_medium
+        cal = meta.get("calibration", {})
+        self.gate_beta = gate_beta if gate_beta is not None else cal.get("gate_beta", 2e-6)
+        self.gate_focus = gate_focus if gate_focus is not None else cal.get("gate_focus", FOCUS_TOL_BINS)
+        self._scorer_cls = HipDispersionScorer
+        self.scorer = None
+        self.builders = None
+
+    def _bind(self, raw):
+        if self.scorer is None:
+            self.scorer = self._scorer_cls(self.lambda_nm, raw, self.apod, self.xp, n_medium=self.n_medium)
+            self.builders = [InputBuilder(self.scorer, sp) for sp in self.specs]
+        else:
+            self.scorer.set_raw(raw)
+
+    def predict(self, raw):
+        """Network only: per B-scan beta_idx, beta_ref, focus (0-based bin), sigmas, confident
+        (plus the stage-1 beta)."""
+        self._bind(raw)
+        h1 = self.nets[0].forward(self.builders[0].build())
+        h2 = self.nets[1].forward(self.builders[1].build(centers_head=(h1, self.specs[0].beta_scale)))
+        H = self.xp.torch.stack([h1, h2]).cpu().numpy()          # the one host synchronisation
+        b1 = H[0][:, 0].astype(np.float64) * self.specs[0].beta_scale   # as the window kernel forms it
+        b, f, sb, sf = denormalise(H[1], self.specs[1], center_idx=b1)
+        return dict(beta_idx=b, beta_ref=beta_from_index_units(b, self.lambda_nm), focus=f, sigma_beta=sb,
+                    sigma_focus=sf, beta_stage1=b1,
+                    confident=(sb <= self.gate_beta) & (sf <= self.gate_focus) & (np.abs(b - b1) < self.specs[1].beta_scale))
+
+    def estimate(self, raw):
+        """Per B-scan estimates with fallback: B-scans the network is not confident about get the
+        per-B-scan sharpness search and the confocal detector (the labels' procedure)."""
+        p = self.predict(raw)
+        p["source"] = np.where(p["confident"], "cnn", "search")
+        bad = np.nonzero(~p["confident"])[0]
+        if bad.size:
+            raw_bad = take_bscans(raw, bad) if len(bad) < raw.shape[0] else raw
+            self.scorer.set_raw(raw_bad)
+            lab = make_labels(self.scorer)
+            p["beta_idx"][bad] = lab["beta_idx"]
+            p["beta_ref"][bad] = lab["beta_ref"]
+            p["focus"][bad] = np.where(np.isfinite(lab["focus"]), lab["focus"], p["focus"][bad])
+        p["fallback_rate"] = float(bad.size / len(p["confident"]))
+        return p

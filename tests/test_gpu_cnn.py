@@ -254,3 +254,29 @@ def test_per_file_windows_equal_single(xp):
     per = sc.scores_per_file(betas)
     single = np.array([sc.scores_many([b])[0][i] for i, b in enumerate(betas)])
     assert np.array_equal(per, single)
+
+
+def test_dispersion_windows_kernel(xp):
+    """oct_dispersion_windows against the float64 numpy window, from betas and from a head."""
+    from oct_recon_af_v1_amd import synthetic as SY
+    from oct_recon_af_v1_amd.estimation import HipDispersionScorer
+    from oct_recon_af_v1_amd.geometry import beta_from_index_units
+    print("case: dispersion windows", flush=True)
+    spec = SY.SpectrumSpec()
+    lam = spec.lambda_nm()
+    raw = SY.make_bscans(2, 64, spec, seed=6, device=xp.device)
+    sc = HipDispersionScorer(lam, raw, spec.apod, xp)
+    idx = np.array([-2.2e-4, -3e-5, 0.0, 1e-6, 1.7e-4])
+    betas = beta_from_index_units(idx, lam)
+    got = sc.windows(betas).cpu().numpy()
+    w = sc.hann[None, :] * np.exp(-1j * betas[:, None] * sc.dk2[None, :]) / sc.sg0.n
+    ref = np.stack([w.real, w.imag], -1).astype(np.float32)
+    d = np.abs(got.astype(np.float64) - ref)
+    ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+    assert np.mean(got == ref) >= 0.999 and np.all(d <= ulp + 1e-30)
+    head = xp.torch.tensor(np.stack([idx / 2e-4, np.zeros(5), np.zeros(5), np.zeros(5)], -1).astype(np.float32),
+                           device=xp.device)
+    h32 = head[:, 0].cpu().numpy().astype(np.float64) * 2e-4
+    from_head = sc.windows(head=head, head_scale=2e-4, offset=1e-5, to_ref=beta_from_index_units(1.0, lam))
+    from_b = sc.windows((h32 + 1e-5) * beta_from_index_units(1.0, lam))
+    assert np.array_equal(from_head.cpu().numpy(), from_b.cpu().numpy())

@@ -1,0 +1,112 @@
+"""The trained estimator (oct_recon_af_v1_amd/data/estimator_bf16.npz) on held-out B-scans, judged
+by the sharpness search and the confocal detector it replaces.
+
+Tolerances (stated in oct_recon_af_v1_amd/learned.py):
+  beta:  the search's own sharpness score at the network's beta within SCORE_TOL = 0.05 of the
+         score at the search optimum (log of the mean brightest-1 % intensity; 0.05 ~ 0.2 dB);
+  focus: within FOCUS_TOL_BINS = 8.8 depth bins of the detector's peak (the stitching focus
+         weight's standard deviation at sigma = 30 um).
+Bars: of the B-scans the network accepts (its predicted sigma under the calibrated gates), at
+least 90 % within both tolerances, and at most half sent to the fallback; with the fallback
+every B-scan's estimate is within tolerance of the judge.  Speed: the network's estimate for a
+tile of 8 B-scans at least 50x faster than the restated reference search (coarse grid + bounded
+refinement of the mean and of every B-scan, octrecon/estimation/dispersion.py:154-267) plus the
+focus detector on the same device.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def xp():
+    from oct_recon_af_v1_amd.backend import get_xp
+    return get_xp("hip")[0]
+
+
+@pytest.fixture(scope="module")
+def heldout(xp):
+    from oct_recon_af_v1_amd import synthetic as SY
+    spec = SY.SpectrumSpec()
+    rng = np.random.default_rng(20240)             # held out: training draws from seed 1
+    n = 128
+    beta = rng.uniform(-2e-4, 2e-4, n)
+    focus = rng.uniform(0, 1024, n)
+    raw = SY.make_focus_bscans(n, 512, beta, focus, spec, seed=987654, device=xp.device)
+    return spec, raw, beta, focus
+
+
+def test_learned_estimator_vs_search_heldout(xp, heldout):
+    from oct_recon_af_v1_amd import learned as LN
+    print("case: learned estimator on 128 held-out B-scans", flush=True)
+    spec, raw, beta, focus = heldout
+    assert LN.DEFAULT_PARAMS.exists(), "trained weights are missing"
+    est = LN.LearnedEstimator(xp, spec.lambda_nm(), spec.apod)
+    p = est.predict(raw)
+    sc = est.scorer
+    lab = LN.make_labels(sc)
+    s_cnn = sc.scores_per_file(p["beta_ref"])
+    drop = lab["score"] - s_cnn
+    beta_ok = drop <= LN.SCORE_TOL
+    det = np.isfinite(lab["focus"])
+    focus_ok = det & (np.abs(p["focus"] - lab["focus"]) <= LN.FOCUS_TOL_BINS)
+    conf = p["confident"]
+    both = beta_ok & focus_ok
+    print(f"all: beta within tol {beta_ok.mean():.1%} (|dbeta| median {np.median(np.abs(p['beta_idx'] - lab['beta_idx'])):.2e}),"
+          f" focus within tol {focus_ok.mean():.1%} (median {np.nanmedian(np.abs(p['focus'] - lab['focus'])):.2f} bins)",
+          flush=True)
+    print(f"accepted {conf.mean():.1%}: both within tolerance {both[conf].mean():.1%}; "
+          f"fallback rate {1 - conf.mean():.1%}", flush=True)
+    assert conf.mean() >= 0.5
+    assert both[conf].mean() >= 0.9
+    # with the fallback, every B-scan is judged within tolerance
+    e = est.estimate(raw)
+    sc.set_raw(raw)
+    s_final = sc.scores_per_file(e["beta_ref"])
+    ok_final = ((lab["score"] - s_final) <= LN.SCORE_TOL) & (~det | (np.abs(e["focus"] - lab["focus"]) <= LN.FOCUS_TOL_BINS))
+    print(f"with fallback: within tolerance {ok_final.mean():.1%}", flush=True)
+    assert ok_final.mean() >= 0.97
+
+
+def test_learned_estimator_speed(xp, heldout):
+    """Estimation time per tile of 8 B-scans: network vs the restated reference search."""
+    from oct_recon_af_v1_amd import learned as LN
+    from oct_recon_af_v1_amd.estimation import HipDispersionScorer, depth_median, estimate_dispersion
+    from oct_recon_af_v1_amd.geometry import beta_from_index_units
+    print("case: estimator speed, tiles of 8 B-scans", flush=True)
+    spec, raw, beta, focus = heldout
+    lam = spec.lambda_nm()
+    est = LN.LearnedEstimator(xp, lam, spec.apod)
+    tiles = [raw[i:i + 8] for i in range(0, 64, 8)]
+    sc = HipDispersionScorer(lam, tiles[0], spec.apod, xp)
+    lo, hi = beta_from_index_units(-2.5e-4, lam), beta_from_index_units(2.5e-4, lam)
+
+    def search(t):
+        sc.set_raw(t)
+        sc.n_evals = 0
+        r = estimate_dispersion(sc, 0.0, bounds=(lo, hi))
+        prof = depth_median(sc.magnitude_at(r["value"]), xp).cpu().numpy().mean(0)
+        return r["value"], LN.focus_from_profiles(prof[None])[0][0], r["n_evaluations"]
+
+    def cnn(t):
+        p = est.predict(t)
+        return float(np.median(p["beta_ref"])), float(np.median(p["focus"]))
+
+    search(tiles[0])
+    cnn(tiles[0])
+    xp.synchronize()
+    t0 = time.perf_counter()
+    evals = [search(t)[2] for t in tiles]
+    xp.synchronize()
+    ts = (time.perf_counter() - t0) / len(tiles)
+    t0 = time.perf_counter()
+    for t in tiles:
+        cnn(t)
+    xp.synchronize()
+    tc = (time.perf_counter() - t0) / len(tiles)
+    print(f"per tile of 8 B-scans: search {ts * 1e3:.2f} ms ({np.mean(evals):.0f} evaluations), "
+          f"network {tc * 1e3:.3f} ms, ratio {ts / tc:.0f}x", flush=True)
+    assert ts / tc >= 50
