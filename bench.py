@@ -210,6 +210,63 @@ def config2(xp, args):
     return res
 
 
+def estimator_bench(xp, args):
+    """Learned beta / focus estimator on 64 held-out B-scans (512 A-scans each, beta uniform over
+    +-2e-4, focus uniform over 0..1023): accuracy against the sharpness search and the confocal
+    detector (the judge), and estimation time per tile of 8 B-scans, network vs the restated
+    reference search (grid + bounded refinement of the mean and of each B-scan) plus detector."""
+    import time
+
+    from oct_recon_af_v1_amd import learned as LN
+    from oct_recon_af_v1_amd import synthetic as SY
+    from oct_recon_af_v1_amd.estimation import HipDispersionScorer, depth_median, estimate_dispersion
+    from oct_recon_af_v1_amd.geometry import beta_from_index_units
+    spec = SY.SpectrumSpec()
+    lam = spec.lambda_nm()
+    rng = np.random.default_rng(args.seed + 500)
+    n = 64
+    raw = SY.make_focus_bscans(n, 512, rng.uniform(-2e-4, 2e-4, n), rng.uniform(0, 1024, n), spec,
+                               seed=args.seed + 501, device=xp.device)
+    est = LN.LearnedEstimator(xp, lam, spec.apod)
+    p = est.predict(raw)
+    lab = LN.make_labels(est.scorer)
+    drop = lab["score"] - est.scorer.scores_per_file(p["beta_ref"])
+    det = np.isfinite(lab["focus"])
+    b_ok = drop <= LN.SCORE_TOL
+    f_ok = det & (np.abs(p["focus"] - lab["focus"]) <= LN.FOCUS_TOL_BINS)
+    conf = p["confident"]
+    tiles = [raw[i:i + 8] for i in range(0, n, 8)]
+    sc = HipDispersionScorer(lam, tiles[0], spec.apod, xp)
+    lo, hi = beta_from_index_units(-2.5e-4, lam), beta_from_index_units(2.5e-4, lam)
+
+    def search(t):
+        sc.set_raw(t)
+        sc.n_evals = 0
+        r = estimate_dispersion(sc, 0.0, bounds=(lo, hi))
+        prof = depth_median(sc.magnitude_at(r["value"]), xp).cpu().numpy().mean(0)
+        LN.focus_from_profiles(prof[None])
+        return r["n_evaluations"]
+
+    search(tiles[0])
+    est.predict(tiles[0])
+    xp.synchronize()
+    t0 = time.perf_counter()
+    evals = [search(t) for t in tiles]
+    ts = (time.perf_counter() - t0) / len(tiles)
+    t0 = time.perf_counter()
+    for t in tiles:
+        est.predict(t)
+    xp.synchronize()
+    tc = (time.perf_counter() - t0) / len(tiles)
+    return {"bscans": n, "beta_within_tol": float(b_ok.mean()), "focus_within_tol": float(f_ok.mean()),
+            "beta_abs_err_median_idx": float(np.median(np.abs(p["beta_idx"] - lab["beta_idx"]))),
+            "focus_abs_err_median_bins": float(np.nanmedian(np.abs(p["focus"] - lab["focus"]))),
+            "accepted": float(conf.mean()), "both_within_tol_accepted": float((b_ok & f_ok)[conf].mean()),
+            "tolerance": {"sharpness_drop": LN.SCORE_TOL, "focus_bins": LN.FOCUS_TOL_BINS},
+            "tile_ms_search": ts * 1e3, "search_evaluations": float(np.mean(evals)), "tile_ms_network": tc * 1e3,
+            "speedup": ts / tc}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,10 +281,11 @@ def main():
     ap.add_argument("--no-mfma", action="store_true", help="skip the operator-form measurements")
     ap.add_argument("--copy-calibration", action="store_true", help="run a 1 GiB device copy first")
     ap.add_argument("--no-config2", action="store_true", help="skip the 200 000-A-scan all-bins measurement")
+    ap.add_argument("--no-estimator", action="store_true", help="skip the learned-estimator measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: multi-GPU runs are not part of this build "
-              "(the path shards by tile / y-row with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the path shards by tile / y-row with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -305,6 +363,9 @@ def main():
         if not args.no_config2:
             res["config2_precision_form"] = config2(xp, args)
             log(f"config2: {res['config2_precision_form']}")
+    if not args.no_estimator:
+        res["learned_estimator"] = estimator_bench(xp, args)
+        log(f"estimator: {res['learned_estimator']}")
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(rec, info, args.cpu_seconds)
     print(json.dumps(res), flush=True)

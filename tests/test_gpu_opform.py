@@ -4,7 +4,7 @@ Tolerances:
   * the float64 operator built on the device equals the oracle's dense float64 operator to
     1e-12 of its largest entry;
   * magnitudes through the fp16 3-product GEMM: with one clim (the oracle's finite dB range),
-    at least 99.9 % of voxels within 1 LSB of the float64 oracle; the
+    at least 99.9 % of voxels within 1 LSB of the float64 oracle (the precision bar); the
     measured rate and dB error are printed;
   * a cropped operator agrees with the same bins of the full one to 1e-6 of the largest value.
 """
