@@ -248,23 +248,36 @@ def estimator_bench(xp, args):
         return r["n_evaluations"]
 
     search(tiles[0])
-    est.predict(tiles[0])
+    est.predict(tiles[0], graph=True)
     xp.synchronize()
     t0 = time.perf_counter()
     evals = [search(t) for t in tiles]
     ts = (time.perf_counter() - t0) / len(tiles)
     t0 = time.perf_counter()
     for t in tiles:
-        est.predict(t)
+        est.predict(t, graph=True)
     xp.synchronize()
     tc = (time.perf_counter() - t0) / len(tiles)
+    t0 = time.perf_counter()
+    for t in tiles:
+        est.predict(t)
+    xp.synchronize()
+    te = (time.perf_counter() - t0) / len(tiles)
+    est.predict(raw, graph=True)
+    xp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        est.predict(raw, graph=True)                  # the 8 tiles in one batch
+    xp.synchronize()
+    tv = (time.perf_counter() - t0) / (4 * len(tiles))
     return {"bscans": n, "beta_within_tol": float(b_ok.mean()), "focus_within_tol": float(f_ok.mean()),
             "beta_abs_err_median_idx": float(np.median(np.abs(p["beta_idx"] - lab["beta_idx"]))),
             "focus_abs_err_median_bins": float(np.nanmedian(np.abs(p["focus"] - lab["focus"]))),
             "accepted": float(conf.mean()), "both_within_tol_accepted": float((b_ok & f_ok)[conf].mean()),
             "tolerance": {"sharpness_drop": LN.SCORE_TOL, "focus_bins": LN.FOCUS_TOL_BINS},
-            "tile_ms_search": ts * 1e3, "search_evaluations": float(np.mean(evals)), "tile_ms_network": tc * 1e3,
-            "speedup": ts / tc}
+            "tile_ms_search": ts * 1e3, "search_evaluations": float(np.mean(evals)),
+            "tile_ms_network_batched": tv * 1e3, "tile_ms_network_single": tc * 1e3,
+            "tile_ms_network_single_eager": te * 1e3, "speedup_batched": ts / tv, "speedup_single": ts / tc}
 
 
 def main():

@@ -152,15 +152,25 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
  * instead of the plan's own window; NULL restores it.  The pointer is captured at each launch,
  * so a batch of candidates can be queued without host synchronisation.
  * oct_topfrac_metric: out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over all
- * x and z >= z_min), M = n_x (n_z - z_min); mag (n_bscans, n_x, n_z) float32, out float64. */
+ * x and z >= z_min), M = n_x (n_z - z_min); mag (n_bscans, n_x, n_z) float32, out float64;
+ * exact (radix selection), deterministic, workspace of oct_topfrac_metric_workspace_bytes. */
 int oct_plan_use_window(oct_plan* plan, const float* dev_window);
 /* Per-file candidates: file b of later oct_spectral launches uses the window at
  * dev_windows + b * file_stride floats (file_stride 0: one window for every file, >= 2N otherwise).
  * Used by the batched per-B-scan search (each B-scan its own bracket), which the reference runs
  * one B-scan at a time (octrecon/estimation/dispersion.py:226-239). */
-int oct_plan_use_windows(oct_plan* plan, const float* dev_windows, long long file_stride);
+int oct_plan_use_windows(oct_plan* plan, const float* dev_windows, long long file_stride, long long repeat_stride);
+/* oct_spectral over the batch n_repeat times in one launch: virtual file r n_files + b reads raw
+ * file b and the window dev_windows + r repeat_stride + b file_stride (oct_plan_use_windows), and
+ * writes out rows of A-scan (r n_files + b) n_x + x; only scan lines [x0, x0 + n_x) of every file
+ * (n_x < 0: all); bg as computed by oct_background for the batch.  Used to reconstruct one batch
+ * at several candidate dispersions at once. */
+int oct_spectral_repeat(const oct_plan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                        int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
+                        long long out_stride, const float* bg, void* stream);
+size_t oct_topfrac_metric_workspace_bytes(int n_bscans, int n_x, int n_z, int z_min);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
-                       void* stream);
+                       void* workspace, size_t workspace_bytes, void* stream);
 /* Candidate windows out (count, n, 2) float32: hann_over_n[j] exp(-i beta_b dk2[j]) in float64, with
  * beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref (device
  * arrays; `head` lets a network's output set the candidates without a host round trip). */
@@ -193,11 +203,11 @@ int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride
  * of octrecon/estimation/dispersion.py:154-267 and focus.py:191-218, which stay the labels, the
  * judge and the fallback).  bfloat16 activations and weights (uint16_t bit patterns, round to
  * nearest even), float32 master weights, gradients and accumulation.
- * Network input: out[b, channel, h, w] (n_bscans, channels, height, width) bf16 =
- *   clamp((20 log10(max_{j<zpool} mag[b, x0 + w, z0 + h zpool + j]) - offset_db) / scale_db, -8, 8). */
+ * Network input: out[b, c, h, w] (n_bscans, channels, height, width) bf16 from magnitude images
+ * v = c n_bscans + b of mag (channels n_bscans, n_x, rows):
+ *   clamp((20 log10(max_{j<zpool} mag[v, x0 + w, z0 + h zpool + j]) - offset_db) / scale_db, -8, 8). */
 int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride, int n_rows, int x0, int z0, int zpool,
-                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, int channel,
-                    void* stream);
+                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, void* stream);
 /* conv 3x3 / stride 2 / pad 1 + ReLU: x (n, ci, h, w) -> y (n, co, h/2, w/2); w (co, ci, 3, 3); co % 16 == 0 */
 int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
                      int wd, int co, void* stream);

@@ -58,10 +58,12 @@ SIGNATURES = {
     "oct_opplan_destroy": (_i, [_vp]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_plan_use_window": (_i, [_vp, _vp]),
-    "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp]),
-    "oct_plan_use_windows": (_i, [_vp, _vp, _ll]),
+    "oct_topfrac_metric_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp, _sz, _vp]),
+    "oct_plan_use_windows": (_i, [_vp, _vp, _ll, _ll]),
+    "oct_spectral_repeat": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_depth_median": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp]),
-    "oct_logmag_crop": (_i, [_vp, _i, _i, _ll, _i, _i, _i, _i, _i, _i, _f, _f, _vp, _i, _i, _vp]),
+    "oct_logmag_crop": (_i, [_vp, _i, _i, _ll, _i, _i, _i, _i, _i, _i, _f, _f, _vp, _i, _vp]),
     "oct_cnn_conv_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_data": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_weight_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),

@@ -25,48 +25,113 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 constexpr int kCnnThreads = 256;
-constexpr int kConvCoB = 16;   // output channels per thread (forward)
+constexpr int kConvCoB = 16;   // output channels per thread / wave (forward)
 constexpr int kConvCiB = 16;   // input channels per thread (backward data)
 
 // y[n, co, oy, ox] = relu(b[co] + sum_ci sum_{ky,kx} w[co, ci, ky, kx] x[n, ci, 2oy-1+ky, 2ox-1+kx])
-// Grid (ceil(N Ho Wo / 256), Co / 16); summation order: ci ascending, then ky, kx.
+// Block = an 8 x 32 tile of output positions of one image and 16 output channels; the input
+// patch of the tile (17 x 65 pixels) and the 16 channels' weights are staged in LDS for 8 input
+// channels at a time, and every thread forms its position's 16 channels from LDS.  Summation
+// order per output: ci ascending, then ky, kx.  Grid (ceil(Wo / 32), ceil(Ho / 8), N Co / 16).
+constexpr int kConvTY = 8, kConvTX = 32, kConvCC = 8;
+constexpr int kConvPH = 2 * kConvTY + 1, kConvPW = 2 * kConvTX + 1;
+
 __global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* __restrict__ x,
                                                                const uint16_t* __restrict__ w,
                                                                const float* __restrict__ bias,
                                                                uint16_t* __restrict__ y, int N, int Ci, int H,
                                                                int W, int Co) {
+    __shared__ float patch[kConvCC][kConvPH][kConvPW];
+    __shared__ float wl[kConvCC][9][16];     // this chunk's weights, [ci][tap][channel]
     const int Ho = H >> 1, Wo = W >> 1;
-    const long long P = (long long)N * Ho * Wo;
-    const long long p = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
-    if (p >= P) return;
-    const int co0 = blockIdx.y * kConvCoB;
+    const int groups = Co / 16;
+    const int n = blockIdx.z / groups, co0 = (blockIdx.z - (blockIdx.z / groups) * groups) * 16;
+    const int oyb = blockIdx.y * kConvTY, oxb = blockIdx.x * kConvTX;
+    const int ty = threadIdx.x / kConvTX, tx = threadIdx.x - (threadIdx.x / kConvTX) * kConvTX;
+    const int iyb = 2 * oyb - 1, ixb = 2 * oxb - 1;
+    float acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = bias[co0 + j];
+    for (int c0 = 0; c0 < Ci; c0 += kConvCC) {
+        const int cc = min(kConvCC, Ci - c0);
+        for (int e = threadIdx.x; e < cc * kConvPH * kConvPW; e += kCnnThreads) {
+            const int c = e / (kConvPH * kConvPW);
+            const int r = e - c * (kConvPH * kConvPW);
+            const int py = r / kConvPW, px = r - (r / kConvPW) * kConvPW;
+            const int iy = iyb + py, ix = ixb + px;
+            patch[c][py][px] = (iy >= 0 && iy < H && ix >= 0 && ix < W)
+                                   ? bf2f(x[(((long long)n * Ci + c0 + c) * H + iy) * W + ix])
+                                   : 0.f;
+        }
+        for (int e = threadIdx.x; e < cc * 9 * 16; e += kCnnThreads) {
+            const int j = e / (cc * 9), r = e - (e / (cc * 9)) * (cc * 9);     // r = c 9 + tap
+            wl[r / 9][r - (r / 9) * 9][j] = bf2f(w[((long long)(co0 + j) * Ci + c0) * 9 + r]);
+        }
+        __syncthreads();
+        for (int c = 0; c < cc; ++c) {
+            float v[9];
+#pragma unroll
+            for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+                for (int kx = 0; kx < 3; ++kx) v[3 * ky + kx] = patch[c][2 * ty + ky][2 * tx + kx];
+#pragma unroll
+            for (int k = 0; k < 9; ++k)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(wl[c][k][j], v[k], acc[j]);
+        }
+        __syncthreads();
+    }
+    const int oy = oyb + ty, ox = oxb + tx;
+    if (oy < Ho && ox < Wo) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            y[((long long)n * Co + co0 + j) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(acc[j], 0.f));
+    }
+}
+
+// The same convolution for layers with few output positions (small batches, deep layers): one
+// wave per (position, 16 output channels); lanes split the reduction over kk = ci 9 + tap (the
+// weights of one output channel are contiguous in kk), then a wave reduction; bias added last.
+// Grid ceil(N Ho Wo (Co / 16) / 4) blocks of 4 waves.
+__global__ __launch_bounds__(kCnnThreads) void conv_fwd_wave_kernel(const uint16_t* __restrict__ x,
+                                                                    const uint16_t* __restrict__ w,
+                                                                    const float* __restrict__ bias,
+                                                                    uint16_t* __restrict__ y, int N, int Ci, int H,
+                                                                    int W, int Co) {
+    const int Ho = H >> 1, Wo = W >> 1;
+    const int groups = Co / 16;
+    const long long task = (long long)blockIdx.x * (kCnnThreads / 64) + (threadIdx.x >> 6);
+    if (task >= (long long)N * Ho * Wo * groups) return;                 // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const int g = (int)(task % groups);
+    const long long p = task / groups;
     const int ox = (int)(p % Wo);
     const long long t = p / Wo;
     const int oy = (int)(t % Ho);
     const int n = (int)(t / Ho);
-    float acc[kConvCoB];
+    const int K = Ci * 9;
+    const int co0 = g * 16;
+    float acc[16];
 #pragma unroll
-    for (int j = 0; j < kConvCoB; ++j) acc[j] = bias[co0 + j];
-    const int iy0 = 2 * oy - 1, ix0 = 2 * ox - 1;     // iy0 + 2 <= H - 1 and ix0 + 2 <= W - 1 always
-    for (int ci = 0; ci < Ci; ++ci) {
-        const uint16_t* xp = x + ((long long)n * Ci + ci) * H * W;
-        float v[9];
+    for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+    const uint16_t* xn = x + (long long)n * Ci * H * W;
+    for (int kk = lane; kk < K; kk += 64) {
+        const int ci = kk / 9, tap = kk - (kk / 9) * 9;
+        const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
+        const float v = (iy >= 0 && ix >= 0) ? bf2f(xn[((long long)ci * H + iy) * W + ix]) : 0.f;
 #pragma unroll
-        for (int ky = 0; ky < 3; ++ky)
-#pragma unroll
-            for (int kx = 0; kx < 3; ++kx) {
-                const int iy = iy0 + ky, ix = ix0 + kx;
-                v[3 * ky + kx] = (iy >= 0 && ix >= 0) ? bf2f(xp[(long long)iy * W + ix]) : 0.f;
-            }
-        const uint16_t* wp = w + ((long long)co0 * Ci + ci) * 9;
-#pragma unroll
-        for (int j = 0; j < kConvCoB; ++j)
-#pragma unroll
-            for (int k = 0; k < 9; ++k) acc[j] = __builtin_fmaf(bf2f(wp[(long long)j * Ci * 9 + k]), v[k], acc[j]);
+        for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(bf2f(w[(long long)(co0 + j) * K + kk]), v, acc[j]);
     }
 #pragma unroll
-    for (int j = 0; j < kConvCoB; ++j)
-        y[((long long)n * Co + co0 + j) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(acc[j], 0.f));
+    for (int j = 0; j < 16; ++j)
+        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off);
+    if (lane < 16) {
+        float v = acc[0];
+#pragma unroll
+        for (int j = 1; j < 16; ++j) v = (lane == j) ? acc[j] : v;
+        v += bias[co0 + lane];
+        y[((long long)n * Co + co0 + lane) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(v, 0.f));
+    }
 }
 
 // dx[n, ci, iy, ix] = sum_co sum_taps w[co, ci, ky, kx] dpre[n, co, oy, ox] over iy = 2oy-1+ky,
@@ -194,21 +259,27 @@ __global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_reduce_kernel(con
     }
 }
 
-// y[n, o] = b[o] + sum_k w[o, k] x[n, k] (k ascending); ReLU + bf16 store, or a float32 store.
+// y[n, o] = b[o] + sum_k w[o, k] x[n, k]: one wave per output (lanes over k, then a wave
+// reduction); ReLU + bf16 store, or a float32 store.  Grid ceil(N O / 4) blocks of 4 waves.
 __global__ __launch_bounds__(kCnnThreads) void fc_fwd_kernel(const uint16_t* __restrict__ x,
                                                              const uint16_t* __restrict__ w,
                                                              const float* __restrict__ bias,
                                                              uint16_t* __restrict__ ybf, float* __restrict__ yf,
                                                              int N, int K, int O) {
-    const long long i = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
-    if (i >= (long long)N * O) return;
+    const long long i = (long long)blockIdx.x * (kCnnThreads / 64) + (threadIdx.x >> 6);
+    if (i >= (long long)N * O) return;                    // uniform per wave
+    const int lane = threadIdx.x & 63;
     const int n = (int)(i / O), o = (int)(i - (long long)(i / O) * O);
     const uint16_t* xr = x + (long long)n * K;
     const uint16_t* wr = w + (long long)o * K;
-    float acc = bias[o];
-    for (int k = 0; k < K; ++k) acc = __builtin_fmaf(bf2f(wr[k]), bf2f(xr[k]), acc);
-    if (ybf) ybf[i] = f2bf(fmaxf(acc, 0.f));
-    else yf[i] = acc;
+    float acc = 0.f;
+    for (int k = lane; k < K; k += 64) acc = __builtin_fmaf(bf2f(wr[k]), bf2f(xr[k]), acc);
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) {
+        acc += bias[o];
+        if (ybf) ybf[i] = f2bf(fmaxf(acc, 0.f));
+        else yf[i] = acc;
+    }
 }
 
 // dx[n, k] = sum_o dpre[n, o] w[o, k] (o ascending), times (xmask[n, k] > 0) when given.
@@ -307,19 +378,21 @@ __global__ __launch_bounds__(kCnnThreads) void adam_kernel(float* __restrict__ p
     pbf[i] = f2bf(pi);
 }
 
-// Network input from a magnitude volume (B, nX, rows) (row stride `xstride` floats per A-scan):
-//   out[b, c, h, w] = bf16(clamp((20 log10(max_{j < zpool} mag[b, x0 + w, z0 + h zpool + j]) - offset)
+// Network input from magnitude volumes (C B, nX, rows), channel-major (image v = c B + b), row
+// stride `xstride` floats per A-scan:
+//   out[b, c, h, w] = bf16(clamp((20 log10(max_{j < zpool} mag[v, x0 + w, z0 + h zpool + j]) - offset)
 //                                 * inv_scale, -8, 8))
-// with magnitudes below 1e-30 taken as 1e-30.  Grid (ceil(H W / 256), B); h runs fastest.
+// with magnitudes below 1e-30 taken as 1e-30.  Grid (ceil(H W / 256), C B); h runs fastest.
 __global__ __launch_bounds__(kCnnThreads) void logmag_crop_kernel(const float* __restrict__ mag, long long xstride,
                                                                   int nX, int x0, int z0, int zpool, int H, int W,
                                                                   float offset, float inv_scale,
-                                                                  uint16_t* __restrict__ out, int C, int c) {
-    const int b = blockIdx.y;
+                                                                  uint16_t* __restrict__ out, int C, int B) {
+    const int v = blockIdx.y;
+    const int c = v / B, b = v - (v / B) * B;
     const int e = blockIdx.x * kCnnThreads + threadIdx.x;
     if (e >= H * W) return;
     const int h = e % H, w = e / H;
-    const float* src = mag + ((long long)b * nX + x0 + w) * xstride + z0 + (long long)h * zpool;
+    const float* src = mag + ((long long)v * nX + x0 + w) * xstride + z0 + (long long)h * zpool;
     float m = src[0];
     for (int j = 1; j < zpool; ++j) m = fmaxf(m, src[j]);
     const float db = 20.0f * log10f(fmaxf(m, 1e-30f));

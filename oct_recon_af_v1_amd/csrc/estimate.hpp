@@ -3,84 +3,149 @@
 // Reference: octrecon/estimation/dispersion.py:46-63 (_metric 'top_percentile': I = A^2 in the
 // magnitude's float32, cast to float64; k = max(1, int(M * 0.01)); mean of the k largest; log).
 //
-// Exact top-k by radix selection on the IEEE bit patterns of I (non-negative floats order as
-// unsigned integers): four 8-bit passes narrow the k-th largest value, then one pass sums
-// everything above it in float64 and adds the ties.  One workgroup per B-scan.
 #pragma once
 #include "oct_common.hpp"
 
 namespace oct {
 
-constexpr int kSelThreads = 1024;
+// Exact top-k by radix selection on the IEEE bit patterns of I (non-negative floats order as
+// unsigned integers), spread over many workgroups per B-scan: three histogram passes fix the
+// bits [20, 31), [10, 20) and [0, 10) of the k-th largest key (each followed by a one-block
+// select), then one pass sums everything above it in float64 (per-workgroup partials, reduced
+// in a fixed order), and the ties at the threshold are added.  Deterministic: counts are
+// integers and every float64 sum runs in a fixed order.
+constexpr int kTfThreads = 256;
+constexpr int kTfBins = 2048;
 
-__global__ __launch_bounds__(kSelThreads) void topfrac_metric_kernel(const float* __restrict__ mag, int nX, int nz,
-                                                                     int zmin, long long kk, double* __restrict__ out) {
-    const int b = blockIdx.x;
+struct TopfracState {
+    unsigned prefix;      // key bits fixed so far
+    int pad;
+    long long k_rem;      // rank (from the top, 1-based) of the threshold among keys sharing the prefix
+};
+
+__device__ __forceinline__ unsigned tf_key(const float* A, long long e, int w, int nz, int zmin) {
+    const long long x = e / w, z = zmin + e - x * w;
+    const float a = A[x * nz + z];
+    return __builtin_bit_cast(unsigned, a * a);
+}
+
+__global__ __launch_bounds__(kTfThreads) void topfrac_init_kernel(unsigned* __restrict__ hist, TopfracState* st,
+                                                                  long long kk, int B) {
+    const long long i = (long long)blockIdx.x * kTfThreads + threadIdx.x;
+    if (i < (long long)B * kTfBins) hist[i] = 0u;
+    if (i < B) {
+        st[i].prefix = 0u;
+        st[i].k_rem = kk;
+    }
+}
+
+// pass: bits [lo, lo + nbits) of keys whose bits above lo + nbits equal the prefix.  Grid (S, B).
+__global__ __launch_bounds__(kTfThreads) void topfrac_hist_kernel(const float* __restrict__ mag, int nX, int nz,
+                                                                  int zmin, const TopfracState* __restrict__ st,
+                                                                  unsigned* __restrict__ hist, int lo, int nbits) {
+    __shared__ unsigned h[kTfBins];
+    const int b = blockIdx.y;
+    for (int i = threadIdx.x; i < kTfBins; i += kTfThreads) h[i] = 0u;
+    __syncthreads();
     const float* A = mag + (long long)b * nX * nz;
     const int w = nz - zmin;
     const long long M = (long long)nX * w;
-    __shared__ unsigned hist[256];
-    __shared__ unsigned s_prefix, s_mask;
-    __shared__ long long s_k;
-    __shared__ double s_sum[kSelThreads];
-    __shared__ unsigned long long s_cnt[kSelThreads];
-    if (threadIdx.x == 0) {
-        s_prefix = 0u;
-        s_mask = 0u;
-        s_k = kk;          // rank (1-based, from the top) of the value searched for
+    const int hi = lo + nbits;
+    const unsigned prefix = st[b].prefix;
+    const unsigned dmask = (1u << nbits) - 1u;
+    for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
+        const unsigned u = tf_key(A, e, w, nz, zmin);
+        if (hi >= 32 || (u >> hi) == (prefix >> hi)) atomicAdd(&h[(u >> lo) & dmask], 1u);
     }
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 24 - 8 * pass;
-        for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0u;
-        __syncthreads();
-        const unsigned prefix = s_prefix, mask = s_mask;
-        for (long long e = threadIdx.x; e < M; e += blockDim.x) {
-            const long long x = e / w, z = zmin + e - x * w;
-            const float a = A[x * nz + z];
-            const unsigned u = __builtin_bit_cast(unsigned, a * a);
-            if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            long long k = s_k;
-            int d = 255;
-            for (; d > 0; --d) {
-                if ((long long)hist[d] >= k) break;
-                k -= hist[d];
-            }
-            s_prefix = prefix | ((unsigned)d << shift);
-            s_mask = mask | (255u << shift);
-            s_k = k;
-        }
-        __syncthreads();
-    }
-    const unsigned thr = s_prefix;        // bit pattern of the k-th largest value
-    double sum = 0.0;
-    unsigned long long cnt = 0;
-    for (long long e = threadIdx.x; e < M; e += blockDim.x) {
-        const long long x = e / w, z = zmin + e - x * w;
-        const float a = A[x * nz + z];
-        const float v = a * a;
-        if (__builtin_bit_cast(unsigned, v) > thr) {
-            sum += (double)v;
-            ++cnt;
-        }
-    }
-    s_sum[threadIdx.x] = sum;
-    s_cnt[threadIdx.x] = cnt;
     __syncthreads();
-    for (int s = kSelThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            s_sum[threadIdx.x] += s_sum[threadIdx.x + s];
-            s_cnt[threadIdx.x] += s_cnt[threadIdx.x + s];
+    unsigned* g = hist + (long long)b * kTfBins;
+    for (int i = threadIdx.x; i <= (int)dmask; i += kTfThreads)
+        if (h[i]) atomicAdd(&g[i], h[i]);
+}
+
+// One block per B-scan: the digit d holding the k_rem-th largest key; prefix |= d << lo,
+// k_rem -= (count of keys above d); clears the histogram for the next pass.
+__global__ __launch_bounds__(kTfThreads) void topfrac_select_kernel(unsigned* __restrict__ hist, TopfracState* st,
+                                                                    int lo, int nbits) {
+    constexpr int kPer = kTfBins / kTfThreads;     // bins per thread, descending digit order
+    __shared__ unsigned long long tot[kTfThreads];
+    __shared__ int s_d;
+    __shared__ long long s_above;
+    const int b = blockIdx.x;
+    const int nb = 1 << nbits;
+    unsigned* g = hist + (long long)b * kTfBins;
+    unsigned c[kPer];
+    unsigned long long t = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int d = nb - 1 - (threadIdx.x * kPer + q);      // thread 0 holds the largest digits
+        c[q] = d >= 0 ? g[d] : 0u;
+        t += c[q];
+    }
+    tot[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long k = st[b].k_rem;
+        long long above = 0;
+        int th = 0;
+        for (; th < kTfThreads - 1; ++th) {
+            if (above + (long long)tot[th] >= k) break;
+            above += (long long)tot[th];
         }
+        s_d = th;
+        s_above = above;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x == s_d) {
+        const long long k = st[b].k_rem;
+        long long above = s_above;
+        int q = 0;
+        for (; q < kPer - 1; ++q) {
+            if (above + (long long)c[q] >= k) break;
+            above += (long long)c[q];
+        }
+        const int d = nb - 1 - (threadIdx.x * kPer + q);
+        st[b].prefix |= (unsigned)d << lo;
+        st[b].k_rem = k - above;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kTfThreads) g[i] = 0u;
+}
+
+// Sum in float64 of the keys above the threshold: per-block partial part[b * S + s].  Grid (S, B).
+__global__ __launch_bounds__(kTfThreads) void topfrac_sum_kernel(const float* __restrict__ mag, int nX, int nz,
+                                                                 int zmin, const TopfracState* __restrict__ st,
+                                                                 double* __restrict__ part) {
+    __shared__ double red[kTfThreads];
+    const int b = blockIdx.y;
+    const float* A = mag + (long long)b * nX * nz;
+    const int w = nz - zmin;
+    const long long M = (long long)nX * w;
+    const unsigned thr = st[b].prefix;
+    double sum = 0.0;
+    for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
+        const unsigned u = tf_key(A, e, w, nz, zmin);
+        if (u > thr) sum += (double)__builtin_bit_cast(float, u);
+    }
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = kTfThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        const double tv = (double)__builtin_bit_cast(float, thr);
-        const double total = s_sum[0] + (double)(kk - (long long)s_cnt[0]) * tv;
-        out[b] = log(total / (double)kk);
-    }
+    if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = red[0];
+}
+
+// out[b] = log((sum of partials in order + k_rem * threshold) / k).  Grid (ceil(B / 256)).
+__global__ __launch_bounds__(kTfThreads) void topfrac_final_kernel(const TopfracState* __restrict__ st,
+                                                                   const double* __restrict__ part, int S, long long kk,
+                                                                   int B, double* __restrict__ out) {
+    const int b = blockIdx.x * kTfThreads + threadIdx.x;
+    if (b >= B) return;
+    double total = 0.0;
+    for (int s = 0; s < S; ++s) total += part[(long long)b * S + s];
+    total += (double)st[b].k_rem * (double)__builtin_bit_cast(float, st[b].prefix);
+    out[b] = log(total / (double)kk);
 }
 
 // Dispersion search windows: out[b, j] = hann_n[j] exp(-i beta_b dk2[j]) (hann_n = Hann/rms/N,

@@ -47,6 +47,7 @@ struct oct_plan {
     float* win;   // window / N
     const float* win_override;   // caller's device window / N (dispersion search), or null
     long long win_fstride;       // floats between per-file override windows (0: one window)
+    long long win_rstride;       // floats between the override windows of consecutive batch repeats
     float2* tw;   // (32, 64) transform twiddles [k1][n2]
     int waves;    // workgroup shape of the fused kernel: 8 (LDS twiddles) or 4
 };
@@ -210,14 +211,38 @@ int oct_background(const oct_plan* p, const void* raw, int raw_dtype, int n_file
     }
 }
 
+static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                         int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
+                         long long out_stride, const float* bg, void* stream);
+
 int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                  int apod_size, int out_kind, int row0, int rows, void* out, long long out_stride,
                  const float* bg, void* stream) {
+    return spectral_impl(p, raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, 0, -1, out_kind, row0, rows, out,
+                         out_stride, bg, stream);
+}
+
+int oct_spectral_repeat(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                        int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
+                        long long out_stride, const float* bg, void* stream) {
+    if (n_repeat < 1) return fail(OCT_EINVAL, "n_repeat %d", n_repeat);
+    return spectral_impl(p, raw, raw_dtype, n_files, lines, apod_mode, apod_size, n_repeat, x0, n_x, out_kind, row0,
+                         rows, out, out_stride, bg, stream);
+}
+
+static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
+                         int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
+                         long long out_stride, const float* bg, void* stream) {
     if (!p || !out || !bg) return fail(OCT_EINVAL, "null argument");
     int line0, nX;
     int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, &line0, &nX);
     if (rc) return rc;
     if (n_files == 0) return OCT_OK;
+    if (n_x >= 0) {                         // a window of the scan lines
+        if (x0 < 0 || n_x < 1 || x0 + n_x > nX) return fail(OCT_EINVAL, "scan lines [%d, %d) outside [0, %d)", x0, x0 + n_x, nX);
+        line0 += x0;
+        nX = n_x;
+    }
     if (out_kind < OCT_OUT_MAG || out_kind > OCT_OUT_LINEARISED) return fail(OCT_EINVAL, "out kind %d", out_kind);
     const int nz = p->n / 2;
     if (out_kind != OCT_OUT_LINEARISED) {
@@ -236,9 +261,11 @@ int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files,
     a.band_w = p->band_w;
     a.win = p->win_override ? p->win_override : p->win;
     a.win_fstride = p->win_override ? p->win_fstride : 0;
+    a.win_rstride = p->win_override ? p->win_rstride : 0;
+    a.raw_files = n_files;
     a.tw = p->tw;
     a.out = out;
-    a.M = (long long)n_files * nX;
+    a.M = (long long)n_repeat * n_files * nX;
     a.lines = lines;
     a.line0 = line0;
     a.nX = nX;
@@ -409,31 +436,67 @@ int oct_plan_use_window(oct_plan* p, const float* dev_window) {
     if (!p) return fail(OCT_EINVAL, "null plan");
     p->win_override = dev_window;
     p->win_fstride = 0;
+    p->win_rstride = 0;
     return OCT_OK;
 }
 
-int oct_plan_use_windows(oct_plan* p, const float* dev_windows, long long file_stride) {
+int oct_plan_use_windows(oct_plan* p, const float* dev_windows, long long file_stride, long long repeat_stride) {
     if (!p) return fail(OCT_EINVAL, "null plan");
-    if (file_stride < 0 || (file_stride > 0 && file_stride < 2LL * p->n))
-        return fail(OCT_EINVAL, "window stride %lld (0 or >= 2N)", file_stride);
-    if (reinterpret_cast<uintptr_t>(dev_windows) % 8 || file_stride % 2)
-        return fail(OCT_EINVAL, "windows must be 8-byte aligned complex pairs");
+    for (long long st : {file_stride, repeat_stride})
+        if (st < 0 || (st > 0 && st < 2LL * p->n) || st % 2)
+            return fail(OCT_EINVAL, "window stride %lld (0 or an even count >= 2N)", st);
+    if (reinterpret_cast<uintptr_t>(dev_windows) % 8) return fail(OCT_EINVAL, "windows must be 8-byte aligned");
     p->win_override = dev_windows;
     p->win_fstride = dev_windows ? file_stride : 0;
+    p->win_rstride = dev_windows ? repeat_stride : 0;
     return OCT_OK;
+}
+
+static int topfrac_splits(int n_bscans, long long M) {
+    long long s = (1024 + n_bscans - 1) / n_bscans;
+    s = std::min<long long>(s, std::max<long long>(1, M / 4096));
+    return (int)std::max<long long>(1, std::min<long long>(s, 4096));
+}
+
+size_t oct_topfrac_metric_workspace_bytes(int n_bscans, int n_x, int n_z, int z_min) {
+    if (n_bscans <= 0 || n_x <= 0 || z_min < 0 || z_min >= n_z) return 0;
+    const long long M = (long long)n_x * (n_z - z_min);
+    const size_t S = (size_t)topfrac_splits(n_bscans, M);
+    return (size_t)n_bscans * (sizeof(unsigned) * oct::kTfBins + sizeof(oct::TopfracState) + sizeof(double) * S);
 }
 
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
-                       void* stream) {
-    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    if (!mag || !out || !workspace) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
     if (n_x <= 0 || z_min < 0 || z_min >= n_z) return fail(OCT_EINVAL, "bad metric shape (%d, %d, zmin %d)", n_x, n_z, z_min);
     if (!(frac > 0.0 && frac <= 1.0)) return fail(OCT_EINVAL, "fraction %g outside (0, 1]", frac);
+    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    if (workspace_bytes < oct_topfrac_metric_workspace_bytes(n_bscans, n_x, n_z, z_min))
+        return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes,
+                    oct_topfrac_metric_workspace_bytes(n_bscans, n_x, n_z, z_min));
+    if (reinterpret_cast<uintptr_t>(workspace) % 16) return fail(OCT_EINVAL, "workspace must be 16-byte aligned");
     const long long M = (long long)n_x * (n_z - z_min);
     long long k = (long long)(M * frac);
     if (k < 1) k = 1;
-    hipLaunchKernelGGL(oct::topfrac_metric_kernel, dim3(n_bscans), dim3(oct::kSelThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), mag, n_x, n_z, z_min, k, out);
+    const int S = topfrac_splits(n_bscans, M);
+    unsigned* hist = static_cast<unsigned*>(workspace);
+    auto* st = reinterpret_cast<oct::TopfracState*>(hist + (size_t)n_bscans * oct::kTfBins);
+    auto* part = reinterpret_cast<double*>(st + n_bscans);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const long long ninit = (long long)n_bscans * oct::kTfBins;
+    hipLaunchKernelGGL(oct::topfrac_init_kernel, dim3((unsigned)((ninit + oct::kTfThreads - 1) / oct::kTfThreads)),
+                       dim3(oct::kTfThreads), 0, s, hist, st, k, n_bscans);
+    const int lo[3] = {20, 10, 0}, nb[3] = {11, 10, 10};
+    for (int p = 0; p < 3; ++p) {
+        hipLaunchKernelGGL(oct::topfrac_hist_kernel, dim3(S, n_bscans), dim3(oct::kTfThreads), 0, s, mag, n_x, n_z, z_min,
+                           st, hist, lo[p], nb[p]);
+        hipLaunchKernelGGL(oct::topfrac_select_kernel, dim3(n_bscans), dim3(oct::kTfThreads), 0, s, hist, st, lo[p], nb[p]);
+    }
+    hipLaunchKernelGGL(oct::topfrac_sum_kernel, dim3(S, n_bscans), dim3(oct::kTfThreads), 0, s, mag, n_x, n_z, z_min, st,
+                       part);
+    hipLaunchKernelGGL(oct::topfrac_final_kernel, dim3((n_bscans + oct::kTfThreads - 1) / oct::kTfThreads),
+                       dim3(oct::kTfThreads), 0, s, st, part, S, k, n_bscans, out);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
@@ -628,10 +691,22 @@ int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
     if (!conv_shape_ok(n, ci, h, wd, co)) return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d", n, ci, h, wd, co);
     if (co % oct::kConvCoB) return fail(OCT_EINVAL, "output channels %d not a multiple of %d", co, oct::kConvCoB);
     const long long P = (long long)n * (h / 2) * (wd / 2);
-    const long long g = (P + oct::kCnnThreads - 1) / oct::kCnnThreads;
-    if (g > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
-    hipLaunchKernelGGL(oct::conv_fwd_kernel, dim3((unsigned)g, co / oct::kConvCoB), dim3(oct::kCnnThreads), 0,
-                       reinterpret_cast<hipStream_t>(stream), x, w, bias, y, n, ci, h, wd, co);
+    if ((P * (co / 16) + 3) / 4 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int Ho = h / 2, Wo = wd / 2;
+    if (Wo >= 2 * oct::kConvTX) {
+        // LDS-tiled: 8 x 32 output positions x 16 channels per block
+        const long long gz = (long long)n * (co / 16);
+        if (gz > 65535) return fail(OCT_EINVAL, "batch too large for one launch");
+        hipLaunchKernelGGL(oct::conv_fwd_kernel, dim3((Wo + oct::kConvTX - 1) / oct::kConvTX,
+                                                      (Ho + oct::kConvTY - 1) / oct::kConvTY, (unsigned)gz),
+                           dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);
+    } else {
+        // narrow layers: one wave per (position, 16 channels), lanes across the reduction
+        const long long waves = P * (co / 16);
+        hipLaunchKernelGGL(oct::conv_fwd_wave_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(oct::kCnnThreads), 0, s,
+                           x, w, bias, y, n, ci, h, wd, co);
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
@@ -689,9 +764,10 @@ int oct_cnn_fc_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint
     if ((y_relu_bf16 != nullptr) == (y_linear != nullptr)) return fail(OCT_EINVAL, "exactly one output (ReLU bf16 or linear f32)");
     if (n <= 0 || k <= 0 || o <= 0) return fail(OCT_EINVAL, "bad fc shape (%d, %d) -> %d", n, k, o);
     const long long tot = (long long)n * o;
-    hipLaunchKernelGGL(oct::fc_fwd_kernel, dim3((unsigned)((tot + oct::kCnnThreads - 1) / oct::kCnnThreads)),
-                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), x, w, bias, y_relu_bf16, y_linear,
-                       n, k, o);
+    const long long wpb = oct::kCnnThreads / 64;                  // outputs (waves) per block
+    if ((tot + wpb - 1) / wpb > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(oct::fc_fwd_kernel, dim3((unsigned)((tot + wpb - 1) / wpb)), dim3(oct::kCnnThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), x, w, bias, y_relu_bf16, y_linear, n, k, o);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
@@ -742,21 +818,21 @@ int oct_cnn_adam(float* params, const float* grads, float* m, float* v, uint16_t
 }
 
 int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride, int n_rows, int x0, int z0, int zpool,
-                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, int channel,
-                    void* stream) {
+                    int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, void* stream) {
     if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
     if (height <= 0 || width <= 0 || zpool <= 0 || x0 < 0 || z0 < 0 || x0 + width > n_x ||
         z0 + (long long)height * zpool > n_rows || n_rows > x_stride)
         return fail(OCT_EINVAL, "crop x [%d, %d) z [%d, %lld) outside (%d, %d)", x0, x0 + width, z0,
                     z0 + (long long)height * zpool, n_x, n_rows);
-    if (channel < 0 || channel >= channels) return fail(OCT_EINVAL, "channel %d of %d", channel, channels);
+    if (channels < 1) return fail(OCT_EINVAL, "%d channels", channels);
     if (!(scale_db > 0.f)) return fail(OCT_EINVAL, "scale must be positive");
-    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    if ((long long)n_bscans * channels > 65535) return fail(OCT_EINVAL, "at most 65535 images per call");
     const long long hw = (long long)height * width;
-    hipLaunchKernelGGL(oct::logmag_crop_kernel, dim3((unsigned)((hw + oct::kCnnThreads - 1) / oct::kCnnThreads), n_bscans),
+    hipLaunchKernelGGL(oct::logmag_crop_kernel, dim3((unsigned)((hw + oct::kCnnThreads - 1) / oct::kCnnThreads),
+                                                     n_bscans * channels),
                        dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, x0, z0, zpool,
-                       height, width, offset_db, 1.0f / scale_db, out, channels, channel);
+                       height, width, offset_db, 1.0f / scale_db, out, channels, n_bscans);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

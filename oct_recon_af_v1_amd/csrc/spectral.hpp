@@ -39,6 +39,8 @@ struct SpectralArgs {
     const float* band_w;     // (N, TP) tap weights
     const float* win;        // (N, 2) complex window scaled by 1/N
     long long win_fstride;   // floats between the windows of consecutive raw files (0: one window)
+    long long win_rstride;   // floats between the windows of consecutive repeats of the batch
+    int raw_files;           // files in raw / bg: A-scan file index f reads raw file f % raw_files
     const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), [k1][n2] (W = 4 variant)
     void* out;
     long long M;             // A-scans = B * nX
@@ -116,8 +118,9 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
 
         // ---- A: load, background, mean, d - m -> LDS
         if (wave < nvalid) {
-            const long long b = p / a.nX;
-            const long long x = p - b * a.nX;
+            const long long bv = p / a.nX;
+            const long long b = bv % a.raw_files;
+            const long long x = p - bv * a.nX;
             const RawT* r = reinterpret_cast<const RawT*>(a.raw) + ((b * a.lines + a.line0 + x) * (long long)kN);
             const float* bgr = a.bg + b * kN;
             float d[NLD][VEC];
@@ -205,7 +208,9 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         if (wave < nvalid) {
             cf* B = &buf[wave][0];
             const float* E = reinterpret_cast<const float*>(B) + kN;
-            const float2* wg = reinterpret_cast<const float2*>(a.win + (p / a.nX) * a.win_fstride);
+            const long long bv = p / a.nX;
+            const float2* wg = reinterpret_cast<const float2*>(a.win + (bv / a.raw_files) * a.win_rstride +
+                                                               (bv % a.raw_files) * a.win_fstride);
             cf v[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {

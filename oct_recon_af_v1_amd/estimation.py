@@ -47,6 +47,7 @@ class HipDispersionScorer:
         self._dk2 = torch.tensor(self.dk2, dtype=torch.float64, device=xp.device)
         self.nz = N // 2
         self._mag = None
+        self._ws = None
         self.n_evals = 0
         self.set_raw(raw)
 
@@ -59,18 +60,22 @@ class HipDispersionScorer:
         if self._mag is None or tuple(self._mag.shape) != shape:
             self._mag = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
 
-    def windows(self, betas=None, head=None, head_scale=1.0, offset=0.0, to_ref=1.0, count=None):
+    def windows(self, betas=None, head=None, head_scale=1.0, offset=0.0, to_ref=1.0, count=None, out=None):
         """(count, N, 2) float32 device windows hann/rms/N exp(-i beta dk2), float64 phase
         (oct_dispersion_windows): beta = betas (reference units), or per B-scan from a device
         network head: (head[b, 0] * head_scale + offset) * to_ref."""
         torch = self.torch
         if head is None:
-            b = torch.tensor(np.atleast_1d(np.asarray(betas, float)), dtype=torch.float64, device=self.xp.device)
+            b = torch.tensor(np.ascontiguousarray(np.atleast_1d(np.asarray(betas, float))), dtype=torch.float64,
+                             device=self.xp.device)
             count, hp, hs = b.numel(), None, 1
         else:
             b, hp, hs = None, head.data_ptr(), head.stride(0)
             count = head.shape[0] if count is None else count
-        out = torch.empty((count, self.sg0.n, 2), dtype=torch.float32, device=self.xp.device)
+        if out is None:
+            out = torch.empty((count, self.sg0.n, 2), dtype=torch.float32, device=self.xp.device)
+        elif tuple(out.shape) != (count, self.sg0.n, 2) or not out.is_contiguous():
+            raise ValueError(f"window buffer must be contiguous ({count}, {self.sg0.n}, 2)")
         _lib.check(_lib.lib().oct_dispersion_windows(None if b is None else b.data_ptr(), hp, hs, head_scale, offset,
                                                      to_ref, count, self._hann_n.data_ptr(), self._dk2.data_ptr(),
                                                      self.sg0.n, out.data_ptr(), self.xp.stream()),
@@ -79,8 +84,12 @@ class HipDispersionScorer:
         return out
 
     def _metric(self, out_row):
-        _lib.check(_lib.lib().oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX,
-                                                 TOP_FRACTION, out_row.data_ptr(), self.xp.stream()),
+        L = _lib.lib()
+        nb = L.oct_topfrac_metric_workspace_bytes(self.B, self.nX, self.nz, Z_MIN_PIX)
+        if self._ws is None or self._ws.numel() < nb:
+            self._ws = self.torch.empty((nb,), dtype=self.torch.uint8, device=self.xp.device)
+        _lib.check(L.oct_topfrac_metric(self._mag.data_ptr(), self.B, self.nX, self.nz, Z_MIN_PIX, TOP_FRACTION,
+                                        out_row.data_ptr(), self._ws.data_ptr(), self._ws.numel(), self.xp.stream()),
                    "oct_topfrac_metric")
 
     def magnitude_at(self, betas, out=None):
@@ -99,12 +108,31 @@ class HipDispersionScorer:
         out = self._mag if out is None else out
         L = _lib.lib()
         try:
-            _lib.check(L.oct_plan_use_windows(self.sp.plan.handle, win.data_ptr(), 2 * self.sg0.n if per_file else 0),
-                       "oct_plan_use_windows")
+            _lib.check(L.oct_plan_use_windows(self.sp.plan.handle, win.data_ptr(), 2 * self.sg0.n if per_file else 0,
+                                              0), "oct_plan_use_windows")
             self.sp.run(self.raw, _lib.OCT_OUT_MAG, 0, self.nz, out=out)
         finally:
             L.oct_plan_use_window(self.sp.plan.handle, None)
         self._keep = win                  # alive until the next call (the launch is asynchronous)
+        return out
+
+    def magnitude_repeat(self, win, n_repeat: int, bg, out, x0: int = 0, nx: int | None = None):
+        """|A| of the batch at n_repeat candidate sets in one launch: win (n_repeat, N, 2) one window
+        per repeat, or (n_repeat B, N, 2) one per repeat and B-scan; scan lines [x0, x0 + nx);
+        out (n_repeat B, nx, N/2)."""
+        N = self.sg0.n
+        per_file = win.shape[0] == n_repeat * self.B and not (self.B == 1)
+        if win.shape[0] not in (n_repeat, n_repeat * self.B):
+            raise ValueError(f"{win.shape[0]} windows for {n_repeat} repeats of {self.B} B-scans")
+        fstride, rstride = (2 * N, 2 * N * self.B) if per_file else (0, 2 * N)
+        L = _lib.lib()
+        try:
+            _lib.check(L.oct_plan_use_windows(self.sp.plan.handle, win.data_ptr(), fstride, rstride),
+                       "oct_plan_use_windows")
+            self.sp.run_repeat(self.raw, n_repeat, bg, out, x0=x0, nx=nx)
+        finally:
+            L.oct_plan_use_window(self.sp.plan.handle, None)
+        self._keep = win
         return out
 
     def scores_many(self, betas):

@@ -242,39 +242,53 @@ class InputBuilder:
         if not spec.relative:
             self.probe_win = scorer.windows([b * self.to_ref for b in spec.probes_idx])
 
-    def build(self, raw=None, out=None, centers_idx=None, centers_head=None):
-        """(B, 2, 256, 256) bf16 for the scorer's B-scans (or `raw`, which becomes its set).  A
-        relative spec takes the centres (index units) as a host array `centers_idx`, or as
-        `centers_head` = (device head (B, 4) float32, beta scale): centre = head[b, 0] * scale,
-        turned into windows on the device without a host round trip."""
+    def build(self, raw=None, out=None, centers_idx=None, centers_head=None, bg=None):
+        """(B, 2, 256, 256) bf16 for the scorer's B-scans (or `raw`, which becomes its set): both
+        probes in one spectral launch (oct_spectral_repeat) and one crop launch.  A relative spec
+        takes the centres (index units) as a host array `centers_idx`, or as `centers_head` =
+        (device head (B, 4) float32, beta scale): centre = head[b, 0] * scale, turned into
+        windows on the device without a host round trip.  bg: the batch's background rows
+        (computed here when not given)."""
         sc, sp = self.scorer, self.spec
         torch = sc.torch
         if raw is not None:
             sc.set_raw(raw)
         if sc.nX < sp.n_x_crop or sp.n_z > sc.nz or sp.n_z // sp.zpool != IMG:
             raise ValueError(f"B-scans of {sc.nX} A-scans x {sc.nz} rows cannot give the {IMG} x {IMG} crop")
-        B = sc.B
-        if sp.relative and centers_head is None:
-            c = np.asarray(centers_idx, float).reshape(-1)
-            if c.size != B:
-                raise ValueError(f"{c.size} centres for {B} B-scans")
-        if centers_head is not None and centers_head[0].shape[0] != B:
-            raise ValueError(f"{centers_head[0].shape[0]} heads for {B} B-scans")
-        out = torch.empty((B, 2, IMG, IMG), dtype=torch.bfloat16, device=sc.xp.device) if out is None else out
-        x0 = (sc.nX - sp.n_x_crop) // 2
-        L = _lib.lib()
-        for k, probe in enumerate(sp.probes_idx):
-            if sp.relative and centers_head is not None:
-                mag = sc.magnitude_with(sc.windows(head=centers_head[0], head_scale=centers_head[1], offset=probe,
-                                                   to_ref=self.to_ref))
-            elif sp.relative:
-                mag = sc.magnitude_with(sc.windows((c + probe) * self.to_ref))
+        B, P = sc.B, len(sp.probes_idx)
+        dev = sc.xp.device
+        if bg is None:
+            bg = sc.sp.background(sc.raw)
+        if sp.relative:
+            win = torch.empty((P * B, sc.sg0.n, 2), dtype=torch.float32, device=dev)
+            if centers_head is not None:
+                if centers_head[0].shape[0] != B:
+                    raise ValueError(f"{centers_head[0].shape[0]} heads for {B} B-scans")
+                for k, probe in enumerate(sp.probes_idx):
+                    sc.windows(head=centers_head[0], head_scale=centers_head[1], offset=probe, to_ref=self.to_ref,
+                               out=win[k * B:(k + 1) * B])
             else:
-                mag = sc.magnitude_with(self.probe_win[k])
-            _lib.check(L.oct_logmag_crop(mag.data_ptr(), B, sc.nX, mag.stride(1), sc.nz, x0, 0, sp.zpool, IMG,
-                                         sp.n_x_crop, sp.offset_db, sp.scale_db, out.data_ptr(), 2, k,
-                                         sc.xp.stream()), "oct_logmag_crop")
+                c = np.asarray(centers_idx, float).reshape(-1)
+                if c.size != B:
+                    raise ValueError(f"{c.size} centres for {B} B-scans")
+                sc.windows(np.concatenate([(c + probe) * self.to_ref for probe in sp.probes_idx]), out=win)
+        else:
+            win = self.probe_win
+        mag = self._mag(P * B)                       # only the cropped scan lines are reconstructed
+        x0 = (sc.nX - sp.n_x_crop) // 2
+        sc.magnitude_repeat(win, P, bg, mag, x0=x0, nx=sp.n_x_crop)
+        out = torch.empty((B, P, IMG, IMG), dtype=torch.bfloat16, device=dev) if out is None else out
+        _lib.check(_lib.lib().oct_logmag_crop(mag.data_ptr(), B, sp.n_x_crop, mag.stride(1), sc.nz, 0, 0, sp.zpool,
+                                              IMG, sp.n_x_crop, sp.offset_db, sp.scale_db, out.data_ptr(), P,
+                                              sc.xp.stream()), "oct_logmag_crop")
         return out
+
+    def _mag(self, n):
+        sc = self.scorer
+        shape = (n, self.spec.n_x_crop, sc.nz)
+        if getattr(self, "_magbuf", None) is None or tuple(self._magbuf.shape) != shape:
+            self._magbuf = sc.torch.empty(shape, dtype=sc.torch.float32, device=sc.xp.device)
+        return self._magbuf
 
 
 def focus_from_profiles(profiles, search_px=None, guess=None):
@@ -462,6 +476,7 @@ class LearnedEstimator:
         self._scorer_cls = HipDispersionScorer
         self.scorer = None
         self.builders = None
+        self._graphs = {}
 
     def _bind(self, raw):
         if self.scorer is None:
@@ -470,13 +485,37 @@ class LearnedEstimator:
         else:
             self.scorer.set_raw(raw)
 
-    def predict(self, raw):
+    def _heads(self):
+        """Both stages on the bound batch, entirely on the device (no host round trip)."""
+        bg = self.scorer.sp.background(self.scorer.raw)
+        h1 = self.nets[0].forward(self.builders[0].build(bg=bg))
+        h2 = self.nets[1].forward(self.builders[1].build(centers_head=(h1, self.specs[0].beta_scale), bg=bg))
+        return self.xp.torch.stack([h1, h2])
+
+    def predict(self, raw, graph: bool = False):
         """Network only: per B-scan beta_idx, beta_ref, focus (0-based bin), sigmas, confident
-        (plus the stage-1 beta)."""
-        self._bind(raw)
-        h1 = self.nets[0].forward(self.builders[0].build())
-        h2 = self.nets[1].forward(self.builders[1].build(centers_head=(h1, self.specs[0].beta_scale)))
-        H = self.xp.torch.stack([h1, h2]).cpu().numpy()          # the one host synchronisation
+        (plus the stage-1 beta).  graph=True replays the whole two-stage sequence as one HIP
+        graph captured per batch shape (the batch is copied into the graph's input buffer)."""
+        torch = self.xp.torch
+        if graph:
+            key = (tuple(raw.shape), raw.dtype)
+            if key not in self._graphs:
+                static = raw.clone()
+                self._bind(static)
+                self._heads()                                  # allocates every buffer
+                self.xp.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    heads = self._heads()
+                self._graphs[key] = (g, static, heads)
+            g, static, heads = self._graphs[key]
+            static.copy_(raw)
+            self._bind(static)
+            g.replay()
+            H = heads.cpu().numpy()
+        else:
+            self._bind(raw)
+            H = self._heads().cpu().numpy()                    # the one host synchronisation
         b1 = H[0][:, 0].astype(np.float64) * self.specs[0].beta_scale   # as the window kernel forms it
         b, f, sb, sf = denormalise(H[1], self.specs[1], center_idx=b1)
         return dict(beta_idx=b, beta_ref=beta_from_index_units(b, self.lambda_nm), focus=f, sigma_beta=sb,

@@ -203,6 +203,38 @@ class HipSpectralProcessor:
             ev.append((e0, e1))
         return out
 
+    def background(self, raw, out=None):
+        """Background rows (B, N) float32 of a raw batch (oct_background), for run_repeat."""
+        t = self._device_raw(raw)
+        B, lines, _ = t.shape
+        if out is None:
+            out = self.torch.empty((B, self.n), dtype=self.torch.float32, device=self.xp.device)
+        mode = _lib.OCT_APOD_LINES if self.apod_mode == "lines" else _lib.OCT_APOD_FRAME_MEAN
+        _lib.check(_lib.lib().oct_background(self.plan.handle, t.data_ptr(), self._code(t), B, lines, mode,
+                                             self.raw_apod_size, self.apod_group, out.data_ptr(), self.xp.stream()),
+                   "oct_background")
+        return out
+
+    def run_repeat(self, raw, n_repeat: int, bg, out, kind: int = _lib.OCT_OUT_MAG, row0: int = 0,
+                   rows: int | None = None, x0: int = 0, nx: int | None = None):
+        """The batch reconstructed n_repeat times in one launch (windows per repeat / file from
+        oct_plan_use_windows), scan lines [x0, x0 + nx) of every file; out (n_repeat B, nx, rows);
+        bg from background()."""
+        t = self._device_raw(raw)
+        B, lines, _ = t.shape
+        nX = self.n_scan_lines(lines) - x0 if nx is None else nx
+        rows = self.nz - row0 if rows is None else rows
+        if tuple(out.shape) != (n_repeat * B, nX, rows) or not out.is_contiguous():
+            raise ValueError(f"out must be contiguous ({n_repeat * B}, {nX}, {rows}), got {tuple(out.shape)}")
+        if tuple(bg.shape) != (B, self.n):
+            raise ValueError(f"bg must be ({B}, {self.n})")
+        mode = _lib.OCT_APOD_LINES if self.apod_mode == "lines" else _lib.OCT_APOD_FRAME_MEAN
+        _lib.check(_lib.lib().oct_spectral_repeat(self.plan.handle, t.data_ptr(), self._code(t), B, lines, mode,
+                                                  self.raw_apod_size, n_repeat, x0, nX, kind, row0, rows,
+                                                  out.data_ptr(), rows, bg.data_ptr(), self.xp.stream()),
+                   "oct_spectral_repeat")
+        return out
+
     def operator_plan(self, z0: int, nzc: int, want_fp64: bool = False) -> OperatorPlan:
         key = (int(z0), int(nzc))
         op = self._ops.get(key)

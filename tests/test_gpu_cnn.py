@@ -2,7 +2,8 @@
 against the CPU oracle (oracle/cnn_ref.py, pinned in test_cnn_oracle.py).
 
 Tolerances: forward outputs are bf16: at least 99.5 % bit-identical to the oracle's rounding of
-its float64 result and none more than 1 bf16 ulp away (float32 accumulation order); gradients are
+its float64 result and none further than 1 bf16 ulp plus 2^-18 of the sum of |terms| (float32
+accumulation where terms cancel); gradients are
 float32 from identical bf16 inputs: max error <= 2e-5 of the largest magnitude per tensor; the
 whole network: head within 2 % of the oracle's (bf16 emulated), and the chained backward from the
 device's stored activations within 1e-4 relative L2 per tensor; the depth median is exact.
@@ -33,29 +34,36 @@ def _np(t):
     return t.float().cpu().numpy()
 
 
-def _bf_close(got, ref):
-    """got: bf16 device result as float32; ref: float64 oracle value -> rounded to bf16."""
+def _bf_close(got, ref, sabs=None):
+    """got: bf16 device result as float32; ref: float64 oracle value -> rounded to bf16.  Returns
+    the bit-identical fraction and the worst error in units of the allowed error: one bf16 ulp
+    of the result plus, where sabs (the sum of |terms|) is given, 2^-18 sabs for the float32
+    accumulation (relevant where the terms cancel)."""
     r = C.bf16_round(ref.astype(np.float32))
     eq = np.mean(got == r)
-    ulp = np.abs(r).astype(np.float64) * 2.0 ** -7 + 1e-30
-    return eq, float(np.max(np.abs(got - r) / ulp))
+    allowed = np.abs(r).astype(np.float64) * 2.0 ** -7 + 1e-30
+    if sabs is not None:
+        allowed = allowed + 2.0 ** -18 * sabs
+    return eq, float(np.max(np.abs(got - r) / allowed))
 
 
 def _rel(got, ref):
     return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-30))
 
 
-LAYERS = [(2, 16, 256), (16, 32, 128), (32, 32, 64), (32, 64, 32), (64, 64, 16), (64, 64, 8)]
+# (ci, co, side, batch): the network's layers (LDS-tiled forward for outputs >= 64 wide, one wave
+# per position below), plus odd sizes for the tile edges and a partial input-channel chunk
+LAYERS = [(2, 16, 256, 3), (2, 16, 256, 8), (16, 32, 128, 3), (16, 32, 128, 32), (32, 32, 64, 3), (32, 64, 32, 3),
+          (64, 64, 16, 3), (64, 64, 8, 3), (16, 16, 200, 2), (18, 16, 140, 2), (18, 16, 70, 2)]
 
 
-@pytest.mark.parametrize("ci,co,h", LAYERS)
-def test_conv_layer_kernels(xp, ci, co, h):
+@pytest.mark.parametrize("ci,co,h,n", LAYERS)
+def test_conv_layer_kernels(xp, ci, co, h, n):
     from oct_recon_af_v1_amd import _lib
-    print(f"case: conv {ci}->{co} at {h}x{h}", flush=True)
+    print(f"case: conv {ci}->{co} at {h}x{h}, batch {n}", flush=True)
     torch = xp.torch
     L = _lib.lib()
-    rng = np.random.default_rng(ci * 1000 + co + h)
-    n = 3
+    rng = np.random.default_rng(ci * 1000 + co + h + n)
     x = C.bf16_round(rng.standard_normal((n, ci, h, h)).astype(np.float32))
     w = C.bf16_round((rng.standard_normal((co, ci, 3, 3)) / np.sqrt(9 * ci)).astype(np.float32))
     b = (rng.standard_normal(co) * 0.1).astype(np.float32)
@@ -64,8 +72,8 @@ def test_conv_layer_kernels(xp, ci, co, h):
     _lib.check(L.oct_cnn_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), n, ci, h, h, co,
                                   xp.stream()), "conv fwd")
     pre = C.conv_fwd(x, w, b)
-    eq, worst = _bf_close(_np(y), np.maximum(pre, 0))
-    print(f"forward: {eq:.4%} bit-identical, worst {worst:.2f} ulp", flush=True)
+    eq, worst = _bf_close(_np(y), np.maximum(pre, 0), C.conv_fwd(np.abs(x), np.abs(w), np.abs(b)))
+    print(f"forward: {eq:.4%} bit-identical, worst {worst:.2f} of the allowed error", flush=True)
     assert eq >= 0.995 and worst <= 1.0
     dpre = rng.standard_normal(pre.shape).astype(np.float32)
     dd = _f32(xp, dpre)
@@ -105,7 +113,7 @@ def test_fc_loss_adam_kernels(xp):
     _lib.check(L.oct_cnn_fc_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), None, yl.data_ptr(), n, k, o,
                                 xp.stream()), "fc linear")
     ref = C.fc_fwd(x, w, b)
-    eq, worst = _bf_close(_np(yb), np.maximum(ref, 0))
+    eq, worst = _bf_close(_np(yb), np.maximum(ref, 0), C.fc_fwd(np.abs(x), np.abs(w), np.abs(b)))
     assert eq >= 0.995 and worst <= 1.0
     assert _rel(yl.cpu().numpy(), ref) <= 2e-6
     dpre = rng.standard_normal((n, o)).astype(np.float32)
@@ -192,28 +200,23 @@ def test_network_gradients_vs_oracle(xp):
 
 
 def test_training_reduces_loss(xp):
-    """A few hundred Adam steps on a fixed synthetic regression (a planted bright row whose depth
-    encodes one target, a brightness offset of the second channel the other) must bring both
-    targets' mean squared error below 0.1 (their variance is 1/3)."""
+    """The training loop (forward, loss, backward, Adam on the device) memorises a fixed batch of
+    32 random inputs with random targets: both targets' mean squared error falls below 0.02
+    (their variance is 1/3) and the loss by more than 3."""
     from oct_recon_af_v1_amd import learned as LN
-    print("case: training smoke", flush=True)
-    torch = xp.torch
+    print("case: training smoke (memorise 32 samples)", flush=True)
     rng = np.random.default_rng(31)
-    M = 256
+    M = 32
     t = rng.uniform(-1, 1, (M, 2)).astype(np.float32)
-    X = rng.standard_normal((M, 2, LN.IMG, LN.IMG)).astype(np.float32) * 0.2
-    rows = ((t[:, 1] + 1) * 100 + 20).astype(int)
-    for i in range(M):
-        X[i, :, rows[i]:rows[i] + 4, :] += 2.0
-        X[i, 1] += 0.5 + 0.5 * t[i, 0]
+    X = rng.standard_normal((M, 2, LN.IMG, LN.IMG)).astype(np.float32)
     Xd = _bf(xp, X)
     Yd = _f32(xp, t)
     net = LN.EstimatorNet(xp, seed=1)
-    hist = LN.train(net, Xd, Yd, epochs=25, batch=32, lr=2e-3, warmup=20)
+    hist = LN.train(net, Xd, Yd, epochs=200, batch=32, lr=1e-3, warmup=10, weight_decay=0.0)
     first, last = hist[0]["train"][0], hist[-1]["train"][0]
-    mse = hist[-1]["train"][1:]
-    print(f"loss {first:.3f} -> {last:.3f}, mse {mse}", flush=True)
-    assert np.isfinite(last) and last < first - 1.0 and max(mse) < 0.1
+    mse = LN.evaluate(net, Xd, Yd, batch=32)
+    print(f"loss {first:.3f} -> {last:.3f}, training-set mse {mse}", flush=True)
+    assert np.isfinite(last) and last < first - 3.0 and max(mse) < 0.02
 
 
 def test_depth_median_and_crop(xp):
@@ -227,15 +230,14 @@ def test_depth_median_and_crop(xp):
         mag[0, :, 7] = 2.5                                       # ties
         med = depth_median(_f32(xp, mag), xp).cpu().numpy()
         assert np.array_equal(med, np.median(mag.astype(np.float64), axis=1)), nX
-    mag = (np.abs(rng.standard_normal((2, 300, 1024))) * 40).astype(np.float32)
-    md = _f32(xp, mag)
+    mag = (np.abs(rng.standard_normal((2, 2, 300, 1024))) * 40).astype(np.float32)   # (channel, B-scan, ...)
+    md = _f32(xp, mag.reshape(4, 300, 1024))
     out = torch.zeros((2, 2, 256, 256), dtype=torch.bfloat16, device=xp.device)
-    for c in (0, 1):
-        _lib.check(_lib.lib().oct_logmag_crop(md.data_ptr(), 2, 300, 1024, 1024, 22, 0, 4, 256, 256, 10.0 * (c + 1),
-                                              20.0, out.data_ptr(), 2, c, xp.stream()), "crop")
+    _lib.check(_lib.lib().oct_logmag_crop(md.data_ptr(), 2, 300, 1024, 1024, 22, 0, 4, 256, 256, 10.0, 20.0,
+                                          out.data_ptr(), 2, xp.stream()), "crop")
     got = out.float().cpu().numpy()
     for c in (0, 1):
-        ref = C.logmag_crop(mag, 22, 0, 4, 256, 256, 10.0 * (c + 1), 20.0)
+        ref = C.logmag_crop(mag[c], 22, 0, 4, 256, 256, 10.0, 20.0)
         eq, worst = _bf_close(got[:, c], ref)
         assert eq >= 0.999 and worst <= 1.0
 
@@ -254,6 +256,22 @@ def test_per_file_windows_equal_single(xp):
     per = sc.scores_per_file(betas)
     single = np.array([sc.scores_many([b])[0][i] for i, b in enumerate(betas)])
     assert np.array_equal(per, single)
+    # the repeat launch (2 candidate sets in one launch, per-file and per-repeat windows) equals
+    # separate launches bit for bit
+    bg = sc.sp.background(sc.raw)
+    two = xp.torch.empty((8, sc.nX, sc.nz), dtype=xp.torch.float32, device=xp.device)
+    sc.magnitude_repeat(sc.windows(np.concatenate([betas, betas[::-1]])), 2, bg, two)
+    a0 = sc.magnitude_at(betas).clone()
+    a1 = sc.magnitude_at(betas[::-1]).clone()
+    assert xp.torch.equal(two[:4], a0) and xp.torch.equal(two[4:], a1)
+    sc.magnitude_repeat(sc.windows(betas[:2]), 2, bg, two)
+    s0 = sc.magnitude_at(betas[0]).clone()
+    s1 = sc.magnitude_at(betas[1]).clone()
+    assert xp.torch.equal(two[:4], s0) and xp.torch.equal(two[4:], s1)
+    # a window of scan lines equals the same lines of the full reconstruction
+    part = xp.torch.empty((8, 50, sc.nz), dtype=xp.torch.float32, device=xp.device)
+    sc.magnitude_repeat(sc.windows(betas[:2]), 2, bg, part, x0=10, nx=50)
+    assert xp.torch.equal(part, two[:, 10:60])
 
 
 def test_dispersion_windows_kernel(xp):

@@ -29,8 +29,14 @@ def test_topfrac_metric_exact(xp):
     A[2, 5, 25] = 1.0
     t = xp.asarray(A)
     out = xp.torch.empty(3, dtype=xp.torch.float64, device=xp.device)
-    _lib.check(_lib.lib().oct_topfrac_metric(t.data_ptr(), 3, 97, 300, 20, 0.01, out.data_ptr(), xp.stream()),
-               "metric")
+    nb = _lib.lib().oct_topfrac_metric_workspace_bytes(3, 97, 300, 20)
+    ws = xp.torch.empty((nb,), dtype=xp.torch.uint8, device=xp.device)
+    _lib.check(_lib.lib().oct_topfrac_metric(t.data_ptr(), 3, 97, 300, 20, 0.01, out.data_ptr(), ws.data_ptr(), nb,
+                                             xp.stream()), "metric")
+    again = xp.torch.empty_like(out)
+    _lib.check(_lib.lib().oct_topfrac_metric(t.data_ptr(), 3, 97, 300, 20, 0.01, again.data_ptr(), ws.data_ptr(), nb,
+                                             xp.stream()), "metric again")
+    assert xp.torch.equal(out, again)                           # deterministic, workspace reused
     g = out.cpu().numpy()
     ref = R.top_percentile_metric(A)
     with np.errstate(divide="ignore"):
