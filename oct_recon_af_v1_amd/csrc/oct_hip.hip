@@ -266,6 +266,7 @@ static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int 
     a.tw = p->tw;
     a.out = out;
     a.M = (long long)n_repeat * n_files * nX;
+    if (a.M >= (1LL << 31)) return fail(OCT_EINVAL, "%lld A-scans in one launch (at most 2^31 - 1)", a.M);
     a.lines = lines;
     a.line0 = line0;
     a.nX = nX;
@@ -369,7 +370,10 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
     a.partial = partial;
     a.blocks_per_plane = oct_stitch_row_blocks(n_out_z, n_out_x);
     if (n_frames > 65535) return fail(OCT_EINVAL, "%d planes in one call (at most 65535)", n_frames);
-    dim3 grid((n_out_x + oct::kRowC - 1) / oct::kRowC, (n_out_z + oct::kRowZ - 1) / oct::kRowZ, n_frames);
+    a.nplanes = n_frames;
+    const long long total = (long long)a.blocks_per_plane * n_frames;
+    if (total > (1LL << 30)) return fail(OCT_EINVAL, "too many planes in one call");
+    dim3 grid((unsigned)((total + 7) / 8 * 8));            // whole multiple of the 8 XCDs
     if (shift)
         hipLaunchKernelGGL(oct::stitch_row_kernel<true>, grid, dim3(64 * oct::kRowWaves), 0,
                            reinterpret_cast<hipStream_t>(stream), a);

@@ -115,14 +115,17 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         const int nvalid = (int)min((long long)kWaves, a.M - p0);
         const long long p = p0 + wave;
         float* D = reinterpret_cast<float*>(&buf[wave][0]);
+        // A-scan p = (rep raw_files + b) nX + x; the host keeps M below 2^31, so 32-bit division
+        const unsigned fv = (unsigned)p / (unsigned)a.nX;          // virtual file
+        const unsigned rep = fv / (unsigned)a.raw_files;
+        const unsigned b = fv - rep * (unsigned)a.raw_files;
+        const unsigned x = (unsigned)p - fv * (unsigned)a.nX;
 
         // ---- A: load, background, mean, d - m -> LDS
         if (wave < nvalid) {
-            const long long bv = p / a.nX;
-            const long long b = bv % a.raw_files;
-            const long long x = p - bv * a.nX;
-            const RawT* r = reinterpret_cast<const RawT*>(a.raw) + ((b * a.lines + a.line0 + x) * (long long)kN);
-            const float* bgr = a.bg + b * kN;
+            const RawT* r = reinterpret_cast<const RawT*>(a.raw) +
+                            (((long long)b * a.lines + a.line0 + x) * (long long)kN);
+            const float* bgr = a.bg + (long long)b * kN;
             float d[NLD][VEC];
             float s = 0.f;
 #pragma unroll
@@ -208,9 +211,8 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
         if (wave < nvalid) {
             cf* B = &buf[wave][0];
             const float* E = reinterpret_cast<const float*>(B) + kN;
-            const long long bv = p / a.nX;
-            const float2* wg = reinterpret_cast<const float2*>(a.win + (bv / a.raw_files) * a.win_rstride +
-                                                               (bv % a.raw_files) * a.win_fstride);
+            const float2* wg = reinterpret_cast<const float2*>(a.win + (long long)rep * a.win_rstride +
+                                                               (long long)b * a.win_fstride);
             cf v[kQ];
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {

@@ -118,6 +118,7 @@ struct RowArgs {
     int plane0;
     int* partial;           // (planes, blocks_per_plane, 2)
     int blocks_per_plane;
+    int nplanes;            // planes of this call
 };
 
 // Block tile: kRowZ output rows x kRowC output columns of one plane.  Lane l of a wave owns row
@@ -133,9 +134,19 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
     __shared__ float tile[kRowZ][kRowC + 1];
     __shared__ int smin[kRowWaves], smax[kRowWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int zo = blockIdx.y * kRowZ + (lane % kRowZ);
-    const int cb0 = blockIdx.x * kRowC;
-    const int f = blockIdx.z;
+    // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so block id i runs
+    // on XCD i % 8.  Each XCD takes a contiguous range of the work list, ordered depth block
+    // fastest, so the depth blocks of one column range share that XCD's L2.
+    const int gx = (a.nOutX + kRowC - 1) / kRowC, gz = (a.nOutZ + kRowZ - 1) / kRowZ;
+    const long long total = (long long)gx * gz * a.nplanes;
+    const long long per_xcd = (long long)gridDim.x / 8;
+    const long long w = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (w >= total) return;                                // uniform per workgroup
+    const int zb = (int)(w % gz);
+    const int xb = (int)((w / gz) % gx);
+    const int f = (int)(w / ((long long)gz * gx));
+    const int zo = zb * kRowZ + (lane % kRowZ);
+    const int cb0 = xb * kRowC;
     const int fb = a.frame_idx[f];
     const bool zok = zo < a.nOutZ;
     float num[kRowPasses], den[kRowPasses];
@@ -214,7 +225,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
     // coalesced stores: consecutive threads write consecutive columns of one output row
     for (int e = threadIdx.x; e < kRowZ * kRowC; e += 64 * kRowWaves) {
         const int zr = e / kRowC, cc = e - zr * kRowC;
-        const int z = blockIdx.y * kRowZ + zr, c = cb0 + cc;
+        const int z = zb * kRowZ + zr, c = cb0 + cc;
         if (z < a.nOutZ && c < a.nOutX) a.db[((long long)(a.plane0 + f) * a.nOutZ + z) * a.nOutX + c] = tile[zr][cc];
     }
     if (threadIdx.x == 0) {
@@ -223,7 +234,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
             mn = min(mn, smin[w]);
             mx = max(mx, smax[w]);
         }
-        const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + blockIdx.y * gridDim.x + blockIdx.x) * 2;
+        const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + (long long)zb * gx + xb) * 2;
         a.partial[o] = mn;
         a.partial[o + 1] = mx;
     }
