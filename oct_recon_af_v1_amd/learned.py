@@ -537,3 +537,43 @@ class LearnedEstimator:
             p["focus"][bad] = np.where(np.isfinite(lab["focus"]), lab["focus"], p["focus"][bad])
         p["fallback_rate"] = float(bad.size / len(p["confident"]))
         return p
+
+
+def _time_main(argv=None):
+    """Time the estimator on synthetic B-scans (one tile at a time and all tiles in one batch):
+    python -m oct_recon_af_v1_amd.learned --tiles 8 --bscans 8 --reps 3"""
+    import argparse
+    from . import synthetic as SY
+    from .backend import get_xp
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--tiles", type=int, default=8)
+    ap.add_argument("--bscans", type=int, default=8)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=5)
+    a = ap.parse_args(argv)
+    xp = get_xp("hip")[0]
+    spec = SY.SpectrumSpec()
+    rng = np.random.default_rng(a.seed)
+    n = a.tiles * a.bscans
+    raw = SY.make_focus_bscans(n, 512, rng.uniform(-2e-4, 2e-4, n), rng.uniform(0, 1024, n), spec, seed=a.seed,
+                               device=xp.device)
+    est = LearnedEstimator(xp, spec.lambda_nm(), spec.apod)
+    est.predict(raw[:a.bscans], graph=True)
+    est.predict(raw, graph=True)
+    xp.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        for i in range(0, n, a.bscans):
+            est.predict(raw[i:i + a.bscans], graph=True)
+    xp.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(a.reps):
+        est.predict(raw, graph=True)
+    xp.synchronize()
+    t2 = time.perf_counter()
+    print(f"per tile of {a.bscans} B-scans: {(t1 - t0) / (a.reps * a.tiles) * 1e3:.3f} ms one tile at a time, "
+          f"{(t2 - t1) / (a.reps * a.tiles) * 1e3:.3f} ms with {a.tiles} tiles in one batch", flush=True)
+
+
+if __name__ == "__main__":
+    _time_main()
