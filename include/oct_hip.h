@@ -145,6 +145,21 @@ int oct_finalize(const float* num, const float* den, float* db, int n_planes, lo
 int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_elems, const float* clim,
                  const float* plane_clim, int legacy_double, void* stream);
 
+/* ---- A-scan binning and repeat averaging (octrecon/core/spectral.py:20-46 bin_ascans /
+ * binned_columns; :168-177 average_repeats)
+ * oct_binned_lines: scan lines kept by AScanBinning of n_scan_lines lines.
+ * oct_bin_lines: raw (n_files, lines, 2048) -> out (n_files, keep_lines + kept, 2048) float32: the
+ * first keep_lines lines copied, then the binned scan lines [skip_lines, lines) (filter2 with
+ * ones(1,B)/B, zero padded, then every B-th line from max(1, B/2) - 1), float32 products summed
+ * in order.
+ * oct_average_repeats: mag (n_frames bscan_avg, size_x ascan_avg, rows) -> out (n_frames,
+ * size_x, rows): mean over the B-scan repeats, then over the A-scan repeats (float32). */
+int oct_binned_lines(int n_scan_lines, int binning);
+int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int keep_lines, int skip_lines,
+                  int binning, float* out, void* stream);
+int oct_average_repeats(const float* mag, float* out, int n_frames, int bscan_avg, int size_x, int ascan_avg, int rows,
+                        long long in_stride, long long out_stride, void* stream);
+
 /* ---- dispersion sharpness search (octrecon/estimation/dispersion.py:46-107, 154-239)
  * One search evaluation is the spectral path with the candidate's window
  * hann/rms * exp(-i beta (k - k0)^2).  oct_plan_use_window makes later oct_spectral launches

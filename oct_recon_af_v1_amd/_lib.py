@@ -75,6 +75,9 @@ SIGNATURES = {
     "oct_cnn_adam": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _f, _f, _f, _f, _f, _i, _vp]),
     "oct_dispersion_windows": (_i, [_vp, _vp, _i, ctypes.c_double, ctypes.c_double, ctypes.c_double, _i, _vp, _vp,
                                     _i, _vp, _vp]),
+    "oct_binned_lines": (_i, [_i, _i]),
+    "oct_bin_lines": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "oct_average_repeats": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _ll, _ll, _vp]),
     "oct_last_error": (ctypes.c_char_p, []),
 }
 

@@ -215,6 +215,59 @@ static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int 
                          int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
                          long long out_stride, const float* bg, void* stream);
 
+int oct_binned_lines(int n_scan_lines, int binning) {
+    if (n_scan_lines <= 0 || binning < 1) return 0;
+    const int s0 = std::max(1, binning / 2) - 1;
+    return s0 >= n_scan_lines ? 0 : (n_scan_lines - s0 + binning - 1) / binning;
+}
+
+int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int keep_lines, int skip_lines,
+                  int binning, float* out, void* stream) {
+    if (!raw || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_files <= 0) return OCT_OK;
+    if (binning < 1 || keep_lines < 0 || skip_lines < keep_lines || skip_lines >= lines)
+        return fail(OCT_EINVAL, "binning %d, keep %d, skip %d of %d lines", binning, keep_lines, skip_lines, lines);
+    if (raw_dtype < OCT_RAW_I16 || raw_dtype > OCT_RAW_F32) return fail(OCT_EINVAL, "raw dtype %d", raw_dtype);
+    const int n_scan = lines - skip_lines;
+    const int n_kept = oct_binned_lines(n_scan, binning);
+    const long long tot = (long long)n_files * (keep_lines + n_kept) * (long long)oct::kN;
+    if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    const unsigned grid = (unsigned)((tot + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (raw_dtype) {
+        case OCT_RAW_I16:
+            hipLaunchKernelGGL(oct::bin_lines_kernel<int16_t>, dim3(grid), dim3(256), 0, s, static_cast<const int16_t*>(raw),
+                               out, n_files, lines, keep_lines, skip_lines, n_scan, binning, n_kept, oct::kN);
+            break;
+        case OCT_RAW_U16:
+            hipLaunchKernelGGL(oct::bin_lines_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, static_cast<const uint16_t*>(raw),
+                               out, n_files, lines, keep_lines, skip_lines, n_scan, binning, n_kept, oct::kN);
+            break;
+        default:
+            hipLaunchKernelGGL(oct::bin_lines_kernel<float>, dim3(grid), dim3(256), 0, s, static_cast<const float*>(raw),
+                               out, n_files, lines, keep_lines, skip_lines, n_scan, binning, n_kept, oct::kN);
+            break;
+    }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_average_repeats(const float* mag, float* out, int n_frames, int bscan_avg, int size_x, int ascan_avg, int rows,
+                        long long in_stride, long long out_stride, void* stream) {
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_frames <= 0 || size_x <= 0 || rows <= 0) return OCT_OK;
+    if (bscan_avg < 1 || ascan_avg < 1 || in_stride < rows || out_stride < rows)
+        return fail(OCT_EINVAL, "bad repeat shape (%d, %d, strides %lld %lld, rows %d)", bscan_avg, ascan_avg, in_stride,
+                    out_stride, rows);
+    const long long tot = (long long)n_frames * size_x * rows;
+    if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(oct::average_repeats_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), mag, out, n_frames, bscan_avg, size_x, ascan_avg, rows,
+                       in_stride, out_stride);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 int oct_spectral(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                  int apod_size, int out_kind, int row0, int rows, void* out, long long out_stride,
                  const float* bg, void* stream) {

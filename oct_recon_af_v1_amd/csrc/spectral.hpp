@@ -269,6 +269,69 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
     }
 }
 
+// A-scan binning of the scan lines (Thorlabs AScanBinning; octrecon/core/spectral.py:20-46, the
+// legacy filter2(ones(1,B)/B) + decimation): kept line c averages scan lines
+// s0 + c B + t - left, t < B (zero outside), s0 = max(1, B / 2) - 1, left = (B + 1) / 2 - 1, as
+// the float32 sum of the float32 products line * (1/B) in t order (no contraction: numpy rounds
+// each product).  The first `keep` lines (background lines) are copied as float32.
+// out (B, keep + n_kept, N) float32; one thread per output sample.
+#pragma clang fp contract(off)
+template <typename RawT>
+__global__ __launch_bounds__(256) void bin_lines_kernel(const RawT* __restrict__ raw, float* __restrict__ out,
+                                                        int files, int lines, int keep, int skip, int n_scan,
+                                                        int binning, int n_kept, int N) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int out_lines = keep + n_kept;
+    if (tid >= (long long)files * out_lines * N) return;
+    const int j = (int)(tid % N);
+    const long long t2 = tid / N;
+    const int l = (int)(t2 % out_lines);
+    const int b = (int)(t2 / out_lines);
+    const RawT* r = raw + (long long)b * lines * N + j;
+    float v;
+    if (l < keep) {
+        v = (float)r[(long long)l * N];
+    } else {
+        const float h = 1.0f / (float)binning;
+        const int s0 = max(1, binning / 2) - 1, left = (binning + 1) / 2 - 1;
+        const int first = s0 + (l - keep) * binning - left;
+        v = 0.f;
+        for (int t = 0; t < binning; ++t) {
+            const int sl = first + t;
+            const float x = (sl >= 0 && sl < n_scan) ? (float)r[(long long)(skip + sl) * N] : 0.f;
+            const float term = x * h;
+            v = (t == 0) ? term : v + term;
+        }
+    }
+    out[tid] = v;
+}
+
+// Repeat averaging (octrecon/core/spectral.py:168-177): mag (n_frames nb, size_x na, rows) ->
+// out (n_frames, size_x, rows) = mean over the A-scan repeats of the means over the B-scan
+// repeats; float32 sums in index order, each divided by its count.
+__global__ __launch_bounds__(256) void average_repeats_kernel(const float* __restrict__ mag, float* __restrict__ out,
+                                                              int n_frames, int nb, int size_x, int na, int rows,
+                                                              long long in_stride, long long out_stride) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long long)n_frames * size_x * rows) return;
+    const int z = (int)(tid % rows);
+    const long long t2 = tid / rows;
+    const int x = (int)(t2 % size_x);
+    const int f = (int)(t2 / size_x);
+    float acc_a = 0.f;
+    for (int ia = 0; ia < na; ++ia) {
+        float acc_b = 0.f;
+        for (int ib = 0; ib < nb; ++ib) {
+            const float v = mag[((long long)(f * nb + ib) * size_x * na + (long long)x * na + ia) * in_stride + z];
+            acc_b = (ib == 0) ? v : acc_b + v;
+        }
+        const float mb = (nb > 1) ? acc_b / (float)nb : acc_b;
+        acc_a = (ia == 0) ? mb : acc_a + mb;
+    }
+    out[((long long)f * size_x + x) * out_stride + z] = (na > 1) ? acc_a / (float)na : acc_a;
+}
+#pragma clang fp contract(fast)
+
 // Background row per raw file.
 //   mode 0 ("lines"): mean of the first `apod` A-lines of the file (float64 sum, stored float32)
 //   mode 1 ("frame_mean"): mean over all A-lines of each group of `group` files, float32

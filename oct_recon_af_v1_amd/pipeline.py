@@ -53,7 +53,11 @@ class Cancelled(Exception):
 
 class Reconstructor:
     def __init__(self, cfg: ReconConfig, scan: ScanGrid, lambda_nm, tiles, apod_size: int,
-                 apod_mode: str = "lines", opc_poly=None, xp=None, log=None):
+                 apod_mode: str = "lines", opc_poly=None, xp=None, log=None, ascan_binning: int = 1,
+                 bscan_avg: int = 1, ascan_avg: int = 1):
+        """ascan_binning, bscan_avg, ascan_avg: the tile header's spectra_avg (A-line binning),
+        B-scan repeats (consecutive raw files per frame) and A-scan repeats (consecutive A-lines
+        per scan position), octrecon/pipeline.py:138-141,177-183,216-219,276-279."""
         if cfg.precision != "float32":
             raise ValueError("the hip device computes in float32")
         self.cfg = cfg
@@ -76,7 +80,10 @@ class Reconstructor:
         poly = opc_poly if cfg.apply_path_length_correction else None
         self.oc = (OpticalPathCorrection.build(poly, self.tg.tile_x_mm, self.tg.tile_y_mm, self.tg.tile_z_mm)
                    if poly is not None else None)
-        self.sp = HipSpectralProcessor(self.sg, apod_size, self.xp, apod_mode=apod_mode)
+        self.nb, self.na = max(1, int(bscan_avg)), max(1, int(ascan_avg))
+        self.sp = HipSpectralProcessor(self.sg, apod_size, self.xp, ascan_binning=ascan_binning,
+                                       apod_mode=apod_mode, apod_group=self.nb)
+        self._rep = None
         self.n_out = (len(self.tg.out_y_mm), len(self.tg.out_z_mm), len(self.tg.out_x_mm))
         self.stitchers = {}
         for xi, xc in enumerate(scan.x_centers_mm):
@@ -91,15 +98,52 @@ class Reconstructor:
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
 
     # ------------------------------------------------------------------ data access
+    def frame_files(self, frames):
+        """Raw file indices of tile-local frames: the bscan_avg repeats of frame f are files
+        f * bscan_avg + a (octrecon/pipeline.py:216-219)."""
+        f = np.asarray(frames, np.int64)
+        return (f[:, None] * self.nb + np.arange(self.nb)).ravel() if self.nb > 1 else f
+
     def tile_frames(self, xi, yi, zi, frames):
+        """Raw files (len(frames) * bscan_avg, lines, N) of the frames; a callable tile source is
+        given the raw file indices."""
         src = self.tiles
+        f = self.frame_files(frames)
         if callable(src):
-            return src(xi, yi, zi, frames)
+            return src(xi, yi, zi, f)
         t = src[(xi, yi, zi)]
-        f = np.asarray(frames)
         if f.size and np.all(np.diff(f) == 1):
             return t[int(f[0]):int(f[-1]) + 1]
-        return t[self.torch.as_tensor(f, device=t.device)]
+        idx = self.torch.as_tensor(f, device=t.device)
+        if t.dtype == self.torch.uint16:          # no uint16 gather on the device: move the bits as int16
+            return t.view(self.torch.int16)[idx].view(self.torch.uint16)
+        return t[idx]
+
+    def _positions(self, lines):
+        """Scan positions per frame of raw files with `lines` A-lines (after binning, repeats)."""
+        n = self.sp.n_scan_lines(lines)
+        if n % self.na:
+            raise ValueError(f"{n} A-lines per file after binning is not a multiple of ascan_avg {self.na}")
+        return n // self.na
+
+    def _magnitudes(self, raw, n_frames, lo, hi, dst):
+        """|scan| rows [lo, hi) of the frames' raw files into dst (n_frames, positions, hi - lo),
+        averaged over the B-scan and A-scan repeats (oct_average_repeats)."""
+        mfma = self.cfg.form == "mfma"
+        if self.nb == 1 and self.na == 1:
+            if mfma:
+                return self.sp.run_operator(raw, lo, hi - lo, out=dst)
+            return self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=dst)
+        shape = (raw.shape[0], self.sp.n_scan_lines(raw.shape[1]), hi - lo)
+        n = shape[0] * shape[1] * shape[2]
+        if self._rep is None or self._rep.numel() < n:
+            self._rep = self.torch.empty(n, dtype=self.torch.float32, device=self.xp.device)
+        tmp = self._rep.narrow(0, 0, n).view(shape)
+        if mfma:
+            self.sp.run_operator(raw, lo, hi - lo, out=tmp)
+        else:
+            self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=tmp)
+        return self.sp.average_repeats(tmp, n_frames, self.nb, dst.shape[1], self.na, out=dst)
 
     def _mag_buffer(self, n):
         if self._mag is None or self._mag.numel() < n:
@@ -136,12 +180,9 @@ class Reconstructor:
                     raise Cancelled()
                 fr = frames[s:s + B]
                 raw = self.tile_frames(xi, yi, zi, fr)
-                nX = self.sp.n_scan_lines(raw.shape[1])
+                nX = self._positions(raw.shape[1])
                 mbuf = self._mag_buffer(len(fr) * nX * (hi - lo)).narrow(0, 0, len(fr) * nX * (hi - lo))
-                if cfg.form == "mfma":
-                    mag = self.sp.run_operator(raw, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
-                else:
-                    mag = self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=mbuf.view(len(fr), nX, hi - lo))
+                mag = self._magnitudes(raw, len(fr), lo, hi, mbuf.view(len(fr), nX, hi - lo))
                 st.contribute_into(mag, fr_dev[s:s + len(fr)], num, den, b0=b_off + s, mag_row0=lo)
 
     # ------------------------------------------------------------------ fused row path
@@ -172,16 +213,12 @@ class Reconstructor:
                     raise Cancelled()
                 fr = frames[s:s + B]
                 raw = self.tile_frames(xi, yi, zi, fr)
-                nX = self.sp.n_scan_lines(raw.shape[1])
+                nX = self._positions(raw.shape[1])
                 shape = (nF, nX, hi - lo)
                 if buf is None or tuple(buf.shape) != shape:
                     buf = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
                     self._tile_mag[key] = buf
-                dst = buf[s:s + len(fr)]
-                if cfg.form == "mfma":
-                    self.sp.run_operator(raw, lo, hi - lo, out=dst)
-                else:
-                    self.sp.run(raw, _lib.OCT_OUT_MAG, lo, hi - lo, out=dst)
+                self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
             out.append((st, buf, lo))
         return out
 

@@ -103,20 +103,23 @@ class HipSpectralProcessor:
 
     sg: SpectralGeometry; apod_size: background A-lines at the start of each raw file
     (apod_mode 'lines') or ignored (apod_mode 'frame_mean': background = mean of all A-lines
-    of each group of apod_group files).  A-scan binning > 1 is not supported by this device."""
+    of each group of apod_group files).  ascan_binning > 1: the scan lines are binned on the
+    device first (oct_bin_lines; the background is taken from the unbinned lines, as
+    octrecon/core/spectral.py:49-71 does)."""
 
     def __init__(self, sg: SpectralGeometry, apod_size: int, xp, ascan_binning: int = 1,
                  apod_mode: str = "lines", apod_group: int = 1):
         if apod_mode not in ("lines", "frame_mean"):
             raise ValueError(apod_mode)
-        if int(ascan_binning) > 1:
-            raise NotImplementedError("A-scan binning is not implemented on the hip device")
+        if int(ascan_binning) < 1:
+            raise ValueError(f"ascan_binning {ascan_binning}")
         self.xp = xp
         self.torch = xp.torch
         self.dtype = np.dtype(np.float32)
         self.apod_mode = apod_mode
         self.apod_group = max(1, int(apod_group))
-        self.ascan_binning = 1
+        self.ascan_binning = int(ascan_binning)
+        self._binned = None
         self.raw_apod_size = apod_size
         self.apod_size = 1 if apod_mode == "frame_mean" else apod_size
         self.n = sg.n
@@ -157,7 +160,37 @@ class HipSpectralProcessor:
         return self._ws
 
     def n_scan_lines(self, lines: int) -> int:
-        return lines - self.raw_apod_size if self.apod_mode == "lines" else lines
+        """Scan A-lines per raw file of `lines` lines, after binning."""
+        n = lines - self.raw_apod_size if self.apod_mode == "lines" else lines
+        return _lib.lib().oct_binned_lines(n, self.ascan_binning) if self.ascan_binning > 1 else n
+
+    def _bin(self, t):
+        """Binned float32 copy of a raw batch: the background lines, then the binned scan lines."""
+        B, lines, N = t.shape
+        keep = self.raw_apod_size if self.apod_mode == "lines" else 0
+        shape = (B, keep + self.n_scan_lines(lines), N)
+        if self._binned is None or tuple(self._binned.shape) != shape:
+            self._binned = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
+        _lib.check(_lib.lib().oct_bin_lines(t.data_ptr(), self._code(t), B, lines, keep, keep, self.ascan_binning,
+                                            self._binned.data_ptr(), self.xp.stream()), "oct_bin_lines")
+        return self._binned
+
+    def average_repeats(self, mag, n_frames: int, bscan_avg: int, size_x: int, ascan_avg: int, out=None):
+        """octrecon/core/spectral.py:168-177 on the device: (n_frames nb, size_x na, rows) ->
+        (n_frames, size_x, rows) (oct_average_repeats)."""
+        nb, na = max(1, int(bscan_avg)), max(1, int(ascan_avg))
+        if nb == 1 and na == 1 and out is None:
+            return mag
+        rows = mag.shape[-1]
+        if tuple(mag.shape) != (n_frames * nb, size_x * na, rows) or mag.stride(2) != 1 or \
+                mag.stride(0) != mag.shape[1] * mag.stride(1):
+            raise ValueError(f"magnitudes {tuple(mag.shape)} do not hold {n_frames} x {nb} files of {size_x} x {na} lines")
+        if out is None:
+            out = self.torch.empty((n_frames, size_x, rows), dtype=self.torch.float32, device=self.xp.device)
+        _lib.check(_lib.lib().oct_average_repeats(mag.data_ptr(), out.data_ptr(), n_frames, nb, size_x, na, rows,
+                                                  mag.stride(1), out.stride(1), self.xp.stream()),
+                   "oct_average_repeats")
+        return out
 
     def run(self, raw, kind: int, row0: int = 0, rows: int | None = None, out=None, out_stride=None):
         """Launch the fused kernel on the current stream; returns the output tensor."""
@@ -190,6 +223,10 @@ class HipSpectralProcessor:
         code = self._code(t)
         _lib.check(L.oct_background(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size,
                                     self.apod_group, ws.data_ptr(), self.xp.stream()), "oct_background")
+        if self.ascan_binning > 1:             # background from the unbinned lines, then bin
+            t = self._bin(t)
+            code = _lib.OCT_RAW_F32
+            lines = t.shape[1]
         ev = self.timing_events
         if ev is not None:
             e0 = torch.cuda.Event(enable_timing=True)
@@ -229,7 +266,12 @@ class HipSpectralProcessor:
         if tuple(bg.shape) != (B, self.n):
             raise ValueError(f"bg must be ({B}, {self.n})")
         mode = _lib.OCT_APOD_LINES if self.apod_mode == "lines" else _lib.OCT_APOD_FRAME_MEAN
-        _lib.check(_lib.lib().oct_spectral_repeat(self.plan.handle, t.data_ptr(), self._code(t), B, lines, mode,
+        code = self._code(t)
+        if self.ascan_binning > 1:
+            t = self._bin(t)
+            code = _lib.OCT_RAW_F32
+            lines = t.shape[1]
+        _lib.check(_lib.lib().oct_spectral_repeat(self.plan.handle, t.data_ptr(), code, B, lines, mode,
                                                   self.raw_apod_size, n_repeat, x0, nX, kind, row0, rows,
                                                   out.data_ptr(), rows, bg.data_ptr(), self.xp.stream()),
                    "oct_spectral_repeat")
@@ -271,6 +313,10 @@ class HipSpectralProcessor:
         code = self._code(t)
         _lib.check(L.oct_background(self.plan.handle, t.data_ptr(), code, B, lines, mode, self.raw_apod_size,
                                     self.apod_group, ws.data_ptr(), self.xp.stream()), "oct_background")
+        if self.ascan_binning > 1:             # background from the unbinned lines, then bin
+            t = self._bin(t)
+            code = _lib.OCT_RAW_F32
+            lines = t.shape[1]
         ev = self.timing_events
         if ev is not None:
             e0 = torch.cuda.Event(enable_timing=True)

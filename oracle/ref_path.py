@@ -132,6 +132,59 @@ def frame_mean_prepare(raw, group=1, dtype=np.float32):
     return np.concatenate([apod, x], axis=1)
 
 
+def binned_columns(n_lines, binning):
+    """0-based scan lines kept by AScanBinning (octrecon/core/spectral.py:20-24): every B-th line
+    from max(1, B // 2) - 1."""
+    return np.arange(max(1, binning // 2) - 1, n_lines, binning)
+
+
+def bin_ascans(x, binning, dtype=np.float32):
+    """AScanBinning along the A-line axis (octrecon/core/spectral.py:27-46): filter2 with
+    ones(1, B) / B, 'same', zero padded, then decimation; each product x * (1/B) is rounded in
+    `dtype` and the products are summed in order.  x (..., nLines, N) -> (..., nKept, N)."""
+    x = np.asarray(x, dtype)
+    n = x.shape[-2]
+    cols = binned_columns(n, binning)
+    left = (binning + 1) // 2 - 1
+    right = binning // 2
+    pad = [(0, 0)] * x.ndim
+    pad[-2] = (left, right)
+    xp_ = np.pad(x, pad)
+    h = np.dtype(dtype).type(1.0 / binning)
+    acc = None
+    for t in range(binning):
+        sl = xp_[..., cols[0] + t: cols[0] + t + binning * (len(cols) - 1) + 1: binning, :]
+        term = (sl * h).astype(dtype)
+        acc = term if acc is None else (acc + term).astype(dtype)
+    return acc
+
+
+def prepare_raw(raw, apod_size, binning=1, apod_mode="lines", apod_group=1, dtype=np.float32):
+    """Raw files -> the array the pre-FFT kernel sees (octrecon/core/spectral.py:49-71): the
+    background lines (or, for 'frame_mean', the frame-mean line computed from the unbinned
+    A-lines) followed by the binned scan lines.  Returns (array, background lines)."""
+    if apod_mode == "frame_mean":
+        pre = frame_mean_prepare(raw, apod_group, dtype)
+        if binning > 1:
+            pre = np.concatenate([pre[:, :1], bin_ascans(pre[:, 1:], binning, dtype)], axis=1)
+        return pre, 1
+    if binning > 1:
+        x = np.asarray(raw, dtype)
+        return np.concatenate([x[:, :apod_size], bin_ascans(x[:, apod_size:], binning, dtype)], axis=1), apod_size
+    return raw, apod_size
+
+
+def average_repeats(mag, n_frames, bscan_avg, size_x, ascan_avg):
+    """octrecon/core/spectral.py:168-177: (n_frames nb, size_x na, nZ) -> (n_frames, size_x, nZ),
+    the mean over B-scan repeats, then over A-scan repeats (numpy, the array's dtype)."""
+    nb, na = max(1, int(bscan_avg)), max(1, int(ascan_avg))
+    if nb == 1 and na == 1:
+        return mag
+    m5 = mag.reshape(n_frames, nb, size_x, na, mag.shape[-1])
+    m4 = m5.mean(1) if nb > 1 else m5[:, 0]
+    return m4.mean(2) if na > 1 else m4[:, :, 0]
+
+
 def fused_pre_fft(raw, apod_size, geo: OracleGeometry):
     """Reference CPU pre-FFT (Numba typing) via the C restatement -> complex64 (B, nX, N)."""
     raw = np.ascontiguousarray(raw)
@@ -151,13 +204,12 @@ def fused_pre_fft(raw, apod_size, geo: OracleGeometry):
     return out
 
 
-def magnitude_cpu(raw, apod_size, geo: OracleGeometry, apod_mode="lines", apod_group=1, workers=None):
+def magnitude_cpu(raw, apod_size, geo: OracleGeometry, apod_mode="lines", apod_group=1, workers=None, binning=1):
     """The reference's float32 CPU path (SpectralProcessor.magnitude with Numba,
-    octrecon/core/spectral.py:136-153): pre-FFT, scipy ifft, keep N/2, abs -> float32."""
+    octrecon/core/spectral.py:112-116, 136-153): prepare (binning / frame mean), pre-FFT, scipy
+    ifft, keep N/2, abs -> float32."""
     import scipy.fft
-    if apod_mode == "frame_mean":
-        raw = frame_mean_prepare(raw, apod_group, np.float32)
-        apod_size = 1
+    raw, apod_size = prepare_raw(raw, apod_size, binning, apod_mode, apod_group, np.float32)
     spec = fused_pre_fft(raw, apod_size, geo)
     ft = scipy.fft.ifft(spec, axis=-1, workers=workers or os.cpu_count(), overwrite_x=True)
     return np.abs(ft[..., : spec.shape[-1] // 2]).astype(np.float32)

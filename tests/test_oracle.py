@@ -145,3 +145,64 @@ def test_colon_linspace_endpoints(n):
     assert y[0] == 0.3 and y[-1] == 1.7
     c = R.colon_m(0.0, 0.1, 0.3)
     assert len(c) == 4 and c[-1] == pytest.approx(0.3)
+
+
+def test_binning_matches_matlab_filter2():
+    """AScanBinning: filter2(ones(1,B)/B, 1:10) then (:, max(1,floor(B/2)):B:end), the MATLAB
+    R2024a values held by the reference's tests (tests/test_formats.py:289-299)."""
+    x = np.arange(1, 11, dtype=float)[None, :, None]
+    full = {2: [1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5, 8.5, 9.5, 5.0],
+            3: [1.0, 2, 3, 4, 5, 6, 7, 8, 9, 19 / 3],
+            4: [1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5, 8.5, 6.75, 4.75]}
+    for B, f in full.items():
+        want = np.asarray(f)[max(1, B // 2) - 1::B]
+        np.testing.assert_allclose(R.bin_ascans(x, B, np.float64)[0, :, 0], want, rtol=1e-15)
+        np.testing.assert_allclose(R.bin_ascans(x, B, np.float32)[0, :, 0], want, rtol=2e-7)
+        assert len(R.binned_columns(10, B)) == len(want)
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 7])
+def test_binning_brute_force_window(B):
+    """Every kept line is the zero-padded mean of lines [c - ceil(B/2) + 1, c + floor(B/2)]
+    (filter2 'same' with a length-B box), checked by a direct loop on ragged line counts."""
+    rng = np.random.default_rng(B)
+    for n in (1, B, 2 * B + 1, 23):
+        if len(R.binned_columns(n, B)) == 0:          # fewer lines than the first kept one
+            continue
+        x = rng.normal(size=(2, n, 3))
+        got = R.bin_ascans(x, B, np.float64)
+        cols = R.binned_columns(n, B)
+        assert got.shape == (2, len(cols), 3)
+        for k, c in enumerate(cols):
+            lo, hi = c - (B + 1) // 2 + 1, c + B // 2
+            want = sum(x[:, j] for j in range(max(lo, 0), min(hi, n - 1) + 1)) / B
+            np.testing.assert_allclose(got[:, k], want, rtol=1e-13, atol=1e-15)
+
+
+def test_prepare_raw_keeps_unbinned_background():
+    """Binning leaves the background lines (or the frame-mean line, taken from the unbinned
+    lines) untouched and bins only the scan lines."""
+    rng = np.random.default_rng(4)
+    raw = rng.integers(0, 4000, size=(4, 25 + 11, 8)).astype(np.uint16)
+    pre, a = R.prepare_raw(raw, 25, 3, "lines")
+    assert a == 25 and pre.shape == (4, 25 + len(R.binned_columns(11, 3)), 8)
+    assert np.array_equal(pre[:, :25], raw[:, :25].astype(np.float32))
+    assert np.array_equal(pre[:, 25:], R.bin_ascans(raw[:, 25:], 3))
+    pre, a = R.prepare_raw(raw, 25, 2, "frame_mean", 2)
+    assert a == 1
+    fm = raw.astype(np.float64).reshape(2, 2, 36, 8).mean(2).mean(1)
+    np.testing.assert_allclose(pre[:, 0], np.repeat(fm, 2, 0), rtol=1e-6)
+    assert np.array_equal(pre[:, 1:], R.bin_ascans(raw, 2))
+
+
+def test_average_repeats_axes():
+    """B-scan repeats are consecutive files, A-scan repeats consecutive A-lines: with values
+    1000 f + 100 b + 10 x + a the means are exact, and a swapped axis changes them."""
+    F, nb, X, na, Z = 3, 4, 5, 2, 3
+    f, b, x, a, z = np.meshgrid(*(np.arange(n) for n in (F, nb, X, na, Z)), indexing="ij")
+    mag = (1000 * f + 100 * b + 10 * x + a + 0 * z).astype(np.float32).reshape(F * nb, X * na, Z)
+    got = R.average_repeats(mag, F, nb, X, na)
+    want = (1000 * np.arange(F)[:, None, None] + 100 * (nb - 1) / 2 + 10 * np.arange(X)[None, :, None]
+            + (na - 1) / 2 + 0 * np.arange(Z))
+    assert got.shape == (F, X, Z) and np.array_equal(got, want.astype(np.float32))
+    assert R.average_repeats(mag, F * nb, 1, X * na, 1) is mag
