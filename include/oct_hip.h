@@ -128,6 +128,13 @@ int oct_stitch_row_blocks(int n_out_z, int n_out_x);
 int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x, int n_z, const int32_t* frame_idx,
                    const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, int n_out_z, int n_out_x,
                    float thr, float* db, int plane0, int* partial, void* stream);
+/* The same for rows of any number of tiles (oct_stitch_row takes at most 40 in its argument
+ * block): dev_tiles is a device copy of the host table `tiles` (which is validated), read by the
+ * kernel. */
+int oct_stitch_row_table(const oct_row_tile* tiles, const oct_row_tile* dev_tiles, int n_tiles, int n_frames, int n_x,
+                         int n_z, const int32_t* frame_idx, const int32_t* shift, const int32_t* valid_lo,
+                         const int32_t* valid_hi, int n_out_z, int n_out_x, float thr, float* db, int plane0,
+                         int* partial, void* stream);
 /* plane clims (n_planes, 2) and the global clim (2) from row partials */
 int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
                     void* stream);
@@ -204,6 +211,15 @@ int oct_opplan_create(oct_opplan** plan, int device, int n_lambda, int taps, con
                       const double* band_w, const double* window, int z0, int nzc, int* scale_log2,
                       double* op64_host);
 int oct_opplan_destroy(oct_opplan* plan);
+/* Rebuild the operator in place, on `stream`, for another window (N, 2) float64 device array
+ * (NULL: the creation window): the same float64 build and fp16 hi/lo split, with the scale
+ * 2^scale_log2 fixed at creation, so |window| must not exceed oct_opplan_scale_bound's value
+ * (a dispersion change alters only the phase).  Launches stay in stream order with the plan's
+ * oct_opspectral calls; calls on one plan must use one stream.  Used to reconstruct each tile
+ * with its own dispersion (the reference applies one dispersion per volume,
+ * octrecon/pipeline.py:155-190). */
+int oct_opplan_set_window(oct_opplan* plan, const double* window, void* stream);
+int oct_opplan_scale_bound(const oct_opplan* plan, double* max_window_abs);
 /* magnitudes |A (raw - bg)| of bins [z0, z0+nzc) -> out (M, out_stride) float32 */
 int oct_opspectral(const oct_opplan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                    int apod_size, float* out, long long out_stride, const float* bg, void* stream);

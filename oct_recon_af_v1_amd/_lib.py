@@ -50,12 +50,16 @@ SIGNATURES = {
     "oct_stitch_row_blocks": (_i, [_i, _i]),
     "oct_stitch_row": (_i, [ctypes.POINTER(RowTile), _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _f, _vp, _i, _vp,
                             _vp]),
+    "oct_stitch_row_table": (_i, [ctypes.POINTER(RowTile), _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _i, _f, _vp,
+                                  _i, _vp, _vp]),
     "oct_volume_clim": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
     "oct_finalize_workspace_bytes": (_sz, [_i, _ll]),
     "oct_finalize": (_i, [_vp, _vp, _vp, _i, _ll, _f, _vp, _vp, _vp, _sz, _vp]),
     "oct_quantize": (_i, [_vp, _vp, _i, _ll, _vp, _vp, _i, _vp]),
     "oct_opplan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), _vp]),
     "oct_opplan_destroy": (_i, [_vp]),
+    "oct_opplan_set_window": (_i, [_vp, _vp, _vp]),
+    "oct_opplan_scale_bound": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_plan_use_window": (_i, [_vp, _vp]),
     "oct_topfrac_metric_workspace_bytes": (_sz, [_i, _i, _i, _i]),

@@ -1,4 +1,5 @@
-"""Build liboct_hip.so (gfx950) in-tree with hipcc.
+"""Build liboct_hip.so (gfx950) in-tree with hipcc, and the host tile generator
+liboct_hostgen.so with gcc.
 
     python -m oct_recon_af_v1_amd.build
 """
@@ -30,6 +31,24 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+HOSTGEN_SRC = PKG / "csrc" / "hostgen.c"
+HOSTGEN_LIB = PKG / "liboct_hostgen.so"
+# host generator: AVX2/FMA (present on every x86-64 host of an MI355X system), OpenMP threads
+GCC_FLAGS = ["-O3", "-mavx2", "-mfma", "-fno-math-errno", "-fopenmp", "-fPIC", "-shared", "-std=c11"]
+
+
+def build_hostgen(verbose: bool = False) -> Path:
+    tmp = HOSTGEN_LIB.with_suffix(".so.tmp")
+    cmd = [os.environ.get("CC", "gcc"), *GCC_FLAGS, str(HOSTGEN_SRC), "-o", str(tmp), "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
+    os.replace(tmp, HOSTGEN_LIB)
+    return HOSTGEN_LIB
+
+
 def build(verbose: bool = False) -> Path:
     srcs = sorted((PKG / "csrc").glob("*.hpp")) + [SRC, ROOT / "include" / "oct_hip.h"]
     tmp = LIB.with_suffix(".so.tmp")
@@ -41,6 +60,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
     os.replace(tmp, LIB)
     del srcs
+    build_hostgen(verbose)
     return LIB
 
 

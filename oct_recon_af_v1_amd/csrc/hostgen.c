@@ -1,0 +1,234 @@
+/* Host (CPU) generator of streamed-volume tiles: seeded synthetic spectral-domain OCT B-scans,
+ * uint16 12-bit counts, written straight into a caller buffer (pinned host memory in the
+ * streaming driver).  Same physics as the device generator of the estimator workload
+ * (oct_recon_af_v1_amd/synthetic.py make_focus_bscans): a scattering medium below a tissue
+ * surface plus discrete reflectors, all under a confocal gate at the tile's focus depth, with
+ * the tile's quadratic dispersion, on the spectrometer's non-uniform k grid, with shot noise.
+ *
+ * Made cheap enough to stream (tens of GB per second of host cores) by two choices:
+ *   - per tile, banks of K_m medium spectra and K_r reflector fringes are built once (the
+ *     medium exactly at the non-uniform k: G(k_j) = sum_z c(z) e^{2 pi i z kn_j / N} by
+ *     Horner's rule in z, vectorised over j; dispersion e^{i beta dq_j} applied to the bank);
+ *     each A-scan is 1 + Re(sum of n_med bank spectra with complex Gaussian weights of
+ *     variance 1/n_med) + sum of n_ref bank fringes with log-uniform amplitudes;
+ *   - shot noise is Gaussian with the Poisson variance (rate + sqrt(rate) n): n is the
+ *     standardised sum of the four bytes of a per-lane xorshift32 draw (Irwin-Hall, 4 terms),
+ *     then counts are rounded to nearest and clipped to [0, 4095].
+ * The per-sample pass is written with AVX2/FMA intrinsics (16 samples per step, one pass over
+ * each line); the banks are plain C left to the compiler's vectoriser.
+ * Every A-line draws from its own stream seeded by (seed, file, line), so the output does
+ * not depend on the thread count.
+ *
+ * params (double[OCTG_NPARAM]): see the OCTG_* indices below.  Returns 0, or -1 on bad
+ * arguments / allocation failure.
+ */
+#include <immintrin.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+enum {
+    OCTG_RMIN, OCTG_RMAX,           /* reflector intensity reflectivity, log-uniform */
+    OCTG_ZLO, OCTG_ZHI,             /* reflector depths (equispaced-k bins), uniform */
+    OCTG_SCAT, OCTG_ATT,            /* medium reflectivity per bin, attenuation length (bins) */
+    OCTG_SURF_LO, OCTG_SURF_HI,     /* tissue surface depth per tile, uniform */
+    OCTG_GFLOOR, OCTG_GWIDTH,       /* confocal gain floor + exp(-(z - focus)^2 / (2 width^2)) */
+    OCTG_NDEPTH,                    /* medium depth bins */
+    OCTG_KM, OCTG_KR,               /* bank sizes */
+    OCTG_NMED, OCTG_NREF,           /* bank terms per A-scan */
+    OCTG_NOISE,                     /* 1: shot noise, 0: expected counts */
+    OCTG_NPARAM
+};
+
+#define OCTG_PI 3.14159265358979323846
+
+/* splitmix64 (Steele, Lea, Flood 2014): seeding and per-line streams */
+static inline uint64_t sm64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static inline double u01(uint64_t* s) { return (double)(sm64(s) >> 11) * (1.0 / 9007199254740992.0); }
+static inline double gauss(uint64_t* s) {
+    double u = u01(s);
+    if (u < 1e-300) u = 1e-300;
+    return sqrt(-2.0 * log(u)) * cos(2.0 * OCTG_PI * u01(s));
+}
+static inline double gate(const double* p, double z, double focus) {
+    const double d = (z - focus) / p[OCTG_GWIDTH];
+    return p[OCTG_GFLOOR] + exp(-0.5 * d * d);
+}
+
+int octg_params_count(void) { return OCTG_NPARAM; }
+
+int octg_focus_tile(uint16_t* out, int n_files, int n_x, int apod, int n, const double* kn, const double* dq,
+                    const double* src, double beta_idx, double focus_bins, uint64_t seed, const double* p,
+                    int threads) {
+    if (!out || !kn || !dq || !src || !p || n_files < 0 || n_x < 0 || apod < 0 || n <= 0 || (n % 16) ||
+        ((uintptr_t)out & 15))
+        return -1;
+    const int km = (int)p[OCTG_KM], kr = (int)p[OCTG_KR], nmed = (int)p[OCTG_NMED], nref = (int)p[OCTG_NREF];
+    const int nd = (int)p[OCTG_NDEPTH];
+    if (km < 1 || kr < 1 || nmed < 0 || nref < 0 || nd < 1 || nmed > 8 || nref > 16) return -1;
+    const int lines = apod + n_x;
+    if (threads < 1) threads = 1;
+
+    float* bank = (float*)malloc(sizeof(float) * (size_t)(2 * km + kr) * n);
+    float* rr = (float*)malloc(sizeof(float) * n);
+    float* ri = (float*)malloc(sizeof(float) * n);
+    float* srcf = (float*)malloc(sizeof(float) * n);
+    uint64_t* seeds = (uint64_t*)malloc(sizeof(uint64_t) * (km + kr));
+    if (!bank || !rr || !ri || !srcf || !seeds) {
+        free(bank); free(rr); free(ri); free(srcf); free(seeds);
+        return -1;
+    }
+    uint64_t ts = seed * 0xD1342543DE82EF95ULL + 0x2545F4914F6CDD1DULL;
+    const double surf = p[OCTG_SURF_LO] + (p[OCTG_SURF_HI] - p[OCTG_SURF_LO]) * u01(&ts);
+    for (int i = 0; i < km + kr; ++i) seeds[i] = sm64(&ts);
+    for (int j = 0; j < n; ++j) {
+        const double th = 2.0 * OCTG_PI * kn[j] / n;
+        rr[j] = (float)cos(th);
+        ri[j] = (float)sin(th);
+        srcf[j] = (float)src[j];
+    }
+
+    /* ---- medium bank: speckle spectra under the gate, with dispersion */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
+    for (int m = 0; m < km; ++m) {
+        uint64_t s = seeds[m];
+        float* gr = bank + (size_t)(2 * m) * n;
+        float* gi = bank + (size_t)(2 * m + 1) * n;
+        float* cre = (float*)malloc(sizeof(float) * nd * 2);
+        if (!cre) continue;
+        float* cim = cre + nd;
+        for (int z = 0; z < nd; ++z) {
+            const double inten = z >= surf ? p[OCTG_SCAT] * exp(-(z - surf) / p[OCTG_ATT]) * gate(p, z, focus_bins) : 0.0;
+            const double a = sqrt(0.5 * inten);
+            cre[z] = (float)(a * gauss(&s));
+            cim[z] = (float)(a * gauss(&s));
+        }
+        memset(gr, 0, sizeof(float) * n);
+        memset(gi, 0, sizeof(float) * n);
+        for (int z = nd - 1; z >= 0; --z) {
+            const float cr = cre[z], ci = cim[z];
+            for (int j = 0; j < n; ++j) {
+                const float a = gr[j], b = gi[j];
+                gr[j] = a * rr[j] - b * ri[j] + cr;
+                gi[j] = a * ri[j] + b * rr[j] + ci;
+            }
+        }
+        for (int j = 0; j < n; ++j) {
+            const double ph = beta_idx * dq[j];
+            const double c = cos(ph), sn = sin(ph);
+            const double a = gr[j], b = gi[j];
+            gr[j] = (float)(a * c - b * sn);
+            gi[j] = (float)(a * sn + b * c);
+        }
+        free(cre);
+    }
+
+    /* ---- reflector bank: gated fringes at random depths and phases, with dispersion */
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4)
+    for (int r = 0; r < kr; ++r) {
+        uint64_t s = seeds[km + r];
+        const double z = p[OCTG_ZLO] + (p[OCTG_ZHI] - p[OCTG_ZLO]) * u01(&s);
+        const double phi = 2.0 * OCTG_PI * u01(&s);
+        const double g = sqrt(gate(p, z, focus_bins));
+        float* f = bank + (size_t)(2 * km + r) * n;
+        for (int j = 0; j < n; ++j) f[j] = (float)(g * cos(2.0 * OCTG_PI * z * kn[j] / n + phi + beta_idx * dq[j]));
+    }
+
+    /* ---- A-lines: 1 + sum_t w_t bank_t[j] over all terms of the line, times the source, then
+     *      noise and rounding, in blocks of 16 samples held in registers (one pass per line) */
+    const int noise = p[OCTG_NOISE] != 0.0;
+    const double lr0 = log(p[OCTG_RMIN]), lr1 = log(p[OCTG_RMAX]);
+    const long long total = (long long)n_files * lines;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 8)
+    for (long long item = 0; item < total; ++item) {
+        const long long b = item / lines;
+        const int l = (int)(item - b * lines);
+        uint64_t s = seed ^ (0x9E3779B97F4A7C15ULL * (uint64_t)(item + 1));
+        sm64(&s);
+        uint16_t* o = out + (size_t)item * n;
+        const float* tp[2 * 8 + 16];
+        float tw[2 * 8 + 16];
+        int nt = 0;
+        float one = 0.f;
+        if (l >= apod) {
+            one = 1.f;
+            const double wsd = nmed > 0 ? sqrt(0.5 / nmed) : 0.0;
+            for (int t = 0; t < nmed; ++t) {
+                const int m = (int)(u01(&s) * km);
+                const float wr = (float)(wsd * gauss(&s)), wi = (float)(wsd * gauss(&s));
+                tp[nt] = bank + (size_t)(2 * m) * n;          /* Re(w G) = wr Re G - wi Im G */
+                tw[nt++] = wr;
+                tp[nt] = bank + (size_t)(2 * m + 1) * n;
+                tw[nt++] = -wi;
+            }
+            for (int t = 0; t < nref; ++t) {
+                const int r = (int)(u01(&s) * kr);
+                tp[nt] = bank + (size_t)(2 * km + r) * n;
+                tw[nt++] = (float)exp(0.5 * (lr0 + (lr1 - lr0) * u01(&s)));
+            }
+        }
+        /* two xorshift32 states of 8 lanes each: 16 samples per step */
+        uint32_t seed_lanes[16];
+        for (int q = 0; q < 16; ++q) {
+            uint32_t v = (uint32_t)sm64(&s);
+            seed_lanes[q] = v ? v : 0x6D2B79F5u;
+        }
+        __m256i x0 = _mm256_loadu_si256((const __m256i*)seed_lanes);
+        __m256i x1 = _mm256_loadu_si256((const __m256i*)(seed_lanes + 8));
+        const __m256 vone = _mm256_set1_ps(one), vzero = _mm256_setzero_ps(), vmax = _mm256_set1_ps(4095.f);
+        const __m256i bytes = _mm256_set1_epi32(255);
+        const __m256 off = _mm256_set1_ps(510.0f), inv_sd = _mm256_set1_ps(1.0f / 147.80054f);
+        const __m256 tiny = _mm256_set1_ps(1e-20f);
+        for (int j0 = 0; j0 < n; j0 += 16) {
+            __m256 a0, a1;
+            if (l < apod) {                                      /* background line: the source */
+                a0 = _mm256_loadu_ps(srcf + j0);
+                a1 = _mm256_loadu_ps(srcf + j0 + 8);
+            } else {
+                a0 = vone;
+                a1 = vone;
+                for (int t = 0; t < nt; ++t) {
+                    const __m256 w = _mm256_set1_ps(tw[t]);
+                    a0 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0), a0);
+                    a1 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0 + 8), a1);
+                }
+                a0 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0), a0), vzero);
+                a1 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0 + 8), a1), vzero);
+            }
+            if (noise) {
+                x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 13));
+                x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 13));
+                x0 = _mm256_xor_si256(x0, _mm256_srli_epi32(x0, 17));
+                x1 = _mm256_xor_si256(x1, _mm256_srli_epi32(x1, 17));
+                x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 5));
+                x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 5));
+                __m256i s0 = _mm256_add_epi32(_mm256_and_si256(x0, bytes), _mm256_and_si256(_mm256_srli_epi32(x0, 8), bytes));
+                __m256i s1 = _mm256_add_epi32(_mm256_and_si256(x1, bytes), _mm256_and_si256(_mm256_srli_epi32(x1, 8), bytes));
+                s0 = _mm256_add_epi32(s0, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x0, 16), bytes), _mm256_srli_epi32(x0, 24)));
+                s1 = _mm256_add_epi32(s1, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x1, 16), bytes), _mm256_srli_epi32(x1, 24)));
+                const __m256 z0 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s0), off), inv_sd);
+                const __m256 z1 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s1), off), inv_sd);
+                /* sqrt(rate) as rate * rsqrt(rate) (12-bit estimate: the noise's own scale) */
+                const __m256 r0 = _mm256_max_ps(a0, tiny), r1 = _mm256_max_ps(a1, tiny);
+                a0 = _mm256_fmadd_ps(_mm256_mul_ps(r0, _mm256_rsqrt_ps(r0)), z0, a0);
+                a1 = _mm256_fmadd_ps(_mm256_mul_ps(r1, _mm256_rsqrt_ps(r1)), z1, a1);
+            }
+            a0 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a0, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
+            a1 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a1, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
+            const __m256i i0 = _mm256_cvtps_epi32(a0), i1 = _mm256_cvtps_epi32(a1);
+            const __m128i p0 = _mm_packus_epi32(_mm256_castsi256_si128(i0), _mm256_extracti128_si256(i0, 1));
+            const __m128i p1 = _mm_packus_epi32(_mm256_castsi256_si128(i1), _mm256_extracti128_si256(i1, 1));
+            _mm_stream_si128((__m128i*)(o + j0), p0);          /* o is 16-byte aligned (checked) */
+            _mm_stream_si128((__m128i*)(o + j0 + 8), p1);
+        }
+        _mm_sfence();                   /* this thread's streaming stores are visible */
+    }
+    free(bank); free(rr); free(ri); free(srcf); free(seeds);
+    return 0;
+}

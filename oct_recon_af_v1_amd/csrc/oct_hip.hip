@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 
@@ -381,15 +382,24 @@ int oct_stitch_row_blocks(int n_out_z, int n_out_x) {
     return gx * gz;
 }
 
-int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x, int n_z, const int32_t* frame_idx,
-                   const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, int n_out_z, int n_out_x,
-                   float thr, float* db, int plane0, int* partial, void* stream) {
+static_assert(sizeof(oct::RowTile) == sizeof(oct_row_tile) && offsetof(oct::RowTile, fz) == offsetof(oct_row_tile, fz) &&
+                  offsetof(oct::RowTile, mstride) == offsetof(oct_row_tile, mag_stride) &&
+                  offsetof(oct::RowTile, zr) == offsetof(oct_row_tile, z_rows),
+              "the device tile table is read with the oct_row_tile layout");
+
+static int stitch_row_impl(const oct_row_tile* tiles, const oct_row_tile* dev_tiles, int n_tiles, int n_frames, int n_x,
+                           int n_z, const int32_t* frame_idx, const int32_t* shift, const int32_t* valid_lo,
+                           const int32_t* valid_hi, int n_out_z, int n_out_x, float thr, float* db, int plane0,
+                           int* partial, void* stream) {
     if (!tiles || !frame_idx || !db || !partial) return fail(OCT_EINVAL, "null argument");
-    if (n_tiles < 0 || n_tiles > oct::kRowMaxTiles) return fail(OCT_EINVAL, "%d tiles (at most %d per row)", n_tiles, oct::kRowMaxTiles);
+    if (n_tiles < 0 || (!dev_tiles && n_tiles > oct::kRowMaxTiles))
+        return fail(OCT_EINVAL, "%d tiles (at most %d per row in the argument block)", n_tiles, oct::kRowMaxTiles);
+    if (dev_tiles && ((uintptr_t)dev_tiles & 7)) return fail(OCT_EINVAL, "tile table must be 8-byte aligned");
     if (shift && (!valid_lo || !valid_hi)) return fail(OCT_EINVAL, "shift without valid bounds");
     if (n_frames <= 0 || n_out_z <= 0 || n_out_x <= 0) return OCT_OK;
     if (n_x <= 0 || n_z <= 0 || plane0 < 0) return fail(OCT_EINVAL, "bad row shape");
     oct::RowArgs a;
+    a.tt = reinterpret_cast<const oct::RowTile*>(dev_tiles);
     for (int i = 0; i < n_tiles; ++i) {
         const oct_row_tile& t = tiles[i];
         if (!t.mag || !t.z_rows || !t.z_w || !t.x_cols || !t.x_w || !t.fz) return fail(OCT_EINVAL, "tile %d: null pointer", i);
@@ -397,6 +407,7 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
         if (((uintptr_t)t.z_rows | (uintptr_t)t.z_w | (uintptr_t)t.x_cols | (uintptr_t)t.x_w) & 7)
             return fail(OCT_EINVAL, "tile %d: tap arrays must be 8-byte aligned", i);
         if (t.c0 < 0 || t.C <= 0 || t.c0 + t.C > n_out_x) return fail(OCT_EINVAL, "tile %d: columns outside the output", i);
+        if (i >= oct::kRowMaxTiles) continue;             // the kernel reads the device table
         a.t[i].mag = t.mag;
         a.t[i].mstride = t.mag_stride;
         a.t[i].mrow0 = t.mag_row0;
@@ -427,14 +438,33 @@ int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x
     const long long total = (long long)a.blocks_per_plane * n_frames;
     if (total > (1LL << 30)) return fail(OCT_EINVAL, "too many planes in one call");
     dim3 grid((unsigned)((total + 7) / 8 * 8));            // whole multiple of the 8 XCDs
-    if (shift)
-        hipLaunchKernelGGL(oct::stitch_row_kernel<true>, grid, dim3(64 * oct::kRowWaves), 0,
-                           reinterpret_cast<hipStream_t>(stream), a);
-    else
-        hipLaunchKernelGGL(oct::stitch_row_kernel<false>, grid, dim3(64 * oct::kRowWaves), 0,
-                           reinterpret_cast<hipStream_t>(stream), a);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const dim3 blk(64 * oct::kRowWaves);
+    if (dev_tiles) {
+        if (shift) hipLaunchKernelGGL((oct::stitch_row_kernel<true, true>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((oct::stitch_row_kernel<false, true>), grid, blk, 0, s, a);
+    } else {
+        if (shift) hipLaunchKernelGGL((oct::stitch_row_kernel<true, false>), grid, blk, 0, s, a);
+        else hipLaunchKernelGGL((oct::stitch_row_kernel<false, false>), grid, blk, 0, s, a);
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
+}
+
+int oct_stitch_row(const oct_row_tile* tiles, int n_tiles, int n_frames, int n_x, int n_z, const int32_t* frame_idx,
+                   const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, int n_out_z, int n_out_x,
+                   float thr, float* db, int plane0, int* partial, void* stream) {
+    return stitch_row_impl(tiles, nullptr, n_tiles, n_frames, n_x, n_z, frame_idx, shift, valid_lo, valid_hi, n_out_z,
+                           n_out_x, thr, db, plane0, partial, stream);
+}
+
+int oct_stitch_row_table(const oct_row_tile* tiles, const oct_row_tile* dev_tiles, int n_tiles, int n_frames, int n_x,
+                         int n_z, const int32_t* frame_idx, const int32_t* shift, const int32_t* valid_lo,
+                         const int32_t* valid_hi, int n_out_z, int n_out_x, float thr, float* db, int plane0,
+                         int* partial, void* stream) {
+    if (!dev_tiles) return fail(OCT_EINVAL, "null argument");
+    return stitch_row_impl(tiles, dev_tiles, n_tiles, n_frames, n_x, n_z, frame_idx, shift, valid_lo, valid_hi, n_out_z,
+                           n_out_x, thr, db, plane0, partial, stream);
 }
 
 int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
@@ -575,10 +605,19 @@ int oct_dispersion_windows(const double* betas, const float* head, int head_stri
 struct oct_opplan {
     int device;
     int n;
-    int z0, nzc, r_pad;
+    int z0, nzc, r_pad, nzc_pad;
     int sa;
+    int tt;                       // transposed-band width
+    double max_win;               // largest |window| of the creation window (sets sa)
     _Float16* op_hi;
     _Float16* op_lo;
+    // kept for in-place rebuilds with another window (oct_opplan_set_window)
+    int* d_start;
+    int* d_count;
+    double* d_w;
+    double* d_win;                // the creation window (N, 2)
+    double* d_u;
+    double2* d_c;                 // (nzc) rank-1 term, rebuilt with each window
 };
 
 int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, const int32_t* band_start,
@@ -640,7 +679,10 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     p->z0 = z0;
     p->nzc = nzc;
     p->r_pad = r_pad;
+    p->nzc_pad = nzc_pad;
     p->sa = sa;
+    p->tt = TT;
+    p->max_win = amax_win;
     int* d_start = nullptr;
     int* d_count = nullptr;
     double *d_w = nullptr, *d_win = nullptr, *d_u = nullptr, *d_op64 = nullptr;
@@ -670,12 +712,12 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
             e = hipMemcpy(op64_host, d_op64, sizeof(double) * 2 * (size_t)nzc * N, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rc = fail(OCT_EHIP, "operator build failed: %s", hipGetErrorString(e));
     }
-    hipFree(d_start);
-    hipFree(d_count);
-    hipFree(d_w);
-    hipFree(d_win);
-    hipFree(d_u);
-    hipFree(d_c);
+    p->d_start = d_start;
+    p->d_count = d_count;
+    p->d_w = d_w;
+    p->d_win = d_win;
+    p->d_u = d_u;
+    p->d_c = d_c;
     if (d_op64) hipFree(d_op64);
     hipSetDevice(prev);
     if (rc != OCT_OK) {
@@ -694,8 +736,32 @@ int oct_opplan_destroy(oct_opplan* p) {
     hipSetDevice(p->device);
     if (p->op_hi) hipFree(p->op_hi);
     if (p->op_lo) hipFree(p->op_lo);
+    if (p->d_start) hipFree(p->d_start);
+    if (p->d_count) hipFree(p->d_count);
+    if (p->d_w) hipFree(p->d_w);
+    if (p->d_win) hipFree(p->d_win);
+    if (p->d_u) hipFree(p->d_u);
+    if (p->d_c) hipFree(p->d_c);
     hipSetDevice(prev);
     delete p;
+    return OCT_OK;
+}
+
+int oct_opplan_scale_bound(const oct_opplan* p, double* max_window_abs) {
+    if (!p || !max_window_abs) return fail(OCT_EINVAL, "null argument");
+    *max_window_abs = p->max_win;
+    return OCT_OK;
+}
+
+int oct_opplan_set_window(oct_opplan* p, const double* window, void* stream) {
+    if (!p) return fail(OCT_EINVAL, "null argument");
+    const double* win = window ? window : p->d_win;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(oct::opbuild_c_kernel, dim3(p->nzc), dim3(256), 0, s, win, p->d_u, p->n, p->z0, p->d_c);
+    hipLaunchKernelGGL(oct::opbuild_a_kernel, dim3((p->n + 255) / 256, p->nzc_pad), dim3(256), 0, s, p->d_start,
+                       p->d_count, p->d_w, p->tt, win, p->d_c, p->n, p->z0, p->nzc, p->nzc_pad, std::ldexp(1.0, p->sa),
+                       p->op_hi, p->op_lo, (double*)nullptr);
+    HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
 

@@ -106,6 +106,7 @@ struct RowTile {
 };
 struct RowArgs {
     RowTile t[kRowMaxTiles];
+    const RowTile* tt;      // the tile table in device memory (the kTable variant: any count)
     int ntiles;
     int nX, nZ;
     const int* frame_idx;   // (nF) tile-local y frame of each output plane (for the optical path)
@@ -129,7 +130,9 @@ constexpr int kRowZ = 32, kRowC = 64, kRowWaves = 4;
 constexpr int kRowLanesC = 64 / kRowZ;                        // columns in flight per wave
 constexpr int kRowPasses = kRowC / (kRowWaves * kRowLanesC);  // column passes per wave
 
-template <bool kShift>   // optical-path shift present: kernels without it carry no per-x lookups
+// kShift: optical-path shift present (kernels without it carry no per-x lookups);
+// kTable: tile descriptors read from the device table a.tt instead of the argument block.
+template <bool kShift, bool kTable>
 __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
     __shared__ float tile[kRowZ][kRowC + 1];
     __shared__ int smin[kRowWaves], smax[kRowWaves];
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
     // element and contribute an exact +0 through their zero weight.
     const int zc = zok ? zo : 0;
     for (int ti = 0; ti < a.ntiles; ++ti) {
-        const RowTile& t = a.t[ti];
+        const RowTile& t = kTable ? a.tt[ti] : a.t[ti];
         if (cb0 + kRowC <= t.c0 || cb0 >= t.c0 + t.C) continue;       // uniform per block
         const int2 rr = reinterpret_cast<const int2*>(t.zr)[zc];
         const float2 ww = reinterpret_cast<const float2*>(t.zw)[zc];

@@ -199,11 +199,7 @@ def build_spectral_geometry(lambda_nm, dispersion_quadratic_term: float, n_mediu
         raise ValueError(interp_method)
 
     lambda_eq = pchip(k, lam, k_lin)
-    k_eq = 2 * np.pi / lambda_eq
-    hann = 0.5 * (1 - np.cos(2 * np.pi * np.arange(N) / (N - 1)))     # symmetric Hann
-    hann = hann / np.sqrt(np.mean(hann ** 2))
-    phase = -dispersion_quadratic_term * (k_eq - k_eq.mean()) ** 2
-    window = hann * np.exp(1j * phase)
+    window = dispersion_window(lambda_eq, dispersion_quadratic_term)
 
     ends = np.sort([lambda_eq[0], lambda_eq[-1]])
     lambda0_um = ends.mean() / 1e3
@@ -212,6 +208,20 @@ def build_spectral_geometry(lambda_nm, dispersion_quadratic_term: float, n_mediu
     z_um = matlab_linspace(0.0, z_step * N / 2, N // 2)
     return SpectralGeometry(lam, k, k_lin, lambda_eq, idx, w, window, z_um, interp_method,
                             bstart, band_w, float(dispersion_quadratic_term), float(n_medium))
+
+
+def dispersion_window(lambda_eq_nm, dispersion_quadratic_term):
+    """Hann/rms x exp(-i beta (k_eq - <k_eq>)^2) on the equispaced-k grid (complex128 (N,)),
+    octrecon/core/geometry.py:115-119; beta may be an array (-> (len(beta), N))."""
+    lambda_eq_nm = np.asarray(lambda_eq_nm, float)
+    N = len(lambda_eq_nm)
+    k_eq = 2 * np.pi / lambda_eq_nm
+    hann = 0.5 * (1 - np.cos(2 * np.pi * np.arange(N) / (N - 1)))     # symmetric Hann
+    hann = hann / np.sqrt(np.mean(hann ** 2))
+    beta = np.asarray(dispersion_quadratic_term, float)
+    phase = -beta[..., None] * (k_eq - k_eq.mean()) ** 2 if beta.ndim else \
+        -dispersion_quadratic_term * (k_eq - k_eq.mean()) ** 2
+    return hann * np.exp(1j * phase)
 
 
 def beta_from_index_units(beta_idx, lambda_nm):

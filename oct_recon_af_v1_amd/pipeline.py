@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib
 from .backend import get_xp, synchronize
-from .geometry import ScanGrid, build_spectral_geometry, build_tiled_geometry
+from .geometry import ScanGrid, build_spectral_geometry, build_tiled_geometry, dispersion_window
 from .quantize import DeviceFinalizer
 from .spectral import HipSpectralProcessor
 from .stitching import HipTileStitcher, OpticalPathCorrection, TileStitcher, min_weight_threshold
@@ -47,6 +47,9 @@ class ReconConfig:
     fused_rows: bool = True                 # one stitch+finalise pass per y row (False: per-tile contribute)
 
 
+_ROW_ARG_TILES = 40        # tiles a row launch carries in its argument block (oct_stitch_row)
+
+
 class Cancelled(Exception):
     pass
 
@@ -54,10 +57,14 @@ class Cancelled(Exception):
 class Reconstructor:
     def __init__(self, cfg: ReconConfig, scan: ScanGrid, lambda_nm, tiles, apod_size: int,
                  apod_mode: str = "lines", opc_poly=None, xp=None, log=None, ascan_binning: int = 1,
-                 bscan_avg: int = 1, ascan_avg: int = 1):
+                 bscan_avg: int = 1, ascan_avg: int = 1, tile_betas=None):
         """ascan_binning, bscan_avg, ascan_avg: the tile header's spectra_avg (A-line binning),
         B-scan repeats (consecutive raw files per frame) and A-scan repeats (consecutive A-lines
-        per scan position), octrecon/pipeline.py:138-141,177-183,216-219,276-279."""
+        per scan position), octrecon/pipeline.py:138-141,177-183,216-219,276-279.
+        tile_betas: optional {(xi, yi, zi): dispersion} (reference units) reconstructing each
+        listed tile with its own dispersion window (plain form: the plan's window pointer per
+        launch; MFMA form: the operator rebuilt on the device per tile); the reference applies
+        cfg.dispersion_quadratic_term to every tile."""
         if cfg.precision != "float32":
             raise ValueError("the hip device computes in float32")
         self.cfg = cfg
@@ -91,6 +98,9 @@ class Reconstructor:
                 base = TileStitcher(self.tg, self.oc, xc, zd, focus[zi], cfg.focus_sigma)
                 self.stitchers[(xi, zi)] = HipTileStitcher(base, self.xp, self.n_out[1])
         self.fin = DeviceFinalizer(self.xp)
+        self._tile_win = None
+        if tile_betas is not None:
+            self.set_tile_betas(tile_betas)
         self._mag = None
         self._tile_mag = {}
         self._partial = None
@@ -98,6 +108,48 @@ class Reconstructor:
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
 
     # ------------------------------------------------------------------ data access
+    def set_tile_betas(self, tile_betas):
+        """Per-tile dispersion windows on the device: float32 (already times 1/N) for the plain
+        kernel, float64 for the operator build; both from geometry.dispersion_window, as the
+        plan's own window."""
+        keys = sorted(tile_betas)
+        if not keys:
+            self._tile_win = None
+            return
+        w = dispersion_window(self.sg.lambda_eq_nm, np.array([tile_betas[k] for k in keys], float))
+        w64 = np.ascontiguousarray(np.stack([w.real, w.imag], -1))
+        w32 = np.ascontiguousarray(w64.astype(np.float32) / np.float32(self.sg.n))   # exact: 2^-11
+        torch = self.torch
+        self._tile_win = {
+            "index": {k: i for i, k in enumerate(keys)},
+            "f32": torch.from_numpy(w32).to(self.xp.device),
+            "f64": torch.from_numpy(w64).to(self.xp.device),
+            "max_abs": float(np.abs(w).max()),
+        }
+
+    def _use_tile_window(self, key, lo, hi):
+        """Point the next spectral launch at tile `key`'s window (None: the volume's)."""
+        tw = self._tile_win
+        if tw is None:
+            return
+        i = tw["index"].get(key)
+        if self.cfg.form == "mfma":
+            op = self.sp.operator_plan(lo, hi - lo)
+            if i is not None and tw["max_abs"] > op.max_window_abs() * (1 + 1e-12):
+                raise ValueError("tile window exceeds the operator's scale bound")
+            ptr = None if i is None else tw["f64"][i].data_ptr()
+            op.set_window(ptr, self.xp.stream())
+        else:
+            ptr = None if i is None else tw["f32"][i].data_ptr()
+            _lib.check(_lib.lib().oct_plan_use_window(self.sp.plan.handle, ptr), "oct_plan_use_window")
+
+    def _restore_windows(self):
+        if self._tile_win is None:
+            return
+        _lib.check(_lib.lib().oct_plan_use_window(self.sp.plan.handle, None), "oct_plan_use_window")
+        for op in self.sp._ops.values():
+            op.set_window(None, self.xp.stream())
+
     def frame_files(self, frames):
         """Raw file indices of tile-local frames: the bscan_avg repeats of frame f are files
         f * bscan_avg + a (octrecon/pipeline.py:216-219)."""
@@ -152,7 +204,7 @@ class Reconstructor:
 
     # ------------------------------------------------------------------ core
     def union_rows(self):
-        """[lo, hi): depth rows read by any stitcher (the operator form computes these for every tile)."""
+        """[lo, hi): depth rows read by any stitcher."""
         rows = [st.needed_rows() for st in self.stitchers.values() if not st.empty]
         return min(r[0] for r in rows), max(r[1] for r in rows)
 
@@ -160,7 +212,6 @@ class Reconstructor:
         cfg = self.cfg
         if cfg.form not in ("plain", "mfma"):
             raise ValueError(f"form must be plain|mfma, got {cfg.form!r}")
-        ulo, uhi = self.union_rows() if cfg.form == "mfma" else (0, 0)
         frames = np.asarray(frames)
         nF = len(frames)
         B = max(1, int(cfg.batch_frames))
@@ -169,12 +220,8 @@ class Reconstructor:
             st = self.stitchers[(xi, zi)]
             if st.empty:
                 continue
-            if cfg.form == "mfma":
-                lo, hi = ulo, uhi
-            elif cfg.spectral_rows == "needed":
-                lo, hi = st.needed_rows()
-            else:
-                lo, hi = 0, self.sp.nz
+            lo, hi = self._tile_rows(st)
+            self._use_tile_window((xi, yi, zi), lo, hi)
             for s in range(0, nF, B):
                 if cancel is not None and cancel.is_set():
                     raise Cancelled()
@@ -187,8 +234,9 @@ class Reconstructor:
 
     # ------------------------------------------------------------------ fused row path
     def _tile_rows(self, st):
-        if self.cfg.form == "mfma":
-            return self.union_rows()
+        """Depth rows computed for a tile: the rows its stitcher reads ("needed"), or all.  The
+        operator form keeps one operator per distinct row window (tiles of one focus depth
+        share it)."""
         if self.cfg.spectral_rows == "needed":
             return st.needed_rows()
         return 0, self.sp.nz
@@ -208,6 +256,7 @@ class Reconstructor:
             key = (xi, zi)
             buf = self._tile_mag.get(key)
             nX = None
+            self._use_tile_window((xi, yi, zi), lo, hi)
             for s in range(0, nF, B):
                 if cancel is not None and cancel.is_set():
                     raise Cancelled()
@@ -225,8 +274,6 @@ class Reconstructor:
     def _stitch_row(self, tiles, frames, db, plane0, partial):
         """One fused stitch + finalise launch for a row (oct_stitch_row)."""
         _, nz_out, nx_out = self.n_out
-        if len(tiles) > 40:
-            raise ValueError("more than 40 tiles in one y row")
         arr = (_lib.RowTile * max(1, len(tiles)))()
         nX = None
         for i, (st, buf, lo) in enumerate(tiles):
@@ -257,10 +304,19 @@ class Reconstructor:
                 raise ValueError("frame index outside the optical-path correction grid")
             shift, vlo, vhi = st0.shift.data_ptr(), st0.vlo.data_ptr(), st0.vhi.data_ptr()
         thr = float(np.float32(min_weight_threshold(np.float32)))
-        _lib.check(_lib.lib().oct_stitch_row(arr, len(tiles), len(frames), nX or 1, self.sp.nz, fr.data_ptr(),
-                                             shift, vlo, vhi, nz_out, nx_out, thr, db.data_ptr(), plane0,
-                                             partial.data_ptr(), self.xp.stream()), "oct_stitch_row")
-        self._keep = (arr, fr)
+        table = None
+        if len(tiles) > _ROW_ARG_TILES:              # descriptors through a device table
+            host = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+            table = self.torch.from_numpy(host).to(self.xp.device)
+            _lib.check(_lib.lib().oct_stitch_row_table(arr, table.data_ptr(), len(tiles), len(frames), nX or 1,
+                                                       self.sp.nz, fr.data_ptr(), shift, vlo, vhi, nz_out, nx_out,
+                                                       thr, db.data_ptr(), plane0, partial.data_ptr(),
+                                                       self.xp.stream()), "oct_stitch_row_table")
+        else:
+            _lib.check(_lib.lib().oct_stitch_row(arr, len(tiles), len(frames), nX or 1, self.sp.nz, fr.data_ptr(),
+                                                 shift, vlo, vhi, nz_out, nx_out, thr, db.data_ptr(), plane0,
+                                                 partial.data_ptr(), self.xp.stream()), "oct_stitch_row")
+        self._keep = (arr, fr, table)
 
     def _row_partials(self, n_planes):
         _, nz_out, nx_out = self.n_out
@@ -282,17 +338,20 @@ class Reconstructor:
         Returns (db, plane_clim) device tensors: (nF, nOutZ, nOutX) float32 and (nF, 2)."""
         frames = np.arange(self.scan.n_y_px) if frames is None else np.asarray(frames)
         _, nz_out, nx_out = self.n_out
-        if self.cfg.fused_rows:
-            db = self.xp.empty((len(frames), nz_out, nx_out))
-            partial, nb = self._row_partials(len(frames))
-            self._stitch_row(self._row_magnitudes(yi, frames, cancel), frames, db, 0, partial)
-            pcl, _ = self._clim(partial, len(frames), nb)
+        try:
+            if self.cfg.fused_rows:
+                db = self.xp.empty((len(frames), nz_out, nx_out))
+                partial, nb = self._row_partials(len(frames))
+                self._stitch_row(self._row_magnitudes(yi, frames, cancel), frames, db, 0, partial)
+                pcl, _ = self._clim(partial, len(frames), nb)
+                return db, pcl
+            num = self.xp.zeros((len(frames), nz_out, nx_out))
+            den = self.xp.zeros((len(frames), nz_out, nx_out))
+            self._accumulate_row(yi, frames, num, den, cancel=cancel)
+            db, pcl, _ = self.fin.finalize(num, den)
             return db, pcl
-        num = self.xp.zeros((len(frames), nz_out, nx_out))
-        den = self.xp.zeros((len(frames), nz_out, nx_out))
-        self._accumulate_row(yi, frames, num, den, cancel=cancel)
-        db, pcl, _ = self.fin.finalize(num, den)
-        return db, pcl
+        finally:
+            self._restore_windows()
 
     def process_planes(self, y_indices0):
         """Arbitrary output y planes (0-based) -> host float32 (n, nOutZ, nOutX)."""
@@ -322,20 +381,23 @@ class Reconstructor:
         """All requested y-tile rows into one device volume: dB, plane clims, global clim and
         uint16 codes (single finalise and quantise passes).  No host synchronisation."""
         ny = self.scan.n_y_px
-        if self.cfg.fused_rows:
-            n_planes = len(vol["rows"]) * ny
-            partial, nb = self._row_partials(n_planes)
-            frames = np.arange(ny)
-            for k, r in enumerate(vol["rows"]):
-                self._stitch_row(self._row_magnitudes(r, frames, cancel), frames, vol["db"], k * ny, partial)
-            pcl, cl = self._clim(partial, n_planes, nb)
-            db = vol["db"]
-        else:
-            vol["num"].zero_()
-            vol["den"].zero_()
-            for k, r in enumerate(vol["rows"]):
-                self._accumulate_row(r, np.arange(ny), vol["num"], vol["den"], b_off=k * ny, cancel=cancel)
-            db, pcl, cl = self.fin.finalize(vol["num"], vol["den"], db=vol["db"])
+        try:
+            if self.cfg.fused_rows:
+                n_planes = len(vol["rows"]) * ny
+                partial, nb = self._row_partials(n_planes)
+                frames = np.arange(ny)
+                for k, r in enumerate(vol["rows"]):
+                    self._stitch_row(self._row_magnitudes(r, frames, cancel), frames, vol["db"], k * ny, partial)
+                pcl, cl = self._clim(partial, n_planes, nb)
+                db = vol["db"]
+            else:
+                vol["num"].zero_()
+                vol["den"].zero_()
+                for k, r in enumerate(vol["rows"]):
+                    self._accumulate_row(r, np.arange(ny), vol["num"], vol["den"], b_off=k * ny, cancel=cancel)
+                db, pcl, cl = self.fin.finalize(vol["num"], vol["den"], db=vol["db"])
+        finally:
+            self._restore_windows()
         self.fin.quantize(db, cl, pcl, legacy_double=self.cfg.legacy_double_quantization, out=vol["bits"])
         vol["plane_clim"], vol["clim"] = pcl, cl
         return vol

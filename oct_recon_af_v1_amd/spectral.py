@@ -84,6 +84,16 @@ class OperatorPlan:
         self.handle = h
         self.scale_log2 = sa.value
 
+    def max_window_abs(self) -> float:
+        v = ctypes.c_double(0.0)
+        _lib.check(_lib.lib().oct_opplan_scale_bound(self.handle, ctypes.byref(v)), "oct_opplan_scale_bound")
+        return v.value
+
+    def set_window(self, window_ptr, stream):
+        """Rebuild the operator on `stream` for the (N, 2) float64 device window at window_ptr
+        (None: the creation window); |window| must stay within max_window_abs()."""
+        _lib.check(_lib.lib().oct_opplan_set_window(self.handle, window_ptr, stream), "oct_opplan_set_window")
+
     def operator_fp64(self):
         """Complex float64 operator (nzc, N) as built on the device (when requested)."""
         return None if self.op64 is None else self.op64[..., 0] + 1j * self.op64[..., 1]

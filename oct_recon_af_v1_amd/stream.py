@@ -1,0 +1,359 @@
+"""Streamed tiled volume: tiles that are not in device memory when the job starts.
+
+One process drives one GPU with three threads:
+  producer  the host generator (liboct_hostgen, OpenMP, GIL released) fills a ring of pinned
+            host tiles;
+  copier    each pinned tile goes to the volume's resident raw array in HBM with an
+            asynchronous copy on its own stream; the slot is handed back when the copy is done;
+  main      as soon as a y-tile row has landed, the dispersion and focus of each of its tiles
+            come from the learned estimator on `est_bscans` B-scans per tile (the sharpness
+            search and confocal detector for the B-scans it is not confident about); after the
+            last row: focus per depth (median of its tiles), every tile reconstructed at its own
+            dispersion (MFMA operator rebuilt on the device per tile) and stitched into one
+            volume with one clim and one quantisation pass.
+
+The reference reads raw files with io_threads reader threads and a bounded read-ahead
+(octrecon/pipeline.py:197-212, 240-288), copies each batch with a pageable default-stream copy
+and runs spectral + stitching per batch; dispersion and focus are measured before the run
+(octrecon/estimation/dispersion.py:154-267, octrecon/estimation/focus.py:221-279) and one
+dispersion is applied to the whole volume.  The whole raw volume stays resident here (a 12 x 9
+x 8 tile volume of 40 x 1000 A-scans is 145 GB of uint16, half of an MI355X's HBM), so the
+reconstruction after estimation reads HBM, not the host link.
+
+    python -m oct_recon_af_v1_amd.stream [--tiles-x 12 --tiles-y 9 --depths 8 --bscans 40 --ascans 1000]
+"""
+from __future__ import annotations
+
+import json
+import queue
+import threading
+import time
+from dataclasses import asdict, dataclass
+
+import numpy as np
+
+from . import synthetic as SY
+from .geometry import ScanGrid, beta_from_index_units, build_spectral_geometry
+from .quantize import DeviceFinalizer
+
+
+@dataclass
+class StreamSpec:
+    tiles_x: int = 12
+    tiles_y: int = 9
+    depths: int = 8
+    n_y: int = 40                   # B-scans per tile
+    n_x: int = 1000                 # A-scans per B-scan
+    overlap: float = 0.1            # lateral (x) overlap of neighbouring tiles
+    depth_step_mm: float = 0.05     # focus planes
+    beta_idx: float = 1e-4          # mean dispersion (phase per squared equispaced-k index)
+    beta_spread: float = 0.2        # tile dispersion = beta_idx (1 + spread u), u ~ U(-1, 1)
+    focus0: float = 420.0           # focus depth bin (0-based) of the first focus plane
+    focus_drift: float = -12.0      # focus bins per focus plane
+    focus_jitter: float = 4.0       # per tile, uniform +-
+    focus_sigma_um: float = 30.0
+    crop_z_mm: tuple = (-0.05, 0.4)
+    seed: int = 1
+
+    @property
+    def n_tiles(self) -> int:
+        return self.tiles_x * self.tiles_y * self.depths
+
+
+class StreamedVolume:
+    def __init__(self, xp, ss: StreamSpec = StreamSpec(), spec: SY.SpectrumSpec = SY.SpectrumSpec(),
+                 fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 4,
+                 gen_threads: int | None = None, estimator: str = "learned", est_bscans: int = 8,
+                 form: str = "mfma", log=None):
+        if estimator not in ("learned", "given"):
+            raise ValueError(f"estimator must be learned|given, got {estimator!r}")
+        if ring < 2:
+            raise ValueError("the ring needs at least 2 pinned tiles")
+        self.xp, self.torch = xp, xp.torch
+        self.ss, self.spec = ss, spec
+        self.form = form
+        self.log = log or (lambda *a: None)
+        self.lam = spec.lambda_nm()
+        self.to_ref = beta_from_index_units(1.0, self.lam)
+        px_mm = 1e-3
+        tx, ty = ss.n_x * px_mm, ss.n_y * px_mm
+        xc = (np.arange(ss.tiles_x) - (ss.tiles_x - 1) / 2) * tx * (1 - ss.overlap)
+        yc = (np.arange(ss.tiles_y) - (ss.tiles_y - 1) / 2) * ty
+        zd = np.arange(ss.depths) * ss.depth_step_mm
+        self.grid = ScanGrid(xc, yc, zd, tx, ty, ss.n_x, ss.n_y, pixel_size_um=1.0)
+        self.keys = [(xi, yi, zi) for yi in range(ss.tiles_y) for (xi, zi) in self.grid.tiles_in_row(yi)]
+        rng = np.random.default_rng(ss.seed)
+        n = len(self.keys)
+        beta = ss.beta_idx * (1 + ss.beta_spread * rng.uniform(-1, 1, n))
+        focus = np.array([ss.focus0 + ss.focus_drift * k[2] for k in self.keys]) + rng.uniform(-1, 1, n) * ss.focus_jitter
+        seeds = rng.integers(1, 2 ** 62, n)
+        self.truth = {"beta_idx": beta, "focus": focus, "seed": seeds}
+        self.gen = SY.HostTileGenerator(spec, fspec, hspec, threads=gen_threads)
+        self.lines = spec.apod + ss.n_x
+        self.n = spec.n
+        torch = self.torch
+        self.raw = torch.empty((n, ss.n_y, self.lines, self.n), dtype=torch.uint16, device=xp.device)
+        self.ring = [torch.empty((ss.n_y, self.lines, self.n), dtype=torch.uint16, pin_memory=True)
+                     for _ in range(ring)]
+        self.estimator = estimator
+        self.est_bscans = int(est_bscans)
+        self.est = None
+        self.fallback = None
+        if estimator == "learned":
+            from .estimation import HipDispersionScorer
+            from .learned import LearnedEstimator
+            self.est = LearnedEstimator(xp, self.lam, spec.apod)
+            self.est_x = 256                            # the network's crop width
+            if ss.n_x < self.est_x:
+                raise ValueError(f"the estimator needs B-scans of at least {self.est_x} A-scans")
+            self.fallback = HipDispersionScorer(self.lam, self.raw[0, :1, :spec.apod + self.est_x], spec.apod, xp)
+        self._est_buf = None
+        self._err = None
+
+    # ------------------------------------------------------------------ threads
+    def _guard(self, fn, *args):
+        try:
+            fn(*args)
+        except BaseException as e:  # noqa: BLE001 - re-raised on the main thread
+            self._err = e
+            for q in args:
+                if isinstance(q, queue.Queue):
+                    q.put(None)
+
+    def _produce(self, free_q, ready_q, stats):
+        for i in range(len(self.keys)):
+            slot = free_q.get()
+            if slot is None or self._err is not None:
+                break
+            t0 = time.perf_counter()
+            self.gen.generate(self.ring[slot], self.ss.n_x, self.truth["beta_idx"][i], self.truth["focus"][i],
+                              int(self.truth["seed"][i]))
+            stats["gen_s"] += time.perf_counter() - t0
+            ready_q.put((i, slot))
+        ready_q.put(None)
+
+    def _copy(self, ready_q, free_q, landed_q, stats):
+        torch = self.torch
+        stream = torch.cuda.Stream(device=self.xp.device)
+        with torch.cuda.stream(stream):
+            while True:
+                item = ready_q.get()
+                if item is None or self._err is not None:
+                    landed_q.put(None)
+                    return
+                i, slot = item
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                self.raw[i].copy_(self.ring[slot], non_blocking=True)
+                e1.record(stream)
+                while not e1.query():                 # poll: the GIL stays free for the others
+                    time.sleep(1e-4)
+                stats["h2d_s"] += e0.elapsed_time(e1) / 1e3
+                free_q.put(slot)
+                landed_q.put(i)
+
+    # ------------------------------------------------------------------ estimation
+    def _gather(self, idx):
+        """The estimator's B-scans of tiles `idx`: est_bscans frames spread over each tile, the
+        background lines and the central est_x scan lines -> (len(idx) E, apod + est_x, N)."""
+        torch = self.torch
+        E, a, w = self.est_bscans, self.spec.apod, self.est_x
+        shape = (len(idx) * E, a + w, self.n)
+        if self._est_buf is None or tuple(self._est_buf.shape) != shape:
+            self._est_buf = torch.empty(shape, dtype=torch.uint16, device=self.xp.device)
+        buf = self._est_buf.view(torch.int16)             # uint16 bits moved as int16
+        rv = self.raw.view(torch.int16)
+        fr = torch.as_tensor(np.round(np.linspace(0, self.ss.n_y - 1, E)).astype(np.int64), device=self.xp.device)
+        x0 = a + (self.ss.n_x - w) // 2
+        for k, i in enumerate(idx):
+            buf[k * E:(k + 1) * E, :a] = rv[i, :, :a].index_select(0, fr)
+            buf[k * E:(k + 1) * E, a:] = rv[i, :, x0:x0 + w].index_select(0, fr)
+        return self._est_buf
+
+    def _estimate(self, idx, stats):
+        """Per tile of `idx`: median dispersion (index units) and focus over its B-scans, and the
+        number of B-scans that fell back to the search."""
+        from .learned import make_labels, take_bscans
+        E = self.est_bscans
+        buf = self._gather(idx)
+        t0 = time.perf_counter()
+        p = self.est.predict(buf, graph=True)
+        t1 = time.perf_counter()
+        beta, focus = p["beta_idx"].copy(), p["focus"].copy()
+        bad = np.nonzero(~p["confident"])[0]
+        if bad.size:
+            self.fallback.set_raw(take_bscans(buf, bad) if bad.size < buf.shape[0] else buf)
+            lab = make_labels(self.fallback)
+            beta[bad] = lab["beta_idx"]
+            focus[bad] = np.where(np.isfinite(lab["focus"]), lab["focus"], focus[bad])
+        stats["net_s"] += t1 - t0
+        stats["fallback_s"] += time.perf_counter() - t1
+        n = len(idx)
+        nb = np.bincount(bad // E, minlength=n) if bad.size else np.zeros(n, int)
+        return np.median(beta.reshape(n, E), 1), np.median(focus.reshape(n, E), 1), nb
+
+    def warmup(self):
+        """Capture the estimator's graph for the row batch shape (outside any timed region)."""
+        if self.est is None:
+            return
+        per_row = len(self.grid.tiles_in_row(0))
+        self.gen.generate(self.ring[0], self.ss.n_x, self.truth["beta_idx"][0], self.truth["focus"][0],
+                          int(self.truth["seed"][0]))
+        self.raw[0].copy_(self.ring[0])
+        self.est.predict(self._gather([0] * per_row), graph=True)
+        self.xp.synchronize()
+
+    # ------------------------------------------------------------------ reconstruction
+    def reconstructor(self, tile_beta_idx, focus_per_depth, form=None):
+        from .pipeline import ReconConfig, Reconstructor
+        ss = self.ss
+        b0 = beta_from_index_units(ss.beta_idx, self.lam)
+        sg = build_spectral_geometry(self.lam, b0, 1.33)
+        dz_um = float(sg.z_um[1] - sg.z_um[0])
+        cfg = ReconConfig(dispersion_quadratic_term=b0, focus_positions=[float(f) for f in focus_per_depth],
+                          focus_sigma=ss.focus_sigma_um / dz_um, crop_z_range_mm=list(ss.crop_z_mm),
+                          output_pixel_size_um=1.0, n_medium=1.33, batch_frames=ss.n_y, form=form or self.form)
+        tiles = {k: self.raw[i] for i, k in enumerate(self.keys)}
+        betas = {k: float(tile_beta_idx[i]) * self.to_ref for i, k in enumerate(self.keys)}
+        return Reconstructor(cfg, self.grid, self.lam, tiles, self.spec.apod, xp=self.xp, tile_betas=betas)
+
+    def focus_per_depth(self, tile_focus):
+        """1-based focus pixel per focus plane: median over the plane's tiles (0-based bins + 1)."""
+        z = np.array([k[2] for k in self.keys])
+        return [float(np.median(tile_focus[z == d])) + 1.0 for d in range(self.ss.depths)]
+
+    # ------------------------------------------------------------------ run
+    def run(self, validate: bool = True):
+        """Stream, estimate and reconstruct the volume; returns measurements (and the volume)."""
+        torch = self.torch
+        ss = self.ss
+        n = len(self.keys)
+        stats = {"gen_s": 0.0, "h2d_s": 0.0, "net_s": 0.0, "fallback_s": 0.0}
+        free_q, ready_q, landed_q = queue.Queue(), queue.Queue(), queue.Queue()
+        for s in range(len(self.ring)):
+            free_q.put(s)
+        rows = {}
+        for i, k in enumerate(self.keys):
+            rows.setdefault(k[1], []).append(i)
+        tile_beta = np.full(n, ss.beta_idx)
+        tile_focus = np.full(n, np.nan)
+        tile_bad = np.zeros(n, int)
+        self.xp.synchronize()
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=self._guard, args=(self._produce, free_q, ready_q, stats), daemon=True),
+                   threading.Thread(target=self._guard, args=(self._copy, ready_q, free_q, landed_q, stats),
+                                    daemon=True)]
+        for t in threads:
+            t.start()
+        landed = np.zeros(n, bool)
+        done_rows = set()
+        t_est = 0.0
+        try:
+            while True:
+                i = landed_q.get()
+                if i is None:
+                    break
+                landed[i] = True
+                yi = self.keys[i][1]
+                if yi not in done_rows and landed[rows[yi]].all():
+                    done_rows.add(yi)
+                    te = time.perf_counter()
+                    idx = rows[yi]
+                    if self.est is not None:
+                        b, f, nb = self._estimate(idx, stats)
+                    else:
+                        b, f, nb = self.truth["beta_idx"][idx], self.truth["focus"][idx], np.zeros(len(idx), int)
+                    tile_beta[idx], tile_focus[idx], tile_bad[idx] = b, f, nb
+                    t_est += time.perf_counter() - te
+        finally:
+            free_q.put(None)
+            for t in threads:
+                t.join(timeout=60)
+        if self._err is not None:
+            raise RuntimeError("streaming thread failed") from self._err
+        if not landed.all():
+            raise RuntimeError(f"{int((~landed).sum())} tiles never landed")
+        t_stream = time.perf_counter()
+        fpd = self.focus_per_depth(tile_focus)
+        rec = self.reconstructor(tile_beta, fpd)
+        vol = rec.reconstruct_volume(rec.allocate_volume())
+        self.xp.synchronize()
+        t1 = time.perf_counter()
+        total = n * ss.n_y * ss.n_x
+        raw_bytes = float(self.raw.numel() * 2)
+        out = {
+            "ascans": total, "tiles": n, "raw_GB": raw_bytes / 1e9,
+            "seconds": t1 - t0, "ascans_per_s": total / (t1 - t0),
+            "stream_seconds": t_stream - t0, "reconstruct_seconds": t1 - t_stream,
+            "reconstruct_ascans_per_s": total / (t1 - t_stream),
+            "host_gen_ascans_per_s": total / stats["gen_s"] * 1.0 if stats["gen_s"] else None,
+            "host_gen_threads": self.gen.threads,
+            "h2d_GB_per_s": raw_bytes / stats["h2d_s"] / 1e9 if stats["h2d_s"] else None,
+            "estimation_seconds": t_est, "network_seconds": stats["net_s"], "fallback_seconds": stats["fallback_s"],
+            "fallback_rate_bscans": float(tile_bad.sum() / (n * self.est_bscans)) if self.est is not None else 0.0,
+            "fallback_rate_tiles": float((tile_bad > 0).mean()) if self.est is not None else 0.0,
+            "beta_err_idx_median": float(np.median(np.abs(tile_beta - self.truth["beta_idx"]))),
+            "focus_err_bins_median": float(np.median(np.abs(tile_focus - self.truth["focus"]))),
+            "focus_per_depth": fpd, "output_shape_yzx": list(vol["db"].shape), "form": self.form,
+        }
+        if validate:
+            ref = self.reconstructor(tile_beta, fpd, form="plain")
+            vref = ref.reconstruct_volume(ref.allocate_volume())
+            frac, mx = lsb_agreement(self.xp, vol["db"], vref["db"], vref["clim"])
+            out["within_1lsb_of_plain"] = frac
+            out["max_lsb_vs_plain"] = mx
+            out["plain_clim_dB"] = [float(v) for v in vref["clim"].cpu().numpy()]
+            del vref, ref
+        out["volume"] = vol
+        return out
+
+
+def lsb_agreement(xp, db, db_ref, clim_ref, planes: int = 16):
+    """Fraction of voxels whose uint16 codes (quantised with the reference volume's clim) are
+    within 1 LSB of the reference volume's, and the largest difference; on the device."""
+    torch = xp.torch
+    fin = DeviceFinalizer(xp)
+    ok = 0
+    mx = 0
+    for p0 in range(0, db.shape[0], planes):
+        a = fin.quantize(db[p0:p0 + planes], clim_ref).view(torch.int16).to(torch.int32) & 0xFFFF
+        b = fin.quantize(db_ref[p0:p0 + planes], clim_ref).view(torch.int16).to(torch.int32) & 0xFFFF
+        d = (a - b).abs_()
+        ok += int((d <= 1).sum())
+        mx = max(mx, int(d.max()))
+    return ok / db.numel(), mx
+
+
+def main(argv=None):
+    import argparse
+
+    from .backend import get_xp
+    ap = argparse.ArgumentParser(description="streamed tiled volume: host generation, pinned async copies, "
+                                             "per-tile estimation, MFMA reconstruction, stitching")
+    ap.add_argument("--tiles-x", type=int, default=12)
+    ap.add_argument("--tiles-y", type=int, default=9)
+    ap.add_argument("--depths", type=int, default=8)
+    ap.add_argument("--bscans", type=int, default=40)
+    ap.add_argument("--ascans", type=int, default=1000)
+    ap.add_argument("--ring", type=int, default=4)
+    ap.add_argument("--gen-threads", type=int, default=None)
+    ap.add_argument("--estimator", default="learned", choices=["learned", "given"])
+    ap.add_argument("--form", default="mfma", choices=["mfma", "plain"])
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    a = ap.parse_args(argv)
+    xp = get_xp("hip")[0]
+    ss = StreamSpec(tiles_x=a.tiles_x, tiles_y=a.tiles_y, depths=a.depths, n_y=a.bscans, n_x=a.ascans, seed=a.seed)
+    sv = StreamedVolume(xp, ss, ring=a.ring, gen_threads=a.gen_threads, estimator=a.estimator, form=a.form,
+                        log=lambda *m: print(*m, flush=True))
+    print(f"streamed volume: {ss.n_tiles} tiles, {sv.raw.numel() * 2 / 1e9:.1f} GB raw resident", flush=True)
+    sv.warmup()
+    r = sv.run(validate=not a.no_validate)
+    r.pop("volume")
+    r["spec"] = asdict(ss)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -178,3 +178,78 @@ def make_focus_bscans(n_files: int, n_x: int, betas, focus_bins, spec: SpectrumS
         blk = torch.cat([bg, sig], dim=1).clamp_(0, 4095)
         out[f0:f0 + nf] = blk.to(torch.int32).to(torch.uint16)
     return out
+
+
+# --------------------------------------------------------------------------- streamed-volume workload
+@dataclass
+class HostTileSpec:
+    """Bank sizes and terms of the host tile generator (csrc/hostgen.c): per tile, km medium
+    speckle spectra and kr reflector fringes; each A-scan combines n_med of the former (complex
+    Gaussian weights) and n_ref of the latter (log-uniform reflectivities)."""
+    km: int = 8
+    kr: int = 64
+    n_med: int = 2
+    n_ref: int = 5
+    noise: bool = True
+
+
+_HOSTGEN = None
+
+
+def _hostgen():
+    global _HOSTGEN
+    if _HOSTGEN is None:
+        import ctypes
+        from pathlib import Path
+        path = Path(__file__).resolve().parent / "liboct_hostgen.so"
+        if not path.exists():
+            raise RuntimeError(f"{path.name} is not built (python -m oct_recon_af_v1_amd.build)")
+        L = ctypes.CDLL(str(path))
+        L.octg_params_count.restype = ctypes.c_int
+        L.octg_focus_tile.restype = ctypes.c_int
+        L.octg_focus_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int]
+        _HOSTGEN = L
+    return _HOSTGEN
+
+
+class HostTileGenerator:
+    """Tiles of the streamed volume generated by host threads (uint16 (n_files, apod + n_x, N)),
+    written into caller memory (a pinned host tensor).  The ctypes call releases the GIL, so a
+    producer thread generates while the main thread drives the device."""
+
+    def __init__(self, spec: SpectrumSpec = SpectrumSpec(), fspec: FocusSpec = FocusSpec(),
+                 hspec: HostTileSpec = HostTileSpec(), threads: int | None = None):
+        import os
+        lam = spec.lambda_nm()
+        kn, k0n = _k_axes(lam)
+        self.spec, self.fspec, self.hspec = spec, fspec, hspec
+        self.kn = np.ascontiguousarray(kn, np.float64)
+        self.dq = np.ascontiguousarray((kn - k0n) ** 2, np.float64)
+        self.src = np.ascontiguousarray(
+            spec.peak * np.exp(-4 * np.log(2) * (lam - spec.center_nm) ** 2 / spec.fwhm_nm ** 2), np.float64)
+        self.params = np.array([spec.r_min, spec.r_max, spec.z_lo, spec.z_hi, fspec.scat_r, fspec.att_bins,
+                                fspec.surf_lo, fspec.surf_hi, fspec.gate_floor, fspec.gate_width_bins,
+                                fspec.n_depth, hspec.km, hspec.kr, hspec.n_med, hspec.n_ref,
+                                1.0 if hspec.noise else 0.0], np.float64)
+        L = _hostgen()
+        if L.octg_params_count() != len(self.params):
+            raise RuntimeError("liboct_hostgen.so does not match this module (rebuild)")
+        self.threads = int(threads or min(16, os.cpu_count() or 1))
+
+    def generate(self, out, n_x: int, beta_idx: float, focus_bins: float, seed: int):
+        """Fill out (n_files, apod + n_x, N) uint16 (numpy array or CPU torch tensor)."""
+        ptr = out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data
+        shape = tuple(out.shape)
+        if len(shape) != 3 or shape[1] != self.spec.apod + n_x or shape[2] != self.spec.n:
+            raise ValueError(f"out must be (files, {self.spec.apod + n_x}, {self.spec.n}), got {shape}")
+        if hasattr(out, "is_contiguous") and not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        rc = _hostgen().octg_focus_tile(ptr, shape[0], n_x, self.spec.apod, self.spec.n, self.kn.ctypes.data,
+                                        self.dq.ctypes.data, self.src.ctypes.data, float(beta_idx),
+                                        float(focus_bins), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                        self.params.ctypes.data, self.threads)
+        if rc != 0:
+            raise RuntimeError("octg_focus_tile failed (arguments or memory)")
+        return out

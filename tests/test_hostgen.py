@@ -1,0 +1,72 @@
+"""The host tile generator of the streamed volume (csrc/hostgen.c via synthetic.HostTileGenerator):
+determinism, thread independence, shot-noise statistics, and the fringe it writes (depth,
+amplitude and dispersion sign) recovered by a direct non-uniform DFT.  CPU only."""
+import numpy as np
+import pytest
+
+from oct_recon_af_v1_amd import synthetic as SY
+
+
+def _gen(threads=4, **h):
+    return SY.HostTileGenerator(hspec=SY.HostTileSpec(**h), threads=threads)
+
+
+def test_deterministic_and_thread_independent():
+    out = [np.empty((3, 25 + 64, 2048), np.uint16) for _ in range(3)]
+    _gen(1).generate(out[0], 64, 1e-4, 300.0, 11)
+    _gen(4).generate(out[1], 64, 1e-4, 300.0, 11)
+    _gen(4).generate(out[2], 64, 1e-4, 300.0, 12)
+    assert np.array_equal(out[0], out[1])
+    assert not np.array_equal(out[0], out[2])
+    assert out[0].max() <= 4095
+
+
+def test_background_lines_shot_noise():
+    g = _gen()
+    out = np.empty((16, 25 + 8, 2048), np.uint16)
+    g.generate(out, 8, 0.0, 300.0, 5)
+    bg = out[:, :25].astype(float).reshape(-1, 2048)
+    src = g.src
+    j = slice(300, 1700)                                    # away from the low-count edges
+    assert np.abs(bg.mean(0)[j] - src[j]).max() < 6 * np.sqrt(src[j].max() / bg.shape[0])
+    ratio = bg.var(0)[j] / src[j]
+    assert 0.9 < ratio.mean() < 1.1                          # Poisson: variance = mean
+
+
+@pytest.mark.parametrize("beta", [0.0, 1.5e-4])
+def test_single_reflector_fringe(beta):
+    """No medium, one reflector per A-scan, no noise: (counts / S - 1) is a g cos(2 pi z kn / N +
+    phi + beta dq); its non-uniform DFT peaks at one depth in [z_lo, z_hi) with height a g N / 2,
+    and removing the dispersion phase with the generator's sign sharpens it."""
+    g = SY.HostTileGenerator(fspec=SY.FocusSpec(scat_r=0.0), hspec=SY.HostTileSpec(km=1, kr=1, n_med=0, n_ref=1,
+                                                                                  noise=False), threads=2)
+    out = np.empty((1, 25 + 2, 2048), np.uint16)
+    g.generate(out, 2, beta, 500.0, 3)
+    j = slice(200, 1848)                                    # counts high enough for 1-count rounding
+    x = out[0, 25].astype(float)[j] / g.src[j] - 1.0
+    kn, dq = g.kn[j], g.dq[j]
+    z = np.arange(10.0, 1010.0, 0.25)
+
+    def spectrum(b):
+        ph = 2 * np.pi * np.outer(z, kn) / 2048 + b * dq
+        return np.abs(np.exp(-1j * ph) @ x)
+
+    s_right, s_zero = spectrum(beta), spectrum(0.0)
+    zpk = z[np.argmax(s_right)]
+    spec = SY.SpectrumSpec()
+    assert spec.z_lo - 1 <= zpk <= spec.z_hi + 1
+    if beta:
+        assert s_right.max() > 2 * s_zero.max()
+    # amplitude: a * sqrt(gate) within the reflectivity and gate ranges
+    amp = 2 * s_right.max() / x.size
+    lo = np.sqrt(spec.r_min) * np.sqrt(SY.FocusSpec().gate_floor)
+    hi = np.sqrt(spec.r_max) * np.sqrt(1 + SY.FocusSpec().gate_floor)
+    assert lo * 0.9 < amp < hi * 1.1
+    x2 = out[0, 26].astype(float)[j] / g.src[j] - 1.0        # one bank entry: the same fringe,
+    assert np.corrcoef(x, x2)[0, 1] > 0.999                   # another reflectivity
+
+
+def test_rejects_bad_arguments():
+    g = _gen()
+    with pytest.raises(ValueError):
+        g.generate(np.empty((1, 10, 2048), np.uint16), 64, 0.0, 300.0, 1)
