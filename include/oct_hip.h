@@ -161,6 +161,13 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
  * in order.
  * oct_average_repeats: mag (n_frames bscan_avg, size_x ascan_avg, rows) -> out (n_frames,
  * size_x, rows): mean over the B-scan repeats, then over the A-scan repeats (float32). */
+/* ---- raw-file ingest: Thorlabs SRR payload (n_files, file_lines, n_lambda) uint16 -> Thorlabs file
+ * layout (n_files, (apod_end - apod_start) + (scan_end - scan_start), n_lambda) int16: the low 12
+ * bits of each sample (legacy rem 4096), background A-lines [apod_start, apod_end) first, then the
+ * scan A-lines [scan_start, scan_end) (octrecon/io/srr.py:107-124, SRRTileReader.read_file /
+ * read_bscans).  n_lambda a multiple of 8, buffers 16-byte aligned. */
+int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda, int apod_start, int apod_end,
+                   int scan_start, int scan_end, int16_t* out, void* stream);
 int oct_binned_lines(int n_scan_lines, int binning);
 int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int keep_lines, int skip_lines,
                   int binning, float* out, void* stream);

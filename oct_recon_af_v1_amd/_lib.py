@@ -59,6 +59,7 @@ SIGNATURES = {
     "oct_opplan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), _vp]),
     "oct_opplan_destroy": (_i, [_vp]),
     "oct_opplan_set_window": (_i, [_vp, _vp, _vp]),
+    "oct_srr_unpack": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp]),
     "oct_opplan_scale_bound": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_plan_use_window": (_i, [_vp, _vp]),

@@ -12,6 +12,7 @@
 #include "opform.hpp"
 #include "estimate.hpp"
 #include "cnn.hpp"
+#include "ingest.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -249,6 +250,26 @@ int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int ke
                                out, n_files, lines, keep_lines, skip_lines, n_scan, binning, n_kept, oct::kN);
             break;
     }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda, int apod_start, int apod_end,
+                   int scan_start, int scan_end, int16_t* out, void* stream) {
+    if (!in || !out) return fail(OCT_EINVAL, "null argument");
+    if (n_files <= 0) return OCT_OK;
+    if (n_lambda <= 0 || (n_lambda % 8)) return fail(OCT_EINVAL, "n_lambda %d must be a positive multiple of 8", n_lambda);
+    if (apod_start < 0 || apod_end <= apod_start || scan_start < 0 || scan_end <= scan_start ||
+        apod_end > file_lines || scan_end > file_lines)
+        return fail(OCT_EINVAL, "regions [%d, %d) / [%d, %d) outside %d A-lines", apod_start, apod_end, scan_start,
+                    scan_end, file_lines);
+    if (((uintptr_t)in | (uintptr_t)out) & 15) return fail(OCT_EINVAL, "buffers must be 16-byte aligned");
+    const long long L = (long long)(apod_end - apod_start) + (scan_end - scan_start);
+    const long long tot = (long long)n_files * L * (n_lambda / 8);
+    if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
+    hipLaunchKernelGGL(oct::srr_unpack_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), in, out, n_files, file_lines, n_lambda, apod_start,
+                       apod_end, scan_start, scan_end);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

@@ -64,7 +64,10 @@ class StreamedVolume:
     def __init__(self, xp, ss: StreamSpec = StreamSpec(), spec: SY.SpectrumSpec = SY.SpectrumSpec(),
                  fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 4,
                  gen_threads: int | None = None, estimator: str = "learned", est_bscans: int = 8,
-                 form: str = "mfma", log=None):
+                 form: str = "mfma", log=None, readers: dict | None = None, io_threads: int = 8):
+        """readers: optional {(xi, yi, zi): rawio.OcityTile | rawio.SrrTile} streaming the tiles
+        from their raw files (n_y files each) instead of the host generator; the generator's
+        per-tile dispersion / focus stay the 'given' estimates."""
         if estimator not in ("learned", "given"):
             raise ValueError(f"estimator must be learned|given, got {estimator!r}")
         if ring < 2:
@@ -92,9 +95,28 @@ class StreamedVolume:
         self.lines = spec.apod + ss.n_x
         self.n = spec.n
         torch = self.torch
-        self.raw = torch.empty((n, ss.n_y, self.lines, self.n), dtype=torch.uint16, device=xp.device)
-        self.ring = [torch.empty((ss.n_y, self.lines, self.n), dtype=torch.uint16, pin_memory=True)
+        self.readers = readers
+        self.io_pool = None
+        slot_lines = self.lines
+        if readers is not None:
+            from concurrent.futures import ThreadPoolExecutor
+            missing = [k for k in self.keys if k not in readers]
+            if missing:
+                raise ValueError(f"no reader for tiles {missing[:4]}")
+            for k in self.keys:
+                r = readers[k]
+                if r.interf_size != self.lines or r.n_lambda != self.n:
+                    raise ValueError(f"tile {k}: files of {r.interf_size} x {r.n_lambda}, the volume needs "
+                                     f"{self.lines} x {self.n}")
+                if r.kind == "srr":
+                    slot_lines = max(slot_lines, r.file_lines)
+            self.io_pool = ThreadPoolExecutor(max(1, int(io_threads)))
+        # files hold int16 counts (Thorlabs); the generator writes uint16
+        self.raw_dtype = torch.int16 if readers is not None else torch.uint16
+        self.raw = torch.empty((n, ss.n_y, self.lines, self.n), dtype=self.raw_dtype, device=xp.device)
+        self.ring = [torch.empty(ss.n_y * slot_lines * self.n, dtype=self.raw_dtype, pin_memory=True)
                      for _ in range(ring)]
+        self._payload = None
         self.estimator = estimator
         self.est_bscans = int(est_bscans)
         self.est = None
@@ -126,11 +148,40 @@ class StreamedVolume:
             if slot is None or self._err is not None:
                 break
             t0 = time.perf_counter()
-            self.gen.generate(self.ring[slot], self.ss.n_x, self.truth["beta_idx"][i], self.truth["focus"][i],
-                              int(self.truth["seed"][i]))
+            self._fill(i, self.ring[slot])
             stats["gen_s"] += time.perf_counter() - t0
             ready_q.put((i, slot))
         ready_q.put(None)
+
+    def _slot_view(self, i, flat):
+        """The ring slot as tile i's host layout: (n_y, lines, N), or the SRR payload."""
+        r = self.readers[self.keys[i]] if self.readers is not None else None
+        rows = r.file_lines if (r is not None and r.kind == "srr") else self.lines
+        return flat[:self.ss.n_y * rows * self.n].view(self.ss.n_y, rows, self.n)
+
+    def _fill(self, i, flat):
+        view = self._slot_view(i, flat)
+        if self.readers is None:
+            self.gen.generate(view, self.ss.n_x, self.truth["beta_idx"][i], self.truth["focus"][i],
+                              int(self.truth["seed"][i]))
+        else:
+            self.readers[self.keys[i]].read(range(self.ss.n_y), view, self.io_pool)
+
+    def _land(self, i, flat, stream):
+        """Host slot -> tile i in HBM, on `stream` (SRR payloads unpacked on the device)."""
+        torch = self.torch
+        src = self._slot_view(i, flat)
+        r = self.readers[self.keys[i]] if self.readers is not None else None
+        if r is not None and r.kind == "srr":
+            from . import _lib
+            if self._payload is None or tuple(self._payload.shape) != tuple(src.shape):
+                self._payload = torch.empty(src.shape, dtype=torch.int16, device=self.xp.device)
+            self._payload.copy_(src, non_blocking=True)
+            _lib.check(_lib.lib().oct_srr_unpack(self._payload.data_ptr(), self.ss.n_y, r.file_lines, self.n, r.apod[0],
+                                                 r.apod[1], r.scan[0], r.scan[1], self.raw[i].data_ptr(),
+                                                 stream.cuda_stream), "oct_srr_unpack")
+        else:
+            self.raw[i].copy_(src, non_blocking=True)
 
     def _copy(self, ready_q, free_q, landed_q, stats):
         torch = self.torch
@@ -145,7 +196,7 @@ class StreamedVolume:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
-                self.raw[i].copy_(self.ring[slot], non_blocking=True)
+                self._land(i, self.ring[slot], stream)
                 e1.record(stream)
                 while not e1.query():                 # poll: the GIL stays free for the others
                     time.sleep(1e-4)
@@ -161,7 +212,7 @@ class StreamedVolume:
         E, a, w = self.est_bscans, self.spec.apod, self.est_x
         shape = (len(idx) * E, a + w, self.n)
         if self._est_buf is None or tuple(self._est_buf.shape) != shape:
-            self._est_buf = torch.empty(shape, dtype=torch.uint16, device=self.xp.device)
+            self._est_buf = torch.empty(shape, dtype=self.raw_dtype, device=self.xp.device)
         buf = self._est_buf.view(torch.int16)             # uint16 bits moved as int16
         rv = self.raw.view(torch.int16)
         fr = torch.as_tensor(np.round(np.linspace(0, self.ss.n_y - 1, E)).astype(np.int64), device=self.xp.device)
@@ -198,9 +249,8 @@ class StreamedVolume:
         if self.est is None:
             return
         per_row = len(self.grid.tiles_in_row(0))
-        self.gen.generate(self.ring[0], self.ss.n_x, self.truth["beta_idx"][0], self.truth["focus"][0],
-                          int(self.truth["seed"][0]))
-        self.raw[0].copy_(self.ring[0])
+        self._fill(0, self.ring[0])
+        self._land(0, self.ring[0], self.torch.cuda.current_stream())
         self.est.predict(self._gather([0] * per_row), graph=True)
         self.xp.synchronize()
 
@@ -296,6 +346,7 @@ class StreamedVolume:
             "beta_err_idx_median": float(np.median(np.abs(tile_beta - self.truth["beta_idx"]))),
             "focus_err_bins_median": float(np.median(np.abs(tile_focus - self.truth["focus"]))),
             "focus_per_depth": fpd, "output_shape_yzx": list(vol["db"].shape), "form": self.form,
+            "source": "files" if self.readers is not None else "host generator",
         }
         if validate:
             ref = self.reconstructor(tile_beta, fpd, form="plain")

@@ -124,3 +124,33 @@ def test_row_table_path_equals_argument_path(xp, monkeypatch):
     rec = sv.reconstructor(sv.truth["beta_idx"], sv.focus_per_depth(sv.truth["focus"]))
     b = rec.reconstruct_volume(rec.allocate_volume())["db"].cpu().numpy()
     assert np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("kind", ["ocity", "srr"])
+def test_streamed_volume_from_files(xp, tmp_path, kind):
+    """The same tiles streamed from raw tile folders (read into the pinned ring, SRR payloads
+    unpacked on the device) give the generator-streamed volume bit for bit."""
+    import rawfiles
+    from oct_recon_af_v1_amd import rawio
+    from oct_recon_af_v1_amd.stream import StreamedVolume
+    _say(f"case: streamed volume from {kind} tile folders")
+    ss = _small()
+    gen = StreamedVolume(xp, ss, ring=2, gen_threads=8, estimator="given")
+    ref = gen.run(validate=False)["volume"]["db"].cpu().numpy()
+    rng = np.random.default_rng(9)
+    readers = {}
+    for i, key in enumerate(gen.keys):
+        raw = gen.raw[i].view(xp.torch.int16).cpu().numpy()
+        folder = tmp_path / f"t{i}"
+        if kind == "ocity":
+            rawfiles.write_ocity(folder, raw)
+            readers[key] = rawio.OcityTile(folder, raw.shape[1], raw.shape[2])
+        else:
+            lines = raw.view(np.uint16) | (rng.integers(0, 16, size=raw.shape).astype(np.uint16) << 12)
+            rawfiles.write_srr(folder, lines, y_total=ss.n_y, b_total=1, apod=(0, 25), scan=(25, raw.shape[1]))
+            readers[key] = rawio.SrrTile(folder)
+    del gen
+    sv = StreamedVolume(xp, ss, ring=2, gen_threads=8, estimator="given", readers=readers, io_threads=4)
+    r = sv.run(validate=False)
+    assert r["source"] == "files"
+    assert np.array_equal(r["volume"]["db"].cpu().numpy(), ref, equal_nan=True)
