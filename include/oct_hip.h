@@ -236,6 +236,13 @@ int oct_opspectral(const oct_opplan* plan, const void* raw, int raw_dtype, int n
  * out[b, z] = median over x of mag[b, x, z] (float64 mean of the two middle values for even n_x),
  * mag rows z in [0, n_z) of A-scans spaced x_stride floats; n_x <= 1024. */
 int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, double* out, void* stream);
+/* The same after the optical-path correction (octrecon/estimation/focus.py:110-129): B-scan b is
+ * tile frame frame_idx[b] (device int32); row z of column x reads row clip(z + shift[f, x]) and
+ * exists only for valid_lo[f, x] <= z <= valid_hi[f, x] ((n_frames, n_x) int32, as the stitcher's
+ * optical-path tables); the median is over the rows that exist (numpy.nanmedian), NaN if none. */
+int oct_depth_median_opc(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, const int32_t* frame_idx,
+                         const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, double* out,
+                         void* stream);
 
 /* ---- learned beta / focus estimator (no counterpart in the reference: it replaces the searches
  * of octrecon/estimation/dispersion.py:154-267 and focus.py:191-218, which stay the labels, the

@@ -68,6 +68,7 @@ SIGNATURES = {
     "oct_plan_use_windows": (_i, [_vp, _vp, _ll, _ll]),
     "oct_spectral_repeat": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_depth_median": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp]),
+    "oct_depth_median_opc": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "oct_logmag_crop": (_i, [_vp, _i, _i, _ll, _i, _i, _i, _i, _i, _i, _f, _f, _vp, _i, _vp]),
     "oct_cnn_conv_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_data": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),

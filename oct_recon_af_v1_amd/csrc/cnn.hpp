@@ -428,26 +428,60 @@ __device__ float wave_select(const float* col, int nX, int k) {
     return __builtin_bit_cast(float, key >= 0 ? key : (key ^ 0x7FFFFFFF));
 }
 
+// kOpc: the optical-path correction applied on the fly (octrecon/estimation/focus.py:110-129):
+// row z of column x reads row clip(z + shift[f, x], 0, nz - 1) and exists only for
+// vlo[f, x] <= z <= vhi[f, x] (f = frame_idx[b]); missing values are left out of the median
+// (numpy.nanmedian), NaN when a row has none.  They are staged as NaN, whose keys sort above
+// every number, so the order statistics of the valid values are unchanged.
+template <bool kOpc>
 __global__ __launch_bounds__(kCnnThreads) void depth_median_kernel(const float* __restrict__ mag, long long xstride,
-                                                                   int nX, int nz, double* __restrict__ out) {
+                                                                   int nX, int nz, const int* __restrict__ fidx,
+                                                                   const int* __restrict__ shift,
+                                                                   const int* __restrict__ vlo,
+                                                                   const int* __restrict__ vhi,
+                                                                   double* __restrict__ out) {
     extern __shared__ float cols[];   // [kMedZ][nX + 1]
     const int b = blockIdx.y;
     const int z0 = blockIdx.x * kMedZ;
     const int ld = nX + 1;
+    const long long orow = kOpc ? (long long)fidx[b] * nX : 0;
     for (int e = threadIdx.x; e < nX * kMedZ; e += kCnnThreads) {
         const int x = e / kMedZ, zz = e - x * kMedZ;
         const int z = z0 + zz;
-        cols[zz * ld + x] = (z < nz) ? mag[((long long)b * nX + x) * xstride + z] : 0.f;
+        float v = 0.f;
+        if (z < nz) {
+            if constexpr (kOpc) {
+                const long long o = orow + x;
+                const bool ok = z >= vlo[o] && z <= vhi[o];
+                const int src = min(max(z + shift[o], 0), nz - 1);
+                v = ok ? mag[((long long)b * nX + x) * xstride + src] : __builtin_nanf("");
+            } else {
+                v = mag[((long long)b * nX + x) * xstride + z];
+            }
+        }
+        cols[zz * ld + x] = v;
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
     for (int zz = wave; zz < kMedZ; zz += kCnnThreads / 64) {
         const int z = z0 + zz;
         if (z >= nz) break;
         const float* col = cols + zz * ld;
-        const float v1 = wave_select(col, nX, (nX - 1) / 2);
-        const float v2 = (nX & 1) ? v1 : wave_select(col, nX, nX / 2);
-        if ((threadIdx.x & 63) == 0) out[(long long)b * nz + z] = 0.5 * ((double)v1 + (double)v2);
+        int nv = nX;
+        if constexpr (kOpc) {
+            int c = 0;
+            for (int x = lane; x < nX; x += 64) c += (col[x] == col[x]) ? 1 : 0;
+            for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+            nv = c;
+        }
+        double r = __builtin_nan("");
+        if (nv > 0) {
+            const float v1 = wave_select(col, nX, (nv - 1) / 2);
+            const float v2 = (nv & 1) ? v1 : wave_select(col, nX, nv / 2);
+            r = 0.5 * ((double)v1 + (double)v2);
+        }
+        if (lane == 0) out[(long long)b * nz + z] = r;
     }
 }
 

@@ -988,8 +988,25 @@ int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride
         return fail(OCT_EINVAL, "median shape (%d x %d, stride %lld; at most %d A-scans)", n_x, n_z, x_stride, oct::kMedMaxX);
     if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
     const size_t lds = sizeof(float) * oct::kMedZ * (size_t)(n_x + 1);
-    hipLaunchKernelGGL(oct::depth_median_kernel, dim3((n_z + oct::kMedZ - 1) / oct::kMedZ, n_bscans),
-                       dim3(oct::kCnnThreads), lds, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, n_z, out);
+    hipLaunchKernelGGL(oct::depth_median_kernel<false>, dim3((n_z + oct::kMedZ - 1) / oct::kMedZ, n_bscans),
+                       dim3(oct::kCnnThreads), lds, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, n_z,
+                       (const int*)nullptr, (const int*)nullptr, (const int*)nullptr, (const int*)nullptr, out);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_depth_median_opc(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, const int32_t* frame_idx,
+                         const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, double* out,
+                         void* stream) {
+    if (!mag || !out || !frame_idx || !shift || !valid_lo || !valid_hi) return fail(OCT_EINVAL, "null argument");
+    if (n_bscans <= 0) return OCT_OK;
+    if (n_x <= 0 || n_x > oct::kMedMaxX || n_z <= 0 || n_z > x_stride)
+        return fail(OCT_EINVAL, "median shape (%d x %d, stride %lld; at most %d A-scans)", n_x, n_z, x_stride, oct::kMedMaxX);
+    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    const size_t lds = sizeof(float) * oct::kMedZ * (size_t)(n_x + 1);
+    hipLaunchKernelGGL(oct::depth_median_kernel<true>, dim3((n_z + oct::kMedZ - 1) / oct::kMedZ, n_bscans),
+                       dim3(oct::kCnnThreads), lds, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, n_z,
+                       frame_idx, shift, valid_lo, valid_hi, out);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
