@@ -168,6 +168,16 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
  * read_bscans).  n_lambda a multiple of 8, buffers 16-byte aligned. */
 int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda, int apod_start, int apod_end,
                    int scan_start, int scan_end, int16_t* out, void* stream);
+/* ---- PackBits (TIFF compression 32773) of uint16 code rows for the legacy BigTIFF output
+ * (octrecon/io/tiff_writer.py:53-71, which has tifffile compress each page): every row is one
+ * strip, encoded as runs over consecutive 128-byte chunks of its little-endian bytes (a uniform
+ * chunk -> replicate run, any other -> literal run).  oct_packbits_chunks: chunks per row;
+ * oct_packbits_sizes: bytes of every chunk (rows x chunks int32); oct_packbits_write: the runs at
+ * the exclusive prefix sum chunk_offsets (int64, same shape) of those sizes. */
+int oct_packbits_chunks(int row_elems);
+int oct_packbits_sizes(const uint16_t* codes, long long rows, int row_elems, int32_t* chunk_bytes, void* stream);
+int oct_packbits_write(const uint16_t* codes, long long rows, int row_elems, const int64_t* chunk_offsets,
+                       uint8_t* out, void* stream);
 int oct_binned_lines(int n_scan_lines, int binning);
 int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int keep_lines, int skip_lines,
                   int binning, float* out, void* stream);

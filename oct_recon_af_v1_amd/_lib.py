@@ -60,6 +60,9 @@ SIGNATURES = {
     "oct_opplan_destroy": (_i, [_vp]),
     "oct_opplan_set_window": (_i, [_vp, _vp, _vp]),
     "oct_srr_unpack": (_i, [_vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp]),
+    "oct_packbits_chunks": (_i, [_i]),
+    "oct_packbits_sizes": (_i, [_vp, _ll, _i, _vp, _vp]),
+    "oct_packbits_write": (_i, [_vp, _ll, _i, _vp, _vp, _vp]),
     "oct_opplan_scale_bound": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_plan_use_window": (_i, [_vp, _vp]),

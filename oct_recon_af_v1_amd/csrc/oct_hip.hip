@@ -274,6 +274,32 @@ int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda
     return OCT_OK;
 }
 
+int oct_packbits_chunks(int row_elems) { return row_elems > 0 ? (2 * row_elems + oct::kPbChunk - 1) / oct::kPbChunk : 0; }
+
+int oct_packbits_sizes(const uint16_t* codes, long long rows, int row_elems, int32_t* chunk_bytes, void* stream) {
+    if (!codes || !chunk_bytes) return fail(OCT_EINVAL, "null argument");
+    if (rows <= 0 || row_elems <= 0) return OCT_OK;
+    const long long tot = rows * oct_packbits_chunks(row_elems);
+    if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "too many rows");
+    hipLaunchKernelGGL(oct::packbits_size_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), codes, rows, row_elems, chunk_bytes);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+int oct_packbits_write(const uint16_t* codes, long long rows, int row_elems, const int64_t* chunk_offsets,
+                       uint8_t* out, void* stream) {
+    if (!codes || !chunk_offsets || !out) return fail(OCT_EINVAL, "null argument");
+    if (rows <= 0 || row_elems <= 0) return OCT_OK;
+    const long long tot = rows * oct_packbits_chunks(row_elems);
+    if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "too many rows");
+    hipLaunchKernelGGL(oct::packbits_write_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), codes, rows, row_elems,
+                       reinterpret_cast<const long long*>(chunk_offsets), out);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 int oct_average_repeats(const float* mag, float* out, int n_frames, int bscan_avg, int size_x, int ascan_avg, int rows,
                         long long in_stride, long long out_stride, void* stream) {
     if (!mag || !out) return fail(OCT_EINVAL, "null argument");

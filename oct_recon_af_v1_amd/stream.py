@@ -393,6 +393,7 @@ def main(argv=None):
     ap.add_argument("--form", default="mfma", choices=["mfma", "plain"])
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tiff", default=None, help="also write the volume as the legacy BigTIFF (+ .json)")
     a = ap.parse_args(argv)
     xp = get_xp("hip")[0]
     ss = StreamSpec(tiles_x=a.tiles_x, tiles_y=a.tiles_y, depths=a.depths, n_y=a.bscans, n_x=a.ascans, seed=a.seed)
@@ -401,7 +402,13 @@ def main(argv=None):
     print(f"streamed volume: {ss.n_tiles} tiles, {sv.raw.numel() * 2 / 1e9:.1f} GB raw resident", flush=True)
     sv.warmup()
     r = sv.run(validate=not a.no_validate)
-    r.pop("volume")
+    vol = r.pop("volume")
+    if a.tiff:
+        from .tiffio import write_legacy_tiff
+        t0 = time.perf_counter()
+        size = write_legacy_tiff(a.tiff, xp, vol["bits"], vol["clim"], {"spec": asdict(ss)})
+        r["tiff_seconds"] = time.perf_counter() - t0
+        r["tiff_bytes"] = size
     r["spec"] = asdict(ss)
     print(json.dumps(r), flush=True)
 
