@@ -373,6 +373,13 @@ static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int 
     a.nX = nX;
     a.row0 = row0;
     a.rows = rows;
+    {
+        // rows inside at most two blocks of 32 bins k = k1 + 32 k2: the kernel forms only those
+        // k2 of its second transform stage (oct::spectral_kernel, pruned stage)
+        const int k2lo = row0 / 32, k2hi = (row0 + rows - 1) / 32;
+        a.k2lo = k2lo;
+        a.k2n = (out_kind != OCT_OUT_LINEARISED && k2hi - k2lo < 2) ? k2hi - k2lo + 1 : 0;
+    }
     a.out_stride = out_stride;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     switch (raw_dtype) {

@@ -48,6 +48,7 @@ struct SpectralArgs {
     int line0;               // first scan A-line of a file
     int nX;                  // scan A-lines per file
     int row0, rows;          // kept depth rows [row0, row0 + rows)
+    int k2lo, k2n;           // k2n in {1, 2}: the rows lie in k2 blocks [k2lo, k2lo + k2n); 0: any rows
     long long out_stride;    // elements between consecutive A-scans in out
 };
 
@@ -92,6 +93,8 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
     __shared__ __align__(16) cf buf[kWaves][kBufCF];
     __shared__ __align__(16) cf tws[kLdsTw ? kQ : 1][kLdsTw ? 64 : 1];
     __shared__ float lmean[kWaves];
+    __shared__ cf tw2[2][kQ];            // pruned stage: W_32^(m k2) for the (at most 2) k2 kept
+    __shared__ cf tw64[2];               //               W_64^(k2)
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -102,6 +105,23 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
             double s, c;
             sincospi((double)(n2 * k1) / (kN / 2), &s, &c);
             tws[k1][n2] = cf{(float)c, (float)s};
+        }
+        // made visible to all waves by the first barrier of the group loop
+    }
+
+    if constexpr (KIND != KIND_EQ) {
+        if (a.k2n > 0 && threadIdx.x < 2 * kQ + 2) {
+            const int t = threadIdx.x < 2 * kQ ? threadIdx.x / kQ : threadIdx.x - 2 * kQ;
+            const int k2 = a.k2lo + t;
+            double s, c;
+            if (threadIdx.x < 2 * kQ) {
+                const int m = threadIdx.x % kQ;
+                sincospi(2.0 * (double)((m * k2) % kQ) / kQ, &s, &c);
+                tw2[t][m] = cf{(float)c, (float)s};
+            } else {
+                sincospi(2.0 * (double)k2 / 64.0, &s, &c);
+                tw64[t] = cf{(float)c, (float)s};
+            }
         }
         // made visible to all waves by the first barrier of the group loop
     }
@@ -240,6 +260,35 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
             const int h = lane & 1;
 #pragma unroll
             for (int m = 0; m < kQ; ++m) v[m] = B[kk1 * kStride + 2 * m + h];
+            if (a.k2n > 0) {
+                // pruned stage: only the k2 blocks holding the kept rows, each
+                // E_h[k2] = sum_m Z[2 m + h] W_32^(m k2) directly (32 MACs with tabled twiddles)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    if (t < a.k2n) {
+                        float ex = 0.f, ey = 0.f;
+#pragma unroll
+                        for (int m = 0; m < kQ; ++m) {
+                            const cf w = tw2[t][m];
+                            ex = __builtin_fmaf(v[m].x, w.x, __builtin_fmaf(-v[m].y, w.y, ex));
+                            ey = __builtin_fmaf(v[m].x, w.y, __builtin_fmaf(v[m].y, w.x, ey));
+                        }
+                        cf e{ex, ey};
+                        if (h) e = cmul(e, tw64[t]);
+                        const cf o{e.x + swap_pair(e.x), e.y + swap_pair(e.y)};
+                        const int k = kk1 + 32 * (a.k2lo + t);
+                        if (h == t && k >= a.row0 && k < a.row0 + a.rows) {
+                            if constexpr (KIND == KIND_MAG) {
+                                reinterpret_cast<float*>(a.out)[p * a.out_stride + (k - a.row0)] =
+                                    __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
+                            } else {
+                                reinterpret_cast<cf*>(a.out)[p * a.out_stride + (k - a.row0)] = o;
+                            }
+                        }
+                    }
+                }
+                continue;
+            }
             dft_inv_dif<kQ>(v);                          // E_h[k2] in v[bitrev(k2)]
             if (h) {
                 static_for<1, kQ>([&](auto k2) {
