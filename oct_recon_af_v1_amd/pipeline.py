@@ -99,6 +99,8 @@ class Reconstructor:
                 self.stitchers[(xi, zi)] = HipTileStitcher(base, self.xp, self.n_out[1])
         self.fin = DeviceFinalizer(self.xp)
         self._tile_win = None
+        self._streams = {}
+        self.concurrent_windows = True
         if tile_betas is not None:
             self.set_tile_betas(tile_betas)
         self._mag = None
@@ -241,13 +243,23 @@ class Reconstructor:
             return st.needed_rows()
         return 0, self.sp.nz
 
+    def _side_streams(self):
+        """Operator-form launches of different row windows run on their own streams: one tile's
+        GEMM (a few hundred workgroups) does not fill the chip, tiles of other focus depths do.
+        Tiles of one window share a stream (their operator may be rebuilt per tile).  Off when
+        binning or repeats share the processor's staging buffers."""
+        return (self.cfg.form == "mfma" and self.concurrent_windows and self.nb == 1 and self.na == 1
+                and self.sp.ascan_binning == 1)
+
     def _row_magnitudes(self, yi, frames, cancel=None):
         """Spectral stage for every tile of row yi into per-tile buffers (nF, nX, rows)."""
-        cfg = self.cfg
+        torch = self.torch
         frames = np.asarray(frames)
-        nF = len(frames)
-        B = max(1, int(cfg.batch_frames))
         out = []
+        side = self._side_streams()
+        main = torch.cuda.current_stream(self.xp.device) if side else None
+        start = None
+        used = {}
         for (xi, zi) in self.scan.tiles_in_row(yi):
             st = self.stitchers[(xi, zi)]
             if st.empty:
@@ -255,21 +267,51 @@ class Reconstructor:
             lo, hi = self._tile_rows(st)
             key = (xi, zi)
             buf = self._tile_mag.get(key)
-            nX = None
-            self._use_tile_window((xi, yi, zi), lo, hi)
-            for s in range(0, nF, B):
-                if cancel is not None and cancel.is_set():
-                    raise Cancelled()
-                fr = frames[s:s + B]
-                raw = self.tile_frames(xi, yi, zi, fr)
-                nX = self._positions(raw.shape[1])
-                shape = (nF, nX, hi - lo)
-                if buf is None or tuple(buf.shape) != shape:
-                    buf = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
-                    self._tile_mag[key] = buf
-                self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
+            ctx = None
+            if side:
+                if start is None:
+                    start = torch.cuda.Event()
+                    start.record(main)
+                s_ = self._streams.get((lo, hi))
+                if s_ is None:
+                    s_ = self._streams[(lo, hi)] = torch.cuda.Stream(device=self.xp.device)
+                if (lo, hi) not in used:
+                    s_.wait_event(start)
+                    used[(lo, hi)] = s_
+                ctx = torch.cuda.stream(s_)
+                ctx.__enter__()
+            try:
+                buf = self._tile_spectral(xi, yi, zi, frames, lo, hi, buf, key, cancel, main)
+            finally:
+                if ctx is not None:
+                    ctx.__exit__(None, None, None)
             out.append((st, buf, lo))
+        for s_ in used.values():
+            ev = torch.cuda.Event()
+            ev.record(s_)
+            main.wait_event(ev)
         return out
+
+    def _tile_spectral(self, xi, yi, zi, frames, lo, hi, buf, key, cancel, main=None):
+        """One tile's rows [lo, hi) of every frame into its buffer, on the current stream; a
+        buffer made on a side stream is marked as used by `main` (the stitch reads it there)."""
+        nF = len(frames)
+        B = max(1, int(self.cfg.batch_frames))
+        self._use_tile_window((xi, yi, zi), lo, hi)
+        for s in range(0, nF, B):
+            if cancel is not None and cancel.is_set():
+                raise Cancelled()
+            fr = frames[s:s + B]
+            raw = self.tile_frames(xi, yi, zi, fr)
+            nX = self._positions(raw.shape[1])
+            shape = (nF, nX, hi - lo)
+            if buf is None or tuple(buf.shape) != shape:
+                buf = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
+                if main is not None:
+                    buf.record_stream(main)
+                self._tile_mag[key] = buf
+            self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
+        return buf
 
     def _stitch_row(self, tiles, frames, db, plane0, partial):
         """One fused stitch + finalise launch for a row (oct_stitch_row)."""

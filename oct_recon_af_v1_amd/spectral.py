@@ -164,10 +164,16 @@ class HipSpectralProcessor:
                 torch.float32: _lib.OCT_RAW_F32}[t.dtype]
 
     def _workspace(self, n_files):
+        """Background-row workspace, one per stream (launches on different streams may overlap)."""
         need = _lib.lib().oct_spectral_workspace_bytes(self.plan.handle, n_files)
-        if self._ws is None or self._ws.numel() < need:
-            self._ws = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.xp.device)
-        return self._ws
+        if not isinstance(self._ws, dict):
+            self._ws = {}
+        key = self.xp.stream()
+        ws = self._ws.get(key)
+        if ws is None or ws.numel() < need:
+            ws = self.torch.empty(max(need, 1), dtype=self.torch.uint8, device=self.xp.device)
+            self._ws[key] = ws
+        return ws
 
     def n_scan_lines(self, lines: int) -> int:
         """Scan A-lines per raw file of `lines` lines, after binning."""
