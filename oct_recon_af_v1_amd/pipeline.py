@@ -99,6 +99,7 @@ class Reconstructor:
                 self.stitchers[(xi, zi)] = HipTileStitcher(base, self.xp, self.n_out[1])
         self.fin = DeviceFinalizer(self.xp)
         self._tile_win = None
+        self._tile_betas = {}
         self._streams = {}
         self.concurrent_windows = True
         if tile_betas is not None:
@@ -110,10 +111,13 @@ class Reconstructor:
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
 
     # ------------------------------------------------------------------ data access
-    def set_tile_betas(self, tile_betas):
+    def set_tile_betas(self, tile_betas, update: bool = False):
         """Per-tile dispersion windows on the device: float32 (already times 1/N) for the plain
         kernel, float64 for the operator build; both from geometry.dispersion_window, as the
-        plan's own window."""
+        plan's own window.  update=True adds to / replaces the tiles already set."""
+        if update and self._tile_betas:
+            tile_betas = {**self._tile_betas, **tile_betas}
+        self._tile_betas = dict(tile_betas)
         keys = sorted(tile_betas)
         if not keys:
             self._tile_win = None
@@ -419,19 +423,38 @@ class Reconstructor:
         vol["rows"] = rows
         return vol
 
+    def volume_row(self, vol, k, cancel=None):
+        """The k-th requested row of a volume (fused path): its planes of vol["db"] and its clim
+        partials.  Rows may be given in any order and at any time before finish_volume; used to
+        reconstruct a streamed volume row by row as its tiles arrive."""
+        if not self.cfg.fused_rows:
+            raise ValueError("row-by-row reconstruction needs the fused row path")
+        ny = self.scan.n_y_px
+        n_planes = len(vol["rows"]) * ny
+        partial, _ = self._row_partials(n_planes)
+        frames = np.arange(ny)
+        self._stitch_row(self._row_magnitudes(vol["rows"][k], frames, cancel), frames, vol["db"], k * ny, partial)
+
+    def finish_volume(self, vol):
+        """Clims and codes of a volume whose rows were all made by volume_row."""
+        ny = self.scan.n_y_px
+        n_planes = len(vol["rows"]) * ny
+        partial, nb = self._row_partials(n_planes)
+        self._restore_windows()
+        pcl, cl = self._clim(partial, n_planes, nb)
+        self.fin.quantize(vol["db"], cl, pcl, legacy_double=self.cfg.legacy_double_quantization, out=vol["bits"])
+        vol["plane_clim"], vol["clim"] = pcl, cl
+        return vol
+
     def reconstruct_volume(self, vol, cancel=None):
         """All requested y-tile rows into one device volume: dB, plane clims, global clim and
         uint16 codes (single finalise and quantise passes).  No host synchronisation."""
         ny = self.scan.n_y_px
         try:
             if self.cfg.fused_rows:
-                n_planes = len(vol["rows"]) * ny
-                partial, nb = self._row_partials(n_planes)
-                frames = np.arange(ny)
-                for k, r in enumerate(vol["rows"]):
-                    self._stitch_row(self._row_magnitudes(r, frames, cancel), frames, vol["db"], k * ny, partial)
-                pcl, cl = self._clim(partial, n_planes, nb)
-                db = vol["db"]
+                for k in range(len(vol["rows"])):
+                    self.volume_row(vol, k, cancel)
+                return self.finish_volume(vol)
             else:
                 vol["num"].zero_()
                 vol["den"].zero_()

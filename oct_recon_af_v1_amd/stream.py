@@ -7,10 +7,12 @@ One process drives one GPU with three threads:
             asynchronous copy on its own stream; the slot is handed back when the copy is done;
   main      as soon as a y-tile row has landed, the dispersion and focus of each of its tiles
             come from the learned estimator on `est_bscans` B-scans per tile (the sharpness
-            search and confocal detector for the B-scans it is not confident about); after the
-            last row: focus per depth (median of its tiles), every tile reconstructed at its own
-            dispersion (MFMA operator rebuilt on the device per tile) and stitched into one
-            volume with one clim and one quantisation pass.
+            search and confocal detector for the B-scans it is not confident about), and the
+            row is reconstructed at once, each tile at its own dispersion (MFMA operator rebuilt
+            on the device per tile), while the next rows stream; the focus per depth is the
+            median of the first row's tiles of that depth (every row holds all depths).  After
+            the last row: one clim and one quantisation pass over the volume.  rowwise=False
+            reconstructs after the stream instead, with the focus per depth from every row.
 
 The reference reads raw files with io_threads reader threads and a bounded read-ahead
 (octrecon/pipeline.py:197-212, 240-288), copies each batch with a pageable default-stream copy
@@ -64,7 +66,8 @@ class StreamedVolume:
     def __init__(self, xp, ss: StreamSpec = StreamSpec(), spec: SY.SpectrumSpec = SY.SpectrumSpec(),
                  fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 4,
                  gen_threads: int | None = None, estimator: str = "learned", est_bscans: int = 8,
-                 form: str = "mfma", log=None, readers: dict | None = None, io_threads: int = 8):
+                 form: str = "mfma", log=None, readers: dict | None = None, io_threads: int = 8,
+                 rowwise: bool = True):
         """readers: optional {(xi, yi, zi): rawio.OcityTile | rawio.SrrTile} streaming the tiles
         from their raw files (n_y files each) instead of the host generator; the generator's
         per-tile dispersion / focus stay the 'given' estimates."""
@@ -131,6 +134,7 @@ class StreamedVolume:
             self.fallback = HipDispersionScorer(self.lam, self.raw[0, :1, :spec.apod + self.est_x], spec.apod, xp)
         self._est_buf = None
         self._err = None
+        self.rowwise = bool(rowwise)
 
     # ------------------------------------------------------------------ threads
     def _guard(self, fn, *args):
@@ -255,7 +259,7 @@ class StreamedVolume:
         self.xp.synchronize()
 
     # ------------------------------------------------------------------ reconstruction
-    def reconstructor(self, tile_beta_idx, focus_per_depth, form=None):
+    def reconstructor(self, tile_beta_idx, focus_per_depth, form=None, idx=None):
         from .pipeline import ReconConfig, Reconstructor
         ss = self.ss
         b0 = beta_from_index_units(ss.beta_idx, self.lam)
@@ -265,13 +269,17 @@ class StreamedVolume:
                           focus_sigma=ss.focus_sigma_um / dz_um, crop_z_range_mm=list(ss.crop_z_mm),
                           output_pixel_size_um=1.0, n_medium=1.33, batch_frames=ss.n_y, form=form or self.form)
         tiles = {k: self.raw[i] for i, k in enumerate(self.keys)}
-        betas = {k: float(tile_beta_idx[i]) * self.to_ref for i, k in enumerate(self.keys)}
+        use = range(len(self.keys)) if idx is None else idx
+        betas = {self.keys[i]: float(tile_beta_idx[i]) * self.to_ref for i in use}
         return Reconstructor(cfg, self.grid, self.lam, tiles, self.spec.apod, xp=self.xp, tile_betas=betas)
 
-    def focus_per_depth(self, tile_focus):
-        """1-based focus pixel per focus plane: median over the plane's tiles (0-based bins + 1)."""
-        z = np.array([k[2] for k in self.keys])
-        return [float(np.median(tile_focus[z == d])) + 1.0 for d in range(self.ss.depths)]
+    def focus_per_depth(self, tile_focus, idx=None):
+        """1-based focus pixel per focus plane: median over the plane's tiles (0-based bins + 1),
+        of the tiles idx (default all)."""
+        idx = np.arange(len(self.keys)) if idx is None else np.asarray(idx)
+        z = np.array([self.keys[i][2] for i in idx])
+        f = np.asarray(tile_focus)[idx]
+        return [float(np.median(f[z == d])) + 1.0 for d in range(self.ss.depths)]
 
     # ------------------------------------------------------------------ run
     def run(self, validate: bool = True):
@@ -298,7 +306,8 @@ class StreamedVolume:
             t.start()
         landed = np.zeros(n, bool)
         done_rows = set()
-        t_est = 0.0
+        t_est = t_rows = 0.0
+        rec = vol = fpd = None
         try:
             while True:
                 i = landed_q.get()
@@ -316,6 +325,19 @@ class StreamedVolume:
                         b, f, nb = self.truth["beta_idx"][idx], self.truth["focus"][idx], np.zeros(len(idx), int)
                     tile_beta[idx], tile_focus[idx], tile_bad[idx] = b, f, nb
                     t_est += time.perf_counter() - te
+                    if self.rowwise:
+                        # reconstruct this row now, while the next rows stream; the focus per
+                        # depth comes from the first row (every row holds all depths)
+                        tr = time.perf_counter()
+                        if rec is None:
+                            fpd = self.focus_per_depth(tile_focus, idx)
+                            rec = self.reconstructor(tile_beta, fpd, idx=idx)
+                            vol = rec.allocate_volume()
+                        else:
+                            rec.set_tile_betas({self.keys[i]: float(tile_beta[i]) * self.to_ref for i in idx},
+                                               update=True)
+                        rec.volume_row(vol, yi)
+                        t_rows += time.perf_counter() - tr
         finally:
             free_q.put(None)
             for t in threads:
@@ -325,9 +347,12 @@ class StreamedVolume:
         if not landed.all():
             raise RuntimeError(f"{int((~landed).sum())} tiles never landed")
         t_stream = time.perf_counter()
-        fpd = self.focus_per_depth(tile_focus)
-        rec = self.reconstructor(tile_beta, fpd)
-        vol = rec.reconstruct_volume(rec.allocate_volume())
+        if self.rowwise:
+            vol = rec.finish_volume(vol)
+        else:
+            fpd = self.focus_per_depth(tile_focus)
+            rec = self.reconstructor(tile_beta, fpd)
+            vol = rec.reconstruct_volume(rec.allocate_volume())
         self.xp.synchronize()
         t1 = time.perf_counter()
         total = n * ss.n_y * ss.n_x
@@ -335,8 +360,9 @@ class StreamedVolume:
         out = {
             "ascans": total, "tiles": n, "raw_GB": raw_bytes / 1e9,
             "seconds": t1 - t0, "ascans_per_s": total / (t1 - t0),
-            "stream_seconds": t_stream - t0, "reconstruct_seconds": t1 - t_stream,
-            "reconstruct_ascans_per_s": total / (t1 - t_stream),
+            "stream_seconds": t_stream - t0, "after_stream_seconds": t1 - t_stream,
+            "row_reconstruction": "as rows land" if self.rowwise else "after the stream",
+            "row_enqueue_seconds": t_rows,
             "host_gen_ascans_per_s": total / stats["gen_s"] * 1.0 if stats["gen_s"] else None,
             "host_gen_threads": self.gen.threads,
             "h2d_GB_per_s": raw_bytes / stats["h2d_s"] / 1e9 if stats["h2d_s"] else None,
@@ -392,13 +418,15 @@ def main(argv=None):
     ap.add_argument("--estimator", default="learned", choices=["learned", "given"])
     ap.add_argument("--form", default="mfma", choices=["mfma", "plain"])
     ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--after-stream", action="store_true",
+                    help="reconstruct once every tile has landed (focus per depth from all rows)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tiff", default=None, help="also write the volume as the legacy BigTIFF (+ .json)")
     a = ap.parse_args(argv)
     xp = get_xp("hip")[0]
     ss = StreamSpec(tiles_x=a.tiles_x, tiles_y=a.tiles_y, depths=a.depths, n_y=a.bscans, n_x=a.ascans, seed=a.seed)
     sv = StreamedVolume(xp, ss, ring=a.ring, gen_threads=a.gen_threads, estimator=a.estimator, form=a.form,
-                        log=lambda *m: print(*m, flush=True))
+                        log=lambda *m: print(*m, flush=True), rowwise=not a.after_stream)
     print(f"streamed volume: {ss.n_tiles} tiles, {sv.raw.numel() * 2 / 1e9:.1f} GB raw resident", flush=True)
     sv.warmup()
     r = sv.run(validate=not a.no_validate)

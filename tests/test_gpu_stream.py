@@ -113,7 +113,8 @@ def test_streamed_volume_learned_estimator(xp):
 
 def test_row_table_path_equals_argument_path(xp, monkeypatch):
     """Rows of more tiles than the argument block holds read the descriptors from a device
-    table: the same volume, bit for bit."""
+    table, and a volume made row by row while streaming equals one made after the stream: the
+    same volume, bit for bit."""
     from oct_recon_af_v1_amd import pipeline as P
     from oct_recon_af_v1_amd.stream import StreamedVolume
     _say("case: row stitch through the device tile table")
@@ -121,7 +122,8 @@ def test_row_table_path_equals_argument_path(xp, monkeypatch):
     r = sv.run(validate=False)
     a = r["volume"]["db"].cpu().numpy()
     monkeypatch.setattr(P, "_ROW_ARG_TILES", 2)
-    rec = sv.reconstructor(sv.truth["beta_idx"], sv.focus_per_depth(sv.truth["focus"]))
+    # the whole volume at once (after the stream) with the focus the rows were made with
+    rec = sv.reconstructor(sv.truth["beta_idx"], r["focus_per_depth"])
     b = rec.reconstruct_volume(rec.allocate_volume())["db"].cpu().numpy()
     assert np.array_equal(a, b, equal_nan=True)
 
