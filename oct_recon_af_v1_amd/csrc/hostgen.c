@@ -14,8 +14,9 @@
  *   - shot noise is Gaussian with the Poisson variance (rate + sqrt(rate) n): n is the
  *     standardised sum of the four bytes of a per-lane xorshift32 draw (Irwin-Hall, 4 terms),
  *     then counts are rounded to nearest and clipped to [0, 4095].
- * The per-sample pass is written with AVX2/FMA intrinsics (16 samples per step, one pass over
- * each line); the banks are plain C left to the compiler's vectoriser.
+ * The per-sample pass is written with intrinsics (16 samples per step, one pass over each
+ * line), AVX-512 where the host has it and AVX2 otherwise, with identical results; the banks
+ * are plain C left to the compiler's vectoriser.
  * Every A-line draws from its own stream seeded by (seed, file, line), so the output does
  * not depend on the thread count.
  *
@@ -61,7 +62,102 @@ static inline double gate(const double* p, double z, double focus) {
     return p[OCTG_GFLOOR] + exp(-0.5 * d * d);
 }
 
+/* ---- the per-sample pass of one A-line: 1 + sum_t tw[t] tp[t][j] (or the source for a
+ * background line), times the source, clipped at 0, shot noise, rounded and clipped to 12 bits.
+ * Lane q of the 16 xorshift32 lanes draws the noise of samples j0 + q.  Two builds of the same
+ * arithmetic (FMA in the same order, correctly rounded sqrt): AVX2 (2 x 8 lanes) and AVX-512
+ * (16 lanes), chosen at run time; both write identical bits. */
+static void line_avx2(uint16_t* o, int n, const uint32_t* seed_lanes, const float* const* tp, const float* tw,
+                      int nt, float one, int bg_line, const float* srcf, int noise) {
+    __m256i x0 = _mm256_loadu_si256((const __m256i*)seed_lanes);
+    __m256i x1 = _mm256_loadu_si256((const __m256i*)(seed_lanes + 8));
+    const __m256 vone = _mm256_set1_ps(one), vzero = _mm256_setzero_ps(), vmax = _mm256_set1_ps(4095.f);
+    const __m256i bytes = _mm256_set1_epi32(255);
+    const __m256 off = _mm256_set1_ps(510.0f), inv_sd = _mm256_set1_ps(1.0f / 147.80054f);
+    for (int j0 = 0; j0 < n; j0 += 16) {
+        __m256 a0, a1;
+        if (bg_line) {
+            a0 = _mm256_loadu_ps(srcf + j0);
+            a1 = _mm256_loadu_ps(srcf + j0 + 8);
+        } else {
+            a0 = vone;
+            a1 = vone;
+            for (int t = 0; t < nt; ++t) {
+                const __m256 w = _mm256_set1_ps(tw[t]);
+                a0 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0), a0);
+                a1 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0 + 8), a1);
+            }
+            a0 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0), a0), vzero);
+            a1 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0 + 8), a1), vzero);
+        }
+        if (noise) {
+            x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 13));
+            x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 13));
+            x0 = _mm256_xor_si256(x0, _mm256_srli_epi32(x0, 17));
+            x1 = _mm256_xor_si256(x1, _mm256_srli_epi32(x1, 17));
+            x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 5));
+            x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 5));
+            __m256i s0 = _mm256_add_epi32(_mm256_and_si256(x0, bytes), _mm256_and_si256(_mm256_srli_epi32(x0, 8), bytes));
+            __m256i s1 = _mm256_add_epi32(_mm256_and_si256(x1, bytes), _mm256_and_si256(_mm256_srli_epi32(x1, 8), bytes));
+            s0 = _mm256_add_epi32(s0, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x0, 16), bytes), _mm256_srli_epi32(x0, 24)));
+            s1 = _mm256_add_epi32(s1, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x1, 16), bytes), _mm256_srli_epi32(x1, 24)));
+            const __m256 z0 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s0), off), inv_sd);
+            const __m256 z1 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s1), off), inv_sd);
+            a0 = _mm256_fmadd_ps(_mm256_sqrt_ps(a0), z0, a0);
+            a1 = _mm256_fmadd_ps(_mm256_sqrt_ps(a1), z1, a1);
+        }
+        a0 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a0, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
+        a1 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a1, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
+        const __m256i i0 = _mm256_cvtps_epi32(a0), i1 = _mm256_cvtps_epi32(a1);
+        const __m128i p0 = _mm_packus_epi32(_mm256_castsi256_si128(i0), _mm256_extracti128_si256(i0, 1));
+        const __m128i p1 = _mm_packus_epi32(_mm256_castsi256_si128(i1), _mm256_extracti128_si256(i1, 1));
+        _mm_stream_si128((__m128i*)(o + j0), p0);              /* o is 16-byte aligned (checked) */
+        _mm_stream_si128((__m128i*)(o + j0 + 8), p1);
+    }
+}
+
+__attribute__((target("avx512f,avx512bw,avx512vl,avx2,fma")))
+static void line_avx512(uint16_t* o, int n, const uint32_t* seed_lanes, const float* const* tp, const float* tw,
+                        int nt, float one, int bg_line, const float* srcf, int noise) {
+    __m512i x = _mm512_loadu_si512((const void*)seed_lanes);
+    const __m512 vone = _mm512_set1_ps(one), vzero = _mm512_setzero_ps(), vmax = _mm512_set1_ps(4095.f);
+    const __m512i bytes = _mm512_set1_epi32(255);
+    const __m512 off = _mm512_set1_ps(510.0f), inv_sd = _mm512_set1_ps(1.0f / 147.80054f);
+    for (int j0 = 0; j0 < n; j0 += 16) {
+        __m512 a;
+        if (bg_line) {
+            a = _mm512_loadu_ps(srcf + j0);
+        } else {
+            a = vone;
+            for (int t = 0; t < nt; ++t) a = _mm512_fmadd_ps(_mm512_set1_ps(tw[t]), _mm512_loadu_ps(tp[t] + j0), a);
+            a = _mm512_max_ps(_mm512_mul_ps(_mm512_loadu_ps(srcf + j0), a), vzero);
+        }
+        if (noise) {
+            x = _mm512_xor_si512(x, _mm512_slli_epi32(x, 13));
+            x = _mm512_xor_si512(x, _mm512_srli_epi32(x, 17));
+            x = _mm512_xor_si512(x, _mm512_slli_epi32(x, 5));
+            __m512i sm = _mm512_add_epi32(_mm512_and_si512(x, bytes), _mm512_and_si512(_mm512_srli_epi32(x, 8), bytes));
+            sm = _mm512_add_epi32(sm, _mm512_add_epi32(_mm512_and_si512(_mm512_srli_epi32(x, 16), bytes), _mm512_srli_epi32(x, 24)));
+            const __m512 z = _mm512_mul_ps(_mm512_sub_ps(_mm512_cvtepi32_ps(sm), off), inv_sd);
+            a = _mm512_fmadd_ps(_mm512_sqrt_ps(a), z, a);
+        }
+        a = _mm512_min_ps(_mm512_max_ps(_mm512_roundscale_ps(a, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
+        const __m256i pk = _mm512_cvtusepi32_epi16(_mm512_cvtps_epi32(a));
+        _mm_stream_si128((__m128i*)(o + j0), _mm256_castsi256_si128(pk));
+        _mm_stream_si128((__m128i*)(o + j0 + 8), _mm256_extracti128_si256(pk, 1));
+    }
+}
+
+/* AVX-512 where the host has it, unless OCTG_ISA=avx2 asks for the AVX2 build */
+static int octg_has_avx512(void) {
+    const char* isa = getenv("OCTG_ISA");
+    if (isa && strcmp(isa, "avx2") == 0) return 0;
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw") && __builtin_cpu_supports("avx512vl");
+}
+
 int octg_params_count(void) { return OCTG_NPARAM; }
+int octg_wide(void) { return octg_has_avx512(); }
 
 int octg_focus_tile(uint16_t* out, int n_files, int n_x, int apod, int n, const double* kn, const double* dq,
                     const double* src, double beta_idx, double focus_bins, uint64_t seed, const double* p,
@@ -144,6 +240,7 @@ int octg_focus_tile(uint16_t* out, int n_files, int n_x, int apod, int n, const 
      *      noise and rounding, in blocks of 16 samples held in registers (one pass per line) */
     const int noise = p[OCTG_NOISE] != 0.0;
     const double lr0 = log(p[OCTG_RMIN]), lr1 = log(p[OCTG_RMAX]);
+    const int wide = octg_has_avx512();
     const long long total = (long long)n_files * lines;
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 8)
     for (long long item = 0; item < total; ++item) {
@@ -173,60 +270,15 @@ int octg_focus_tile(uint16_t* out, int n_files, int n_x, int apod, int n, const 
                 tw[nt++] = (float)exp(0.5 * (lr0 + (lr1 - lr0) * u01(&s)));
             }
         }
-        /* two xorshift32 states of 8 lanes each: 16 samples per step */
         uint32_t seed_lanes[16];
         for (int q = 0; q < 16; ++q) {
             uint32_t v = (uint32_t)sm64(&s);
             seed_lanes[q] = v ? v : 0x6D2B79F5u;
         }
-        __m256i x0 = _mm256_loadu_si256((const __m256i*)seed_lanes);
-        __m256i x1 = _mm256_loadu_si256((const __m256i*)(seed_lanes + 8));
-        const __m256 vone = _mm256_set1_ps(one), vzero = _mm256_setzero_ps(), vmax = _mm256_set1_ps(4095.f);
-        const __m256i bytes = _mm256_set1_epi32(255);
-        const __m256 off = _mm256_set1_ps(510.0f), inv_sd = _mm256_set1_ps(1.0f / 147.80054f);
-        const __m256 tiny = _mm256_set1_ps(1e-20f);
-        for (int j0 = 0; j0 < n; j0 += 16) {
-            __m256 a0, a1;
-            if (l < apod) {                                      /* background line: the source */
-                a0 = _mm256_loadu_ps(srcf + j0);
-                a1 = _mm256_loadu_ps(srcf + j0 + 8);
-            } else {
-                a0 = vone;
-                a1 = vone;
-                for (int t = 0; t < nt; ++t) {
-                    const __m256 w = _mm256_set1_ps(tw[t]);
-                    a0 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0), a0);
-                    a1 = _mm256_fmadd_ps(w, _mm256_loadu_ps(tp[t] + j0 + 8), a1);
-                }
-                a0 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0), a0), vzero);
-                a1 = _mm256_max_ps(_mm256_mul_ps(_mm256_loadu_ps(srcf + j0 + 8), a1), vzero);
-            }
-            if (noise) {
-                x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 13));
-                x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 13));
-                x0 = _mm256_xor_si256(x0, _mm256_srli_epi32(x0, 17));
-                x1 = _mm256_xor_si256(x1, _mm256_srli_epi32(x1, 17));
-                x0 = _mm256_xor_si256(x0, _mm256_slli_epi32(x0, 5));
-                x1 = _mm256_xor_si256(x1, _mm256_slli_epi32(x1, 5));
-                __m256i s0 = _mm256_add_epi32(_mm256_and_si256(x0, bytes), _mm256_and_si256(_mm256_srli_epi32(x0, 8), bytes));
-                __m256i s1 = _mm256_add_epi32(_mm256_and_si256(x1, bytes), _mm256_and_si256(_mm256_srli_epi32(x1, 8), bytes));
-                s0 = _mm256_add_epi32(s0, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x0, 16), bytes), _mm256_srli_epi32(x0, 24)));
-                s1 = _mm256_add_epi32(s1, _mm256_add_epi32(_mm256_and_si256(_mm256_srli_epi32(x1, 16), bytes), _mm256_srli_epi32(x1, 24)));
-                const __m256 z0 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s0), off), inv_sd);
-                const __m256 z1 = _mm256_mul_ps(_mm256_sub_ps(_mm256_cvtepi32_ps(s1), off), inv_sd);
-                /* sqrt(rate) as rate * rsqrt(rate) (12-bit estimate: the noise's own scale) */
-                const __m256 r0 = _mm256_max_ps(a0, tiny), r1 = _mm256_max_ps(a1, tiny);
-                a0 = _mm256_fmadd_ps(_mm256_mul_ps(r0, _mm256_rsqrt_ps(r0)), z0, a0);
-                a1 = _mm256_fmadd_ps(_mm256_mul_ps(r1, _mm256_rsqrt_ps(r1)), z1, a1);
-            }
-            a0 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a0, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
-            a1 = _mm256_min_ps(_mm256_max_ps(_mm256_round_ps(a1, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC), vzero), vmax);
-            const __m256i i0 = _mm256_cvtps_epi32(a0), i1 = _mm256_cvtps_epi32(a1);
-            const __m128i p0 = _mm_packus_epi32(_mm256_castsi256_si128(i0), _mm256_extracti128_si256(i0, 1));
-            const __m128i p1 = _mm_packus_epi32(_mm256_castsi256_si128(i1), _mm256_extracti128_si256(i1, 1));
-            _mm_stream_si128((__m128i*)(o + j0), p0);          /* o is 16-byte aligned (checked) */
-            _mm_stream_si128((__m128i*)(o + j0 + 8), p1);
-        }
+        if (wide)
+            line_avx512(o, n, seed_lanes, tp, tw, nt, one, l < apod, srcf, noise);
+        else
+            line_avx2(o, n, seed_lanes, tp, tw, nt, one, l < apod, srcf, noise);
         _mm_sfence();                   /* this thread's streaming stores are visible */
     }
     free(bank); free(rr); free(ri); free(srcf); free(seeds);

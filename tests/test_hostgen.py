@@ -21,6 +21,19 @@ def test_deterministic_and_thread_independent():
     assert out[0].max() <= 4095
 
 
+def test_avx2_and_avx512_builds_agree(monkeypatch):
+    """The AVX-512 and AVX2 builds of the per-sample pass write identical bits."""
+    import ctypes
+    L = SY._hostgen()
+    L.octg_wide.restype = ctypes.c_int
+    out = [np.empty((2, 25 + 40, 2048), np.uint16) for _ in range(2)]
+    _gen(2).generate(out[0], 40, 1.2e-4, 350.0, 21)
+    monkeypatch.setenv("OCTG_ISA", "avx2")
+    assert L.octg_wide() == 0
+    _gen(2).generate(out[1], 40, 1.2e-4, 350.0, 21)
+    assert np.array_equal(out[0], out[1])
+
+
 def test_background_lines_shot_noise():
     g = _gen()
     out = np.empty((16, 25 + 8, 2048), np.uint16)
