@@ -64,7 +64,7 @@ class StreamSpec:
 
 class StreamedVolume:
     def __init__(self, xp, ss: StreamSpec = StreamSpec(), spec: SY.SpectrumSpec = SY.SpectrumSpec(),
-                 fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 4,
+                 fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 8,
                  gen_threads: int | None = None, estimator: str = "learned", est_bscans: int = 8,
                  form: str = "mfma", log=None, readers: dict | None = None, io_threads: int = 8,
                  rowwise: bool = True):
@@ -94,6 +94,9 @@ class StreamedVolume:
         focus = np.array([ss.focus0 + ss.focus_drift * k[2] for k in self.keys]) + rng.uniform(-1, 1, n) * ss.focus_jitter
         seeds = rng.integers(1, 2 ** 62, n)
         self.truth = {"beta_idx": beta, "focus": focus, "seed": seeds}
+        if gen_threads is None:               # one host core stays with the copier and main threads
+            import os
+            gen_threads = max(1, min(16, os.cpu_count() or 1) - 1)
         self.gen = SY.HostTileGenerator(spec, fspec, hspec, threads=gen_threads)
         self.lines = spec.apod + ss.n_x
         self.n = spec.n
@@ -188,25 +191,42 @@ class StreamedVolume:
             self.raw[i].copy_(src, non_blocking=True)
 
     def _copy(self, ready_q, free_q, landed_q, stats):
+        """Issue each ready tile's copy at once (copies queue back to back on the copy stream),
+        and hand slots back / report tiles as their copies complete."""
+        from collections import deque
         torch = self.torch
         stream = torch.cuda.Stream(device=self.xp.device)
+        pending = deque()                     # (tile, slot, start event, end event, queued behind)
+        done = False
         with torch.cuda.stream(stream):
-            while True:
-                item = ready_q.get()
+            while not done or pending:
+                while pending and pending[0][3].query():
+                    i, slot, e0, e1, prev = pending.popleft()
+                    t = e0.elapsed_time(e1)                 # from the later of its record and the
+                    if prev is not None:                    # previous copy's end: its own time
+                        t = min(t, prev.elapsed_time(e1))
+                    stats["h2d_s"] += t / 1e3
+                    free_q.put(slot)
+                    landed_q.put(i)
+                if done:
+                    time.sleep(1e-4)          # the GIL stays free for the others
+                    continue
+                try:
+                    item = ready_q.get(timeout=1e-4) if pending else ready_q.get()
+                except queue.Empty:
+                    continue
                 if item is None or self._err is not None:
-                    landed_q.put(None)
-                    return
+                    done = True
+                    continue
                 i, slot = item
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
+                prev = pending[-1][3] if pending else None
                 e0.record(stream)
                 self._land(i, self.ring[slot], stream)
                 e1.record(stream)
-                while not e1.query():                 # poll: the GIL stays free for the others
-                    time.sleep(1e-4)
-                stats["h2d_s"] += e0.elapsed_time(e1) / 1e3
-                free_q.put(slot)
-                landed_q.put(i)
+                pending.append((i, slot, e0, e1, prev))
+        landed_q.put(None)
 
     # ------------------------------------------------------------------ estimation
     def _gather(self, idx):
@@ -413,7 +433,7 @@ def main(argv=None):
     ap.add_argument("--depths", type=int, default=8)
     ap.add_argument("--bscans", type=int, default=40)
     ap.add_argument("--ascans", type=int, default=1000)
-    ap.add_argument("--ring", type=int, default=4)
+    ap.add_argument("--ring", type=int, default=8)
     ap.add_argument("--gen-threads", type=int, default=None)
     ap.add_argument("--estimator", default="learned", choices=["learned", "given"])
     ap.add_argument("--form", default="mfma", choices=["mfma", "plain"])
