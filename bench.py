@@ -231,7 +231,7 @@ def config2(xp, args):
 def estimator_bench(xp, args):
     """Learned beta / focus estimator on 64 held-out B-scans (512 A-scans each, beta uniform over
     +-2e-4, focus uniform over 0..1023): accuracy against the sharpness search and the confocal
-    detector (the judge), and estimation time per tile of 8 B-scans, network vs the restated
+    detector (the referee), and estimation time per tile of 8 B-scans, network vs the restated
     reference search (grid + bounded refinement of the mean and of each B-scan) plus detector."""
     import time
 

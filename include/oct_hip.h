@@ -256,7 +256,7 @@ int oct_depth_median_opc(const float* mag, int n_bscans, int n_x, long long x_st
 
 /* ---- learned beta / focus estimator (no counterpart in the reference: it replaces the searches
  * of octrecon/estimation/dispersion.py:154-267 and focus.py:191-218, which stay the labels, the
- * judge and the fallback).  bfloat16 activations and weights (uint16_t bit patterns, round to
+ * referee and the fallback).  bfloat16 activations and weights (uint16_t bit patterns, round to
  * nearest even), float32 master weights, gradients and accumulation.
  * Network input: out[b, c, h, w] (n_bscans, channels, height, width) bf16 from magnitude images
  * v = c n_bscans + b of mag (channels n_bscans, n_x, rows):

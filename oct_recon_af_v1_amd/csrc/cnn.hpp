@@ -9,7 +9,7 @@
 // dpre of the layer below.  Every index below is checked against the shapes on the host.
 //
 // There is no counterpart in the reference (octrecon/estimation/dispersion.py:154-267 and
-// focus.py:191-218 are the searches this network replaces; they remain the labels and the judge).
+// focus.py:191-218 are the searches this network replaces; they remain the labels and the referee).
 #pragma once
 #include "oct_common.hpp"
 

@@ -3,7 +3,7 @@ package's own HIP kernels (oct_cnn_* in liboct_hip).
 
 It stands in for the dispersion sharpness search (octrecon/estimation/dispersion.py:154-267) and
 the confocal focus detector (octrecon/estimation/focus.py:191-218, 221-279).  Both stay: they make
-the training labels, judge the network's error on held-out data, and are the fallback for
+the training labels, referee the network's error on held-out data, and are the fallback for
 B-scans where the network's own predicted uncertainty is above a threshold.
 
 Input: two 256 x 256 log-magnitude images of the central 256 A-scans of a B-scan, reconstructed
@@ -433,9 +433,9 @@ def calibrate_gate(err, sigma, tol, target=0.99):
 
 # --------------------------------------------------------------------------- estimator with fallback
 DEFAULT_PARAMS = __import__("pathlib").Path(__file__).resolve().parent / "data" / "estimator_bf16.npz"
-SCORE_TOL = 0.05          # judge tolerance: sharpness (log top-1 % intensity) within 0.05 of the optimum
+SCORE_TOL = 0.05          # referee tolerance: sharpness (log top-1 % intensity) within 0.05 of the optimum
 BETA_TOL_IDX = 5.5e-6     # |beta error| (index units) at which that drop is reached on this spectrometer
-FOCUS_TOL_BINS = 8.8      # judge tolerance: the stitching focus weight's standard deviation (sqrt(2) x 30 um / dz)
+FOCUS_TOL_BINS = 8.8      # referee tolerance: the stitching focus weight's standard deviation (sqrt(2) x 30 um / dz)
 
 
 def take_bscans(raw, idx):

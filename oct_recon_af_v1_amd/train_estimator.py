@@ -139,7 +139,7 @@ def main(argv=None):
     log(f"labels: focus found on {ok.mean():.2%}; |beta_search - beta_true| median "
         f"{np.median(np.abs(lab['beta_idx'] - lab['beta_true'])):.3g}, |focus_det - focus_true| median "
         f"{np.nanmedian(np.abs(lab['focus'] - lab['focus_true'])):.2f} bins")
-    # B-scans without a detected focus keep the truth as their target (the detector is the judge
+    # B-scans without a detected focus keep the truth as their target (the detector is the referee
     # of the held-out set, the generator's gate position its stand-in for training)
     focus_t = np.where(ok, lab["focus"], lab["focus_true"])
     vfocus_t = np.where(np.isfinite(vlab["focus"]), vlab["focus"], vlab["focus_true"])

@@ -1,4 +1,4 @@
-"""The trained estimator (oct_recon_af_v1_amd/data/estimator_bf16.npz) on held-out B-scans, judged
+"""The trained estimator (oct_recon_af_v1_amd/data/estimator_bf16.npz) on held-out B-scans, scored
 by the sharpness search and the confocal detector it replaces.
 
 Tolerances (stated in oct_recon_af_v1_amd/learned.py):
@@ -8,7 +8,7 @@ Tolerances (stated in oct_recon_af_v1_amd/learned.py):
          weight's standard deviation at sigma = 30 um).
 Bars: of the B-scans the network accepts (its predicted sigma under the calibrated gates), at
 least 90 % within both tolerances, and at most half sent to the fallback; with the fallback
-every B-scan's estimate is within tolerance of the judge.  Speed, per tile of 8 B-scans, against
+every B-scan's estimate is within tolerance of the referee (search and detector).  Speed, per tile of 8 B-scans, against
 the restated reference search (coarse grid + bounded refinement of the mean and of every B-scan,
 octrecon/estimation/dispersion.py:154-267) plus the focus detector on the same device: at least
 50x faster when the tiles of a volume are estimated in one batch (how the pipeline runs it), and
@@ -63,7 +63,7 @@ def test_learned_estimator_vs_search_heldout(xp, heldout):
           f"fallback rate {1 - conf.mean():.1%}", flush=True)
     assert conf.mean() >= 0.5
     assert both[conf].mean() >= 0.9
-    # with the fallback, every B-scan is judged within tolerance
+    # with the fallback, every B-scan is scored within tolerance
     e = est.estimate(raw)
     sc.set_raw(raw)
     s_final = sc.scores_per_file(e["beta_ref"])
