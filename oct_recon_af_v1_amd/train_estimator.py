@@ -20,10 +20,13 @@ import numpy as np
 
 
 def build_dataset(xp, n_bscans: int, seed: int, n_x: int = 512, chunk: int = 250, keep_raw: bool = False,
-                  log=print):
+                  log=print, host_every: int = 0):
     """Stage-1 and stage-2 inputs X1, X2 (M, 2, 256, 256) bf16 device, labels and truth for
     M = n_bscans B-scans; stage-2 probes are centred at the label plus a uniform +-3e-5 offset
-    (the spread of stage-1 errors it corrects).  keep_raw: also return the raw B-scans."""
+    (the spread of stage-1 errors it corrects).  keep_raw: also return the raw B-scans.
+    host_every = k > 0: every k-th chunk comes from the host tile generator (the streamed
+    volume's source: bank-built speckle, one generator call per B-scan) instead of the device
+    generator, so the network sees both."""
     from . import learned as LN
     from . import synthetic as SY
     from .estimation import HipDispersionScorer
@@ -43,14 +46,25 @@ def build_dataset(xp, n_bscans: int, seed: int, n_x: int = 512, chunk: int = 250
     else:
         raw = torch.empty((chunk, spec.apod + n_x, spec.n), dtype=torch.uint16, device=xp.device)
     scorer = b1 = b2 = None
+    hostgen = hbuf = None
     t0 = time.perf_counter()
     tg = tl = 0.0
     for c0 in range(0, n_bscans, chunk):
         nc = min(chunk, n_bscans - c0)
         sl = slice(c0, c0 + nc)
         ta = time.perf_counter()
-        r = SY.make_focus_bscans(nc, n_x, beta_true[sl], focus_true[sl], spec, seed=seed * 100003 + c0,
-                                 device=xp.device, out=raw_all[sl] if keep_raw else raw[:nc])
+        dst = raw_all[sl] if keep_raw else raw[:nc]
+        if host_every and (c0 // chunk) % host_every == host_every - 1:
+            if hostgen is None:
+                hostgen = SY.HostTileGenerator(spec)
+                hbuf = torch.empty((chunk, spec.apod + n_x, spec.n), dtype=torch.uint16, pin_memory=True)
+            for k in range(nc):
+                hostgen.generate(hbuf[k:k + 1], n_x, beta_true[c0 + k], focus_true[c0 + k], seed * 100003 + c0 + k)
+            dst.copy_(hbuf[:nc])
+            r = dst
+        else:
+            r = SY.make_focus_bscans(nc, n_x, beta_true[sl], focus_true[sl], spec, seed=seed * 100003 + c0,
+                                     device=xp.device, out=dst)
         xp.synchronize()
         tb = time.perf_counter()
         if scorer is None:
@@ -112,6 +126,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--host-every", type=int, default=0,
+                    help="every k-th chunk of B-scans from the host tile generator (0: none)")
     ap.add_argument("--out", required=True)
     ap.add_argument("--metrics", default=None)
     ap.add_argument("--calibrate-only", default=None, metavar="WEIGHTS",
@@ -128,13 +144,13 @@ def main(argv=None):
     torch = xp.torch
     if a.calibrate_only:
         params, meta = LN.load_params(a.calibrate_only)
-        _, _, vlab, vraw = build_dataset(xp, a.n_val, a.seed + 1, keep_raw=True, log=log)
+        _, _, vlab, vraw = build_dataset(xp, a.n_val, a.seed + 1, keep_raw=True, log=log, host_every=a.host_every)
         meta.update(calibrate(xp, params, vlab, vraw))
         log(json.dumps(meta))
         LN.save_params(a.out, params, meta=meta)
         return 0
-    X1, X2, lab, _ = build_dataset(xp, a.n_bscans, a.seed, log=log)
-    V1, V2, vlab, vraw = build_dataset(xp, a.n_val, a.seed + 1, keep_raw=True, log=log)
+    X1, X2, lab, _ = build_dataset(xp, a.n_bscans, a.seed, log=log, host_every=a.host_every)
+    V1, V2, vlab, vraw = build_dataset(xp, a.n_val, a.seed + 1, keep_raw=True, log=log, host_every=a.host_every)
     ok = np.isfinite(lab["focus"])
     log(f"labels: focus found on {ok.mean():.2%}; |beta_search - beta_true| median "
         f"{np.median(np.abs(lab['beta_idx'] - lab['beta_true'])):.3g}, |focus_det - focus_true| median "
@@ -143,7 +159,7 @@ def main(argv=None):
     # of the held-out set, the generator's gate position its stand-in for training)
     focus_t = np.where(ok, lab["focus"], lab["focus_true"])
     vfocus_t = np.where(np.isfinite(vlab["focus"]), vlab["focus"], vlab["focus_true"])
-    res = dict(n_train=a.n_bscans, n_val=a.n_val, epochs=a.epochs, batch=a.batch, lr=a.lr,
+    res = dict(n_train=a.n_bscans, n_val=a.n_val, epochs=a.epochs, batch=a.batch, lr=a.lr, host_every=a.host_every,
                dataset_s=lab["elapsed_s"], generate_s=lab["generate_s"], label_s=lab["label_s"], stages=[])
     nets = []
     for st, (X, V, spec, centre, vcentre) in enumerate(((X1, V1, LN.STAGE1, 0.0, 0.0),
