@@ -29,43 +29,86 @@ constexpr int kConvCoB = 16;   // output channels per thread / wave (forward)
 constexpr int kConvCiB = 16;   // input channels per thread (backward data)
 
 // y[n, co, oy, ox] = relu(b[co] + sum_ci sum_{ky,kx} w[co, ci, ky, kx] x[n, ci, 2oy-1+ky, 2ox-1+kx])
-// Block = an 8 x 32 tile of output positions of one image and 16 output channels; the input
-// patch of the tile (17 x 65 pixels) and the 16 channels' weights are staged in LDS for 8 input
-// channels at a time, and every thread forms its position's 16 channels from LDS.  Summation
-// order per output: ci ascending, then ky, kx.  Grid (ceil(Wo / 32), ceil(Ho / 8), N Co / 16).
+// Block = an 8 x 32 tile of output positions of one image and CB (16, or 8 to double the blocks
+// of a small launch) output channels; the input patch of the tile and the CB channels' weights
+// are staged in LDS for 8 input channels at a time, and every thread forms its position's CB
+// channels from LDS.  The patch rows are read as 33 aligned bf16 pairs (input columns
+// 2 ox0 - 2 .. 2 ox0 + 63; W is even, so a pair is wholly inside or outside the image).  Staging
+// is unrolled: a thread issues all its loads of a chunk before its first LDS store, so a chunk
+// costs one round of load latency.  Summation order per output: ci ascending, then ky, kx.
+// Grid (ceil(Wo / 32), ceil(Ho / 8), N Co / CB).
 constexpr int kConvTY = 8, kConvTX = 32, kConvCC = 8;
-constexpr int kConvPH = 2 * kConvTY + 1, kConvPW = 2 * kConvTX + 1;
+constexpr int kConvPH = 2 * kConvTY + 1, kConvPP = kConvTX + 1, kConvPW = 2 * kConvPP;
+constexpr int kConvPairs = kConvCC * kConvPH * kConvPP;                     // [c][py][pair]
+constexpr int kConvPairsPT = (kConvPairs + kCnnThreads - 1) / kCnnThreads;  // per thread
 
+template <int CB>
 __global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* __restrict__ x,
                                                                const uint16_t* __restrict__ w,
                                                                const float* __restrict__ bias,
                                                                uint16_t* __restrict__ y, int N, int Ci, int H,
                                                                int W, int Co) {
-    __shared__ float patch[kConvCC][kConvPH][kConvPW];
-    __shared__ float wl[kConvCC][9][16];     // this chunk's weights, [ci][tap][channel]
+    constexpr int kWts = kConvCC * 9 * CB;                                  // [c][tap][channel]
+    constexpr int kWtsPT = (kWts + kCnnThreads - 1) / kCnnThreads;
+    __shared__ float patch[kConvCC * kConvPH * kConvPW];
+    __shared__ float wl[kWts];
     const int Ho = H >> 1, Wo = W >> 1;
-    const int groups = Co / 16;
-    const int n = blockIdx.z / groups, co0 = (blockIdx.z - (blockIdx.z / groups) * groups) * 16;
+    const int groups = Co / CB;
+    const int n = blockIdx.z / groups, co0 = (blockIdx.z - (blockIdx.z / groups) * groups) * CB;
     const int oyb = blockIdx.y * kConvTY, oxb = blockIdx.x * kConvTX;
     const int ty = threadIdx.x / kConvTX, tx = threadIdx.x - (threadIdx.x / kConvTX) * kConvTX;
-    const int iyb = 2 * oyb - 1, ixb = 2 * oxb - 1;
-    float acc[16];
+    const int iyb = 2 * oyb - 1, ixb = 2 * oxb - 2;                      // patch column 0 = input ixb
+    // pair e = c (17 33) + py 33 + k holds input columns ixb + 2k, +1 of chunk channel c: its
+    // offset in the chunk, or -1 outside the image (the same for every chunk)
+    int offs[kConvPairsPT];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = bias[co0 + j];
+    for (int i = 0; i < kConvPairsPT; ++i) {
+        const int e = threadIdx.x + i * kCnnThreads;
+        const int c = e / (kConvPH * kConvPP);
+        const int r = e - c * (kConvPH * kConvPP);
+        const int py = r / kConvPP, k = r - (r / kConvPP) * kConvPP;
+        const int iy = iyb + py, ix = ixb + 2 * k;
+        offs[i] = ((unsigned)iy < (unsigned)H && (unsigned)ix < (unsigned)W) ? (c * H + iy) * W + ix : -1;
+    }
+    float acc[CB];
+#pragma unroll
+    for (int j = 0; j < CB; ++j) acc[j] = bias[co0 + j];
     for (int c0 = 0; c0 < Ci; c0 += kConvCC) {
         const int cc = min(kConvCC, Ci - c0);
-        for (int e = threadIdx.x; e < cc * kConvPH * kConvPW; e += kCnnThreads) {
-            const int c = e / (kConvPH * kConvPW);
-            const int r = e - c * (kConvPH * kConvPW);
-            const int py = r / kConvPW, px = r - (r / kConvPW) * kConvPW;
-            const int iy = iyb + py, ix = ixb + px;
-            patch[c][py][px] = (iy >= 0 && iy < H && ix >= 0 && ix < W)
-                                   ? bf2f(x[(((long long)n * Ci + c0 + c) * H + iy) * W + ix])
-                                   : 0.f;
+        const uint16_t* xc = x + ((long long)n * Ci + c0) * H * W;
+        const int cw = cc * 9;
+        // all loads of the chunk first; a pair is inside the image and the chunk iff
+        // 0 <= offset < cc H W (the others read the chunk's first word and are zeroed at the
+        // store); weight e = j (cc 9) + c 9 + tap, the layout of w
+        const unsigned lim = (unsigned)(cc * H * W);
+        uint32_t pv[kConvPairsPT];
+        uint16_t pw[kWtsPT];
+#pragma unroll
+        for (int i = 0; i < kConvPairsPT; ++i)
+            pv[i] = *reinterpret_cast<const uint32_t*>(xc + ((unsigned)offs[i] < lim ? offs[i] : 0));
+#pragma unroll
+        for (int i = 0; i < kWtsPT; ++i) {
+            const int e = threadIdx.x + i * kCnnThreads;
+            const int j = e / cw, r = e - (e / cw) * cw;
+            pw[i] = w[e < cw * CB ? ((long long)(co0 + j) * Ci + c0) * 9 + r : 0LL];
         }
-        for (int e = threadIdx.x; e < cc * 9 * 16; e += kCnnThreads) {
-            const int j = e / (cc * 9), r = e - (e / (cc * 9)) * (cc * 9);     // r = c 9 + tap
-            wl[r / 9][r - (r / 9) * 9][j] = bf2f(w[((long long)(co0 + j) * Ci + c0) * 9 + r]);
+        if (c0) __syncthreads();                                         // last chunk's reads done
+#pragma unroll
+        for (int i = 0; i < kConvPairsPT; ++i) {
+            const int e = threadIdx.x + i * kCnnThreads;
+            if (e < kConvPairs) {
+                const uint32_t v = (unsigned)offs[i] < lim ? pv[i] : 0u;
+                patch[2 * e] = bf2f((uint16_t)(v & 0xFFFFu));            // little endian: lower column first
+                patch[2 * e + 1] = bf2f((uint16_t)(v >> 16));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kWtsPT; ++i) {
+            const int e = threadIdx.x + i * kCnnThreads;
+            if (e < cw * CB) {
+                const int j = e / cw, r = e - (e / cw) * cw;
+                wl[r * CB + j] = bf2f(pw[i]);
+            }
         }
         __syncthreads();
         for (int c = 0; c < cc; ++c) {
@@ -73,64 +116,91 @@ __global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* _
 #pragma unroll
             for (int ky = 0; ky < 3; ++ky)
 #pragma unroll
-                for (int kx = 0; kx < 3; ++kx) v[3 * ky + kx] = patch[c][2 * ty + ky][2 * tx + kx];
+                for (int kx = 0; kx < 3; ++kx)
+                    v[3 * ky + kx] = patch[(c * kConvPH + 2 * ty + ky) * kConvPW + 2 * tx + kx + 1];
 #pragma unroll
             for (int k = 0; k < 9; ++k)
 #pragma unroll
-                for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(wl[c][k][j], v[k], acc[j]);
+                for (int j = 0; j < CB; ++j) acc[j] = __builtin_fmaf(wl[(c * 9 + k) * CB + j], v[k], acc[j]);
         }
-        __syncthreads();
     }
     const int oy = oyb + ty, ox = oxb + tx;
     if (oy < Ho && ox < Wo) {
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
+        for (int j = 0; j < CB; ++j)
             y[((long long)n * Co + co0 + j) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(acc[j], 0.f));
     }
 }
 
 // The same convolution for layers with few output positions (small batches, deep layers): one
-// wave per (position, 16 output channels); lanes split the reduction over kk = ci 9 + tap (the
-// weights of one output channel are contiguous in kk), then a wave reduction; bias added last.
-// Grid ceil(N Ho Wo (Co / 16) / 4) blocks of 4 waves.
+// wave per (P horizontally adjacent output positions, 16 output channels); lanes split the
+// reduction over kk = ci 9 + tap (the weights of one output channel are contiguous in kk) and use
+// each loaded weight for the P positions.  The 64 lanes' partial sums of the 16 P outputs are then
+// combined by a reduce-scatter butterfly: at distance m = 32, 16, ..., 1 a lane keeps half of its
+// values and adds the partner lane's (lane ^ m) copies of them, own value first.  Every output is
+// thereby summed over the same pairs in the same order as a full xor butterfly of each value, with
+// about 16 P exchanges instead of 96 P; the lane that ends with output idx = q 16 + j writes it.
+// Bias last.  Wo % P == 0; grid ceil(N Ho (Wo / P) (Co / 16) / 4) blocks of 4 waves.
+template <int P>
 __global__ __launch_bounds__(kCnnThreads) void conv_fwd_wave_kernel(const uint16_t* __restrict__ x,
                                                                     const uint16_t* __restrict__ w,
                                                                     const float* __restrict__ bias,
                                                                     uint16_t* __restrict__ y, int N, int Ci, int H,
                                                                     int W, int Co) {
-    const int Ho = H >> 1, Wo = W >> 1;
+    constexpr int V = 16 * P;                 // outputs per wave: [q][j]
+    constexpr int R = 64 / V;                 // lanes that end up holding each output
+    const int Ho = H >> 1, Wo = W >> 1, Wg = Wo / P;
     const int groups = Co / 16;
     const long long task = (long long)blockIdx.x * (kCnnThreads / 64) + (threadIdx.x >> 6);
-    if (task >= (long long)N * Ho * Wo * groups) return;                 // uniform per wave
+    if (task >= (long long)N * Ho * Wg * groups) return;                 // uniform per wave
     const int lane = threadIdx.x & 63;
     const int g = (int)(task % groups);
     const long long p = task / groups;
-    const int ox = (int)(p % Wo);
-    const long long t = p / Wo;
+    const int ox0 = (int)(p % Wg) * P;
+    const long long t = p / Wg;
     const int oy = (int)(t % Ho);
     const int n = (int)(t / Ho);
     const int K = Ci * 9;
     const int co0 = g * 16;
-    float acc[16];
+    float a[V];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = 0.f;
+    for (int i = 0; i < V; ++i) a[i] = 0.f;
     const uint16_t* xn = x + (long long)n * Ci * H * W;
+#pragma unroll(P == 1 ? 4 : P == 2 ? 2 : 1)
     for (int kk = lane; kk < K; kk += 64) {
         const int ci = kk / 9, tap = kk - (kk / 9) * 9;
-        const int iy = 2 * oy - 1 + tap / 3, ix = 2 * ox - 1 + tap % 3;
-        const float v = (iy >= 0 && ix >= 0) ? bf2f(xn[((long long)ci * H + iy) * W + ix]) : 0.f;
+        const int iy = 2 * oy - 1 + tap / 3, dx = tap % 3 - 1;
+        float wv[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) acc[j] = __builtin_fmaf(bf2f(w[(long long)(co0 + j) * K + kk]), v, acc[j]);
+        for (int j = 0; j < 16; ++j) wv[j] = bf2f(w[(long long)(co0 + j) * K + kk]);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const int ix = 2 * (ox0 + q) + dx;
+            const float v = (iy >= 0 && ix >= 0) ? bf2f(xn[((long long)ci * H + iy) * W + ix]) : 0.f;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[q * 16 + j] = __builtin_fmaf(wv[j], v, a[q * 16 + j]);
+        }
     }
 #pragma unroll
-    for (int j = 0; j < 16; ++j)
-        for (int off = 32; off > 0; off >>= 1) acc[j] += __shfl_xor(acc[j], off);
-    if (lane < 16) {
-        float v = acc[0];
+    for (int s = 0; s < 6; ++s) {
+        const int m = 32 >> s;
+        const int cnt = (V >> s) > 1 ? (V >> s) : 1;                     // values held before this step
+        if (cnt > 1) {
+            const bool upper = (lane & m) != 0;                          // keeps the upper half
 #pragma unroll
-        for (int j = 1; j < 16; ++j) v = (lane == j) ? acc[j] : v;
-        v += bias[co0 + lane];
-        y[((long long)n * Co + co0 + lane) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(v, 0.f));
+            for (int i = 0; i < cnt / 2; ++i) {
+                const float keep = upper ? a[i + cnt / 2] : a[i];
+                const float send = upper ? a[i] : a[i + cnt / 2];
+                a[i] = keep + __shfl_xor(send, m);
+            }
+        } else {
+            a[0] += __shfl_xor(a[0], m);
+        }
+    }
+    if ((lane & (R - 1)) == 0) {
+        const int idx = lane / R, q = idx / 16, j = idx - (idx / 16) * 16;
+        const float v = a[0] + bias[co0 + j];
+        y[((long long)n * Co + co0 + j) * Ho * Wo + (long long)oy * Wo + ox0 + q] = f2bf(fmaxf(v, 0.f));
     }
 }
 
@@ -273,6 +343,7 @@ __global__ __launch_bounds__(kCnnThreads) void fc_fwd_kernel(const uint16_t* __r
     const uint16_t* xr = x + (long long)n * K;
     const uint16_t* wr = w + (long long)o * K;
     float acc = 0.f;
+#pragma unroll 8
     for (int k = lane; k < K; k += 64) acc = __builtin_fmaf(bf2f(wr[k]), bf2f(xr[k]), acc);
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) {
@@ -382,22 +453,48 @@ __global__ __launch_bounds__(kCnnThreads) void adam_kernel(float* __restrict__ p
 // stride `xstride` floats per A-scan:
 //   out[b, c, h, w] = bf16(clamp((20 log10(max_{j < zpool} mag[v, x0 + w, z0 + h zpool + j]) - offset)
 //                                 * inv_scale, -8, 8))
-// with magnitudes below 1e-30 taken as 1e-30.  Grid (ceil(H W / 256), C B); h runs fastest.
+// with magnitudes below 1e-30 taken as 1e-30.  Block = a 64 (h) x 64 (w) tile of one image:
+// lanes run along h to read (rows are contiguous in mag; kVec4: zpool = 4 read as one aligned
+// float4), the bf16 results are transposed through LDS, and lanes run along w to write (w is
+// contiguous in out).  Grid (ceil(H / 64) ceil(W / 64), C B).
+constexpr int kCropT = 64;
+
+template <bool kVec4>
 __global__ __launch_bounds__(kCnnThreads) void logmag_crop_kernel(const float* __restrict__ mag, long long xstride,
                                                                   int nX, int x0, int z0, int zpool, int H, int W,
                                                                   float offset, float inv_scale,
                                                                   uint16_t* __restrict__ out, int C, int B) {
+    __shared__ uint16_t tile[kCropT][kCropT + 2];                        // [w][h]
+    constexpr int kRows = kCropT / (kCnnThreads / 64);                   // 16 per wave
     const int v = blockIdx.y;
     const int c = v / B, b = v - (v / B) * B;
-    const int e = blockIdx.x * kCnnThreads + threadIdx.x;
-    if (e >= H * W) return;
-    const int h = e % H, w = e / H;
-    const float* src = mag + ((long long)v * nX + x0 + w) * xstride + z0 + (long long)h * zpool;
-    float m = src[0];
-    for (int j = 1; j < zpool; ++j) m = fmaxf(m, src[j]);
-    const float db = 20.0f * log10f(fmaxf(m, 1e-30f));
-    const float y = fminf(fmaxf((db - offset) * inv_scale, -8.f), 8.f);
-    out[(((long long)b * C + c) * H + h) * W + w] = f2bf(y);
+    const int th = (H + kCropT - 1) / kCropT;
+    const int h0 = (blockIdx.x % th) * kCropT, w0 = (blockIdx.x / th) * kCropT;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int h = h0 + lane;
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+        const int wl = q + 4 * i, w = w0 + wl;
+        if (h < H && w < W) {
+            const float* src = mag + ((long long)v * nX + x0 + w) * xstride + z0 + (long long)h * zpool;
+            float m;
+            if constexpr (kVec4) {
+                const float4 t = *reinterpret_cast<const float4*>(src);
+                m = fmaxf(fmaxf(fmaxf(t.x, t.y), t.z), t.w);
+            } else {
+                m = src[0];
+                for (int j = 1; j < zpool; ++j) m = fmaxf(m, src[j]);
+            }
+            const float db = 20.0f * log10f(fmaxf(m, 1e-30f));
+            tile[wl][lane] = f2bf(fminf(fmaxf((db - offset) * inv_scale, -8.f), 8.f));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRows; ++i) {
+        const int hl = q + 4 * i, w = w0 + lane;
+        if (h0 + hl < H && w < W) out[(((long long)b * C + c) * H + h0 + hl) * W + w] = tile[lane][hl];
+    }
 }
 
 // Exact median over x of mag[b, x, z] for every (b, z): float64 mean of the two middle order

@@ -858,6 +858,8 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
 
 
 // ---------------------------------------------------------------- learned estimator
+static constexpr long long kConvWaveMin = 2048;
+
 static int conv_shape_ok(int n, int ci, int h, int w, int co) {
     return n > 0 && ci > 0 && co > 0 && h >= 2 && w >= 2 && (h % 2) == 0 && (w % 2) == 0;
 }
@@ -872,17 +874,31 @@ int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int Ho = h / 2, Wo = wd / 2;
     if (Wo >= 2 * oct::kConvTX) {
-        // LDS-tiled: 8 x 32 output positions x 16 channels per block
-        const long long gz = (long long)n * (co / 16);
+        // LDS-tiled: 8 x 32 output positions x 16 channels per block, 8 channels when that
+        // would leave fewer than 2 blocks per CU
+        const unsigned gx = (Wo + oct::kConvTX - 1) / oct::kConvTX, gy = (Ho + oct::kConvTY - 1) / oct::kConvTY;
+        const int cb = ((long long)gx * gy * n * (co / 16) < 512) ? 8 : 16;
+        const long long gz = (long long)n * (co / cb);
         if (gz > 65535) return fail(OCT_EINVAL, "batch too large for one launch");
-        hipLaunchKernelGGL(oct::conv_fwd_kernel, dim3((Wo + oct::kConvTX - 1) / oct::kConvTX,
-                                                      (Ho + oct::kConvTY - 1) / oct::kConvTY, (unsigned)gz),
-                           dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);
+        if ((long long)oct::kConvCC * h * wd > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "image too large");
+        if (cb == 8)
+            hipLaunchKernelGGL(oct::conv_fwd_kernel<8>, dim3(gx, gy, (unsigned)gz), dim3(oct::kCnnThreads), 0, s, x, w,
+                               bias, y, n, ci, h, wd, co);
+        else
+            hipLaunchKernelGGL(oct::conv_fwd_kernel<16>, dim3(gx, gy, (unsigned)gz), dim3(oct::kCnnThreads), 0, s, x, w,
+                               bias, y, n, ci, h, wd, co);
     } else {
-        // narrow layers: one wave per (position, 16 channels), lanes across the reduction
+        // narrow layers: one wave per (P adjacent positions, 16 channels), lanes across the
+        // reduction; P = 4 or 2 while that still leaves >= 2048 waves (8 per CU)
         const long long waves = P * (co / 16);
-        hipLaunchKernelGGL(oct::conv_fwd_wave_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(oct::kCnnThreads), 0, s,
-                           x, w, bias, y, n, ci, h, wd, co);
+        const int pp = (Wo % 4 == 0 && waves / 4 >= kConvWaveMin) ? 4 : (Wo % 2 == 0 && waves / 2 >= kConvWaveMin) ? 2 : 1;
+        const dim3 grid((unsigned)((waves / pp + 3) / 4));
+        if (pp == 4)
+            hipLaunchKernelGGL(oct::conv_fwd_wave_kernel<4>, grid, dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);
+        else if (pp == 2)
+            hipLaunchKernelGGL(oct::conv_fwd_wave_kernel<2>, grid, dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);
+        else
+            hipLaunchKernelGGL(oct::conv_fwd_wave_kernel<1>, grid, dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);
     }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
@@ -1005,11 +1021,18 @@ int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride,
     if (channels < 1) return fail(OCT_EINVAL, "%d channels", channels);
     if (!(scale_db > 0.f)) return fail(OCT_EINVAL, "scale must be positive");
     if ((long long)n_bscans * channels > 65535) return fail(OCT_EINVAL, "at most 65535 images per call");
-    const long long hw = (long long)height * width;
-    hipLaunchKernelGGL(oct::logmag_crop_kernel, dim3((unsigned)((hw + oct::kCnnThreads - 1) / oct::kCnnThreads),
-                                                     n_bscans * channels),
-                       dim3(oct::kCnnThreads), 0, reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, x0, z0, zpool,
-                       height, width, offset_db, 1.0f / scale_db, out, channels, n_bscans);
+    const long long tiles = (long long)((height + oct::kCropT - 1) / oct::kCropT) * ((width + oct::kCropT - 1) / oct::kCropT);
+    if (tiles > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "crop too large");
+    const dim3 grid((unsigned)tiles, (unsigned)(n_bscans * channels));
+    const bool vec4 = zpool == 4 && z0 % 4 == 0 && x_stride % 4 == 0 && (reinterpret_cast<uintptr_t>(mag) & 15) == 0;
+    if (vec4)
+        hipLaunchKernelGGL(oct::logmag_crop_kernel<true>, grid, dim3(oct::kCnnThreads), 0,
+                           reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, x0, z0, zpool, height, width,
+                           offset_db, 1.0f / scale_db, out, channels, n_bscans);
+    else
+        hipLaunchKernelGGL(oct::logmag_crop_kernel<false>, grid, dim3(oct::kCnnThreads), 0,
+                           reinterpret_cast<hipStream_t>(stream), mag, x_stride, n_x, x0, z0, zpool, height, width,
+                           offset_db, 1.0f / scale_db, out, channels, n_bscans);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

@@ -52,9 +52,11 @@ def _rel(got, ref):
 
 
 # (ci, co, side, batch): the network's layers (LDS-tiled forward for outputs >= 64 wide, one wave
-# per position below), plus odd sizes for the tile edges and a partial input-channel chunk
+# per 1, 2 or 4 positions below, by the wave count), plus odd sizes for the tile edges and a
+# partial input-channel chunk
 LAYERS = [(2, 16, 256, 3), (2, 16, 256, 8), (16, 32, 128, 3), (16, 32, 128, 32), (32, 32, 64, 3), (32, 64, 32, 3),
-          (64, 64, 16, 3), (64, 64, 8, 3), (16, 16, 200, 2), (18, 16, 140, 2), (18, 16, 70, 2)]
+          (64, 64, 16, 3), (64, 64, 8, 3), (16, 16, 200, 2), (18, 16, 140, 2), (18, 16, 70, 2),
+          (32, 32, 64, 8), (32, 64, 32, 16)]
 
 
 @pytest.mark.parametrize("ci,co,h,n", LAYERS)
@@ -93,6 +95,31 @@ def test_conv_layer_kernels(xp, ci, co, h, n):
         _lib.check(L.oct_cnn_conv_bwd_data(dd.data_ptr(), wd.data_ptr(), None, dx.data_ptr(), n, ci, h, h, co,
                                            xp.stream()), "conv bwd data (no mask)")
         assert _rel(dx.cpu().numpy(), dx_ref) <= 2e-5
+
+
+@pytest.mark.parametrize("ci,co,h", [(32, 32, 64), (32, 64, 32), (2, 16, 256), (16, 32, 128)])
+def test_conv_forward_batch_invariant(xp, ci, co, h):
+    """An image's convolution is bit-identical whether it runs in a batch of 16 (4 or 2 positions
+    per wave on the narrow layers) or alone (1 position per wave): the launch shape changes only
+    which lanes hold which sums, not the order they are summed in."""
+    from oct_recon_af_v1_amd import _lib
+    print(f"case: conv {ci}->{co} at {h}x{h}, batch 16 vs 1", flush=True)
+    torch = xp.torch
+    L = _lib.lib()
+    rng = np.random.default_rng(ci + co + h)
+    n = 16
+    x = _bf(xp, rng.standard_normal((n, ci, h, h)).astype(np.float32))
+    w = _bf(xp, (rng.standard_normal((co, ci, 3, 3)) / np.sqrt(9 * ci)).astype(np.float32))
+    b = _f32(xp, rng.standard_normal(co) * 0.1)
+    y = torch.empty((n, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
+    _lib.check(L.oct_cnn_conv_fwd(x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(), n, ci, h, h, co,
+                                  xp.stream()), "conv fwd (batch)")
+    y1 = torch.empty((1, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
+    for k in (0, 5, 15):
+        xk = x[k:k + 1].contiguous()
+        _lib.check(L.oct_cnn_conv_fwd(xk.data_ptr(), w.data_ptr(), b.data_ptr(), y1.data_ptr(), 1, ci, h, h, co,
+                                      xp.stream()), "conv fwd (one image)")
+        assert torch.equal(y1[0].view(torch.int16), y[k].view(torch.int16)), k
 
 
 def test_fc_loss_adam_kernels(xp):
@@ -238,6 +265,15 @@ def test_depth_median_and_crop(xp):
     got = out.float().cpu().numpy()
     for c in (0, 1):
         ref = C.logmag_crop(mag[c], 22, 0, 4, 256, 256, 10.0, 20.0)
+        eq, worst = _bf_close(got[:, c], ref)
+        assert eq >= 0.999 and worst <= 1.0
+    # the general path: pooling of 3 rows, an unaligned start row, partial 64 x 64 tiles
+    out = torch.zeros((2, 2, 100, 70), dtype=torch.bfloat16, device=xp.device)
+    _lib.check(_lib.lib().oct_logmag_crop(md.data_ptr(), 2, 300, 1024, 1024, 5, 13, 3, 100, 70, 10.0, 20.0,
+                                          out.data_ptr(), 2, xp.stream()), "crop (general)")
+    got = out.float().cpu().numpy()
+    for c in (0, 1):
+        ref = C.logmag_crop(mag[c], 5, 13, 3, 100, 70, 10.0, 20.0)
         eq, worst = _bf_close(got[:, c], ref)
         assert eq >= 0.999 and worst <= 1.0
 
