@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* _
 // wave per (P horizontally adjacent output positions, 16 output channels); lanes split the
 // reduction over kk = ci 9 + tap (the weights of one output channel are contiguous in kk) and use
 // each loaded weight for the P positions.  The 64 lanes' partial sums of the 16 P outputs are then
-// combined by a reduce-scatter butterfly: at distance m = 32, 16, ..., 1 a lane keeps half of its
+// combined by full xor butterflies (P = 1) or, for P > 1, by a reduce-scatter butterfly: at distance m = 32, 16, ..., 1 a lane keeps half of its
 // values and adds the partner lane's (lane ^ m) copies of them, own value first.  Every output is
 // thereby summed over the same pairs in the same order as a full xor butterfly of each value, with
 // about 16 P exchanges instead of 96 P; the lane that ends with output idx = q 16 + j writes it.
@@ -166,7 +166,6 @@ __global__ __launch_bounds__(kCnnThreads) void conv_fwd_wave_kernel(const uint16
 #pragma unroll
     for (int i = 0; i < V; ++i) a[i] = 0.f;
     const uint16_t* xn = x + (long long)n * Ci * H * W;
-#pragma unroll(P == 1 ? 4 : P == 2 ? 2 : 1)
     for (int kk = lane; kk < K; kk += 64) {
         const int ci = kk / 9, tap = kk - (kk / 9) * 9;
         const int iy = 2 * oy - 1 + tap / 3, dx = tap % 3 - 1;
@@ -180,6 +179,21 @@ __global__ __launch_bounds__(kCnnThreads) void conv_fwd_wave_kernel(const uint16
 #pragma unroll
             for (int j = 0; j < 16; ++j) a[q * 16 + j] = __builtin_fmaf(wv[j], v, a[q * 16 + j]);
         }
+    }
+    if constexpr (P == 1) {
+        // one position: 16 independent full butterflies pipeline better than the dependent
+        // reduce-scatter steps (same pairs, same order, same sums)
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            for (int off = 32; off > 0; off >>= 1) a[j] += __shfl_xor(a[j], off);
+        if (lane < 16) {
+            float v = a[0];
+#pragma unroll
+            for (int j = 1; j < 16; ++j) v = (lane == j) ? a[j] : v;
+            v += bias[co0 + lane];
+            y[((long long)n * Co + co0 + lane) * Ho * Wo + (long long)oy * Wo + ox0] = f2bf(fmaxf(v, 0.f));
+        }
+        return;
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {

@@ -889,9 +889,10 @@ int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
                                bias, y, n, ci, h, wd, co);
     } else {
         // narrow layers: one wave per (P adjacent positions, 16 channels), lanes across the
-        // reduction; P = 4 or 2 while that still leaves >= 2048 waves (8 per CU)
+        // reduction; P = 4 while that leaves >= 2048 waves (8 per CU), P = 2 only where Wo is
+        // not a multiple of 4 and >= 4096 waves remain
         const long long waves = P * (co / 16);
-        const int pp = (Wo % 4 == 0 && waves / 4 >= kConvWaveMin) ? 4 : (Wo % 2 == 0 && waves / 2 >= kConvWaveMin) ? 2 : 1;
+        const int pp = (Wo % 4 == 0 && waves / 4 >= kConvWaveMin) ? 4 : (Wo % 2 == 0 && waves / 2 >= 2 * kConvWaveMin) ? 2 : 1;
         const dim3 grid((unsigned)((waves / pp + 3) / 4));
         if (pp == 4)
             hipLaunchKernelGGL(oct::conv_fwd_wave_kernel<4>, grid, dim3(oct::kCnnThreads), 0, s, x, w, bias, y, n, ci, h, wd, co);

@@ -56,7 +56,7 @@ def _rel(got, ref):
 # partial input-channel chunk
 LAYERS = [(2, 16, 256, 3), (2, 16, 256, 8), (16, 32, 128, 3), (16, 32, 128, 32), (32, 32, 64, 3), (32, 64, 32, 3),
           (64, 64, 16, 3), (64, 64, 8, 3), (16, 16, 200, 2), (18, 16, 140, 2), (18, 16, 70, 2),
-          (32, 32, 64, 8), (32, 64, 32, 16)]
+          (32, 32, 64, 8), (32, 64, 32, 16), (16, 16, 20, 128)]
 
 
 @pytest.mark.parametrize("ci,co,h,n", LAYERS)
