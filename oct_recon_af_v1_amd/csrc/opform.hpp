@@ -170,8 +170,28 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
     const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
     const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
 
+    // Software pipeline: the raw samples and operator rows of step k0 + kOpK are loaded into
+    // registers while the matrix cores work on step k0 from LDS; the background row is read
+    // (L1-resident, one row per file) when the staged samples are converted.
+    constexpr int kRawVec = 16 / sizeof(RawT);                  // samples per 16-byte load
+    constexpr int kRawLd = kOpK / kRawVec;                      // 16-byte loads per thread per step
+    uint4 rv[kRawLd];
+    half8_t oh[2], ol[2];
+    auto fetch = [&](int k0) {
+        if (pvalid) {
+#pragma unroll
+            for (int g = 0; g < kRawLd; ++g) rv[g] = *reinterpret_cast<const uint4*>(rrow + k0 + kRawVec * g);
+        }
+        const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
+        const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
+        oh[0] = sh[0];
+        oh[1] = sh[1];
+        ol[0] = sl[0];
+        ol[1] = sl[1];
+    };
+    fetch(0);
     for (int k0 = 0; k0 < a.N; k0 += kOpK) {
-        // ---- stage data: d = raw - bg -> fp16 hi / lo
+        // ---- stage data: d = raw - bg -> fp16 hi / lo, and the operator rows
         {
             _Float16* hrow = dhi + tid * kOpPad;
             _Float16* lrow = dlo + tid * kOpPad;
@@ -180,13 +200,16 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
                 for (int g = 0; g < kOpK / 8; ++g) {
                     float f[8];
                     if constexpr (sizeof(RawT) == 2) {
-                        const uint4 u = *reinterpret_cast<const uint4*>(rrow + k0 + 8 * g);
-                        unpack16<RawT>(u, f);
+                        unpack16<RawT>(rv[g], f);
                     } else {
-                        const float4 u0 = *reinterpret_cast<const float4*>(rrow + k0 + 8 * g);
-                        const float4 u1 = *reinterpret_cast<const float4*>(rrow + k0 + 8 * g + 4);
-                        f[0] = u0.x; f[1] = u0.y; f[2] = u0.z; f[3] = u0.w;
-                        f[4] = u1.x; f[5] = u1.y; f[6] = u1.z; f[7] = u1.w;
+                        float f0[4], f1[4];
+                        unpack16<RawT>(rv[2 * g], f0);
+                        unpack16<RawT>(rv[2 * g + 1], f1);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            f[e] = f0[e];
+                            f[4 + e] = f1[e];
+                        }
                     }
                     const float4 b0 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g);
                     const float4 b1 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g + 4);
@@ -210,19 +233,15 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
                     *reinterpret_cast<half8_t*>(lrow + 8 * g) = z;
                 }
             }
-        }
-        // ---- stage operator rows
-        {
-            const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
-            const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
             half8_t* th = reinterpret_cast<half8_t*>(ahi + orow * kOpPad + ohalf);
             half8_t* tl = reinterpret_cast<half8_t*>(alo + orow * kOpPad + ohalf);
-            th[0] = sh[0];
-            th[1] = sh[1];
-            tl[0] = sl[0];
-            tl[1] = sl[1];
+            th[0] = oh[0];
+            th[1] = oh[1];
+            tl[0] = ol[0];
+            tl[1] = ol[1];
         }
         __syncthreads();
+        if (k0 + kOpK < a.N) fetch(k0 + kOpK);                  // in flight during the products
 #pragma unroll
         for (int ks = 0; ks < kOpK; ks += 16) {
             const int kk = ks + 8 * (lane >> 5);
