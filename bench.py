@@ -33,16 +33,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def profiled_traffic(name: str):
-    """HBM bytes per A-scan measured by rocprofv3 FETCH_SIZE/WRITE_SIZE (copy-calibrated) for
-    this kernel on this workload, as recorded in profiles/<name>.md; None if absent."""
+def profiled_value(name: str, key: str = "traffic_bytes_per_ascan"):
+    """A per-A-scan figure recorded in profiles/<name>.md as '<key>: value' (HBM bytes from
+    rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated; VALU wave instructions from SQ_INSTS_VALU)
+    for this kernel on this workload; None if absent."""
     p = ROOT / "profiles" / f"{name}.md"
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.strip().startswith("traffic_bytes_per_ascan:"):
+        if line.strip().startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+# VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
+VALU_PEAK_GIPS = 256 * 4 * 2.4 / 4
 
 
 def make_workload(xp, args):
@@ -342,7 +347,15 @@ def main():
     clim = vol["clim"].cpu().numpy().tolist()
     log(f"plain: step {el / args.steps * 1e3:.2f} ms; spectral launch avg {avg_ms:.3f} ms over {launches} launches "
         f"({per_launch:.0f} A-scans, {rows} rows); clim {clim}")
-    tpa = profiled_traffic("spectral_kernel_tiled_fp32") if args.spectral_rows == "needed" else None
+    tpa = profiled_value("spectral_kernel_tiled_fp32") if args.spectral_rows == "needed" else None
+    ipa = profiled_value("spectral_kernel_tiled_fp32", "valu_instructions_per_ascan") \
+        if args.spectral_rows == "needed" else None
+    valu = None
+    if ipa:
+        gips = ipa * per_launch / (avg_ms * 1e-3) / 1e9
+        valu = {"achieved": gips, "peak": VALU_PEAK_GIPS, "unit": "G wave-instructions/s", "frac": gips / VALU_PEAK_GIPS,
+                "instructions_per_ascan": ipa,
+                "source": "profiles/spectral_kernel_tiled_fp32.md (rocprofv3 SQ_INSTS_VALU); peak = non-packed FP32 issue"}
     res = {
         "metric": METRIC, "value": value, "unit": "A-scans/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
@@ -360,7 +373,7 @@ def main():
                      "kernel": "spectral_kernel<uint16_t,12,MAG,8>",
                      "bound_note": "VALU-issue bound (non-packed FP32): the HBM fraction is reported as asked",
                      "bytes_per_ascan": bytes_per_ascan, "ascans_per_launch": per_launch,
-                     "avg_launch_ms": avg_ms},
+                     "avg_launch_ms": avg_ms, "valu_issue": valu},
     }
 
     if not args.no_mfma:
