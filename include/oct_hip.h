@@ -6,7 +6,8 @@
  * return without synchronising the host, like the reference's backend calls between its
  * synchronize() points (octrecon/pipeline.py:276, 281, 285).  Every function returns 0 on
  * success or a negative status; oct_last_error() gives the message of the calling thread's
- * last failure.
+ * last failure.  A call on an empty batch (zero files, B-scans, rows, planes or frames) returns
+ * 0 before looking at its pointers (an empty tensor's may be NULL) and launches nothing.
  *
  * Reference interfaces these replace (atesfet/oct_recon_AF_v1):
  *   oct_plan_create / oct_plan_destroy

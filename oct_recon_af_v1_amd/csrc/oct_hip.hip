@@ -199,11 +199,11 @@ static int check_raw(const void* raw, int raw_dtype, int n_files, int lines, int
 
 int oct_background(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                    int apod_size, int apod_group, float* bg, void* stream) {
+    if (n_files == 0) return OCT_OK;
     if (!p || !bg) return fail(OCT_EINVAL, "null argument");
     int line0, nX;
     int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, apod_group, &line0, &nX);
     if (rc) return rc;
-    if (n_files == 0) return OCT_OK;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int mode = apod_mode == OCT_APOD_LINES ? 0 : 1;
     switch (raw_dtype) {
@@ -225,8 +225,8 @@ int oct_binned_lines(int n_scan_lines, int binning) {
 
 int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int keep_lines, int skip_lines,
                   int binning, float* out, void* stream) {
-    if (!raw || !out) return fail(OCT_EINVAL, "null argument");
     if (n_files <= 0) return OCT_OK;
+    if (!raw || !out) return fail(OCT_EINVAL, "null argument");
     if (binning < 1 || keep_lines < 0 || skip_lines < keep_lines || skip_lines >= lines)
         return fail(OCT_EINVAL, "binning %d, keep %d, skip %d of %d lines", binning, keep_lines, skip_lines, lines);
     if (raw_dtype < OCT_RAW_I16 || raw_dtype > OCT_RAW_F32) return fail(OCT_EINVAL, "raw dtype %d", raw_dtype);
@@ -256,8 +256,8 @@ int oct_bin_lines(const void* raw, int raw_dtype, int n_files, int lines, int ke
 
 int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda, int apod_start, int apod_end,
                    int scan_start, int scan_end, int16_t* out, void* stream) {
-    if (!in || !out) return fail(OCT_EINVAL, "null argument");
     if (n_files <= 0) return OCT_OK;
+    if (!in || !out) return fail(OCT_EINVAL, "null argument");
     if (n_lambda <= 0 || (n_lambda % 8)) return fail(OCT_EINVAL, "n_lambda %d must be a positive multiple of 8", n_lambda);
     if (apod_start < 0 || apod_end <= apod_start || scan_start < 0 || scan_end <= scan_start ||
         apod_end > file_lines || scan_end > file_lines)
@@ -277,8 +277,8 @@ int oct_srr_unpack(const uint16_t* in, int n_files, int file_lines, int n_lambda
 int oct_packbits_chunks(int row_elems) { return row_elems > 0 ? (2 * row_elems + oct::kPbChunk - 1) / oct::kPbChunk : 0; }
 
 int oct_packbits_sizes(const uint16_t* codes, long long rows, int row_elems, int32_t* chunk_bytes, void* stream) {
-    if (!codes || !chunk_bytes) return fail(OCT_EINVAL, "null argument");
     if (rows <= 0 || row_elems <= 0) return OCT_OK;
+    if (!codes || !chunk_bytes) return fail(OCT_EINVAL, "null argument");
     const long long tot = rows * oct_packbits_chunks(row_elems);
     if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "too many rows");
     hipLaunchKernelGGL(oct::packbits_size_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
@@ -289,8 +289,8 @@ int oct_packbits_sizes(const uint16_t* codes, long long rows, int row_elems, int
 
 int oct_packbits_write(const uint16_t* codes, long long rows, int row_elems, const int64_t* chunk_offsets,
                        uint8_t* out, void* stream) {
-    if (!codes || !chunk_offsets || !out) return fail(OCT_EINVAL, "null argument");
     if (rows <= 0 || row_elems <= 0) return OCT_OK;
+    if (!codes || !chunk_offsets || !out) return fail(OCT_EINVAL, "null argument");
     const long long tot = rows * oct_packbits_chunks(row_elems);
     if ((tot + 255) / 256 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "too many rows");
     hipLaunchKernelGGL(oct::packbits_write_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
@@ -302,8 +302,8 @@ int oct_packbits_write(const uint16_t* codes, long long rows, int row_elems, con
 
 int oct_average_repeats(const float* mag, float* out, int n_frames, int bscan_avg, int size_x, int ascan_avg, int rows,
                         long long in_stride, long long out_stride, void* stream) {
-    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (n_frames <= 0 || size_x <= 0 || rows <= 0) return OCT_OK;
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (bscan_avg < 1 || ascan_avg < 1 || in_stride < rows || out_stride < rows)
         return fail(OCT_EINVAL, "bad repeat shape (%d, %d, strides %lld %lld, rows %d)", bscan_avg, ascan_avg, in_stride,
                     out_stride, rows);
@@ -334,11 +334,11 @@ int oct_spectral_repeat(const oct_plan* p, const void* raw, int raw_dtype, int n
 static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                          int apod_size, int n_repeat, int x0, int n_x, int out_kind, int row0, int rows, void* out,
                          long long out_stride, const float* bg, void* stream) {
+    if (n_files == 0) return OCT_OK;
     if (!p || !out || !bg) return fail(OCT_EINVAL, "null argument");
     int line0, nX;
     int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, &line0, &nX);
     if (rc) return rc;
-    if (n_files == 0) return OCT_OK;
     if (n_x >= 0) {                         // a window of the scan lines
         if (x0 < 0 || n_x < 1 || x0 + n_x > nX) return fail(OCT_EINVAL, "scan lines [%d, %d) outside [0, %d)", x0, x0 + n_x, nX);
         line0 += x0;
@@ -445,12 +445,12 @@ static int stitch_row_impl(const oct_row_tile* tiles, const oct_row_tile* dev_ti
                            int n_z, const int32_t* frame_idx, const int32_t* shift, const int32_t* valid_lo,
                            const int32_t* valid_hi, int n_out_z, int n_out_x, float thr, float* db, int plane0,
                            int* partial, void* stream) {
+    if (n_frames <= 0 || n_out_z <= 0 || n_out_x <= 0) return OCT_OK;
     if (!tiles || !frame_idx || !db || !partial) return fail(OCT_EINVAL, "null argument");
     if (n_tiles < 0 || (!dev_tiles && n_tiles > oct::kRowMaxTiles))
         return fail(OCT_EINVAL, "%d tiles (at most %d per row in the argument block)", n_tiles, oct::kRowMaxTiles);
     if (dev_tiles && ((uintptr_t)dev_tiles & 7)) return fail(OCT_EINVAL, "tile table must be 8-byte aligned");
     if (shift && (!valid_lo || !valid_hi)) return fail(OCT_EINVAL, "shift without valid bounds");
-    if (n_frames <= 0 || n_out_z <= 0 || n_out_x <= 0) return OCT_OK;
     if (n_x <= 0 || n_z <= 0 || plane0 < 0) return fail(OCT_EINVAL, "bad row shape");
     oct::RowArgs a;
     a.tt = reinterpret_cast<const oct::RowTile*>(dev_tiles);
@@ -562,8 +562,8 @@ int oct_finalize(const float* num, const float* den, float* db, int n_planes, lo
 
 int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_elems, const float* clim,
                  const float* plane_clim, int legacy_double, void* stream) {
-    if (!db || !bits || !clim || (legacy_double && !plane_clim)) return fail(OCT_EINVAL, "null argument");
     if (n_planes <= 0 || plane_elems <= 0) return OCT_OK;
+    if (!db || !bits || !clim || (legacy_double && !plane_clim)) return fail(OCT_EINVAL, "null argument");
     const long long n = (long long)n_planes * plane_elems;
     const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 16);
     hipLaunchKernelGGL(oct::quantize_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), db,
@@ -608,8 +608,8 @@ size_t oct_topfrac_metric_workspace_bytes(int n_bscans, int n_x, int n_z, int z_
 
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* workspace, size_t workspace_bytes, void* stream) {
-    if (!mag || !out || !workspace) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
+    if (!mag || !out || !workspace) return fail(OCT_EINVAL, "null argument");
     if (n_x <= 0 || z_min < 0 || z_min >= n_z) return fail(OCT_EINVAL, "bad metric shape (%d, %d, zmin %d)", n_x, n_z, z_min);
     if (!(frac > 0.0 && frac <= 1.0)) return fail(OCT_EINVAL, "fraction %g outside (0, 1]", frac);
     if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
@@ -645,8 +645,8 @@ int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_m
 int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,
                            double to_ref, int count, const double* hann_over_n, const double* dk2, int n, float* out,
                            void* stream) {
-    if ((!betas && !head) || !hann_over_n || !dk2 || !out) return fail(OCT_EINVAL, "null argument");
     if (count <= 0) return OCT_OK;
+    if ((!betas && !head) || !hann_over_n || !dk2 || !out) return fail(OCT_EINVAL, "null argument");
     if (n <= 0 || count > 65535 || (head && head_stride <= 0)) return fail(OCT_EINVAL, "bad window shape (%d x %d)", count, n);
     if (reinterpret_cast<uintptr_t>(out) % 8) return fail(OCT_EINVAL, "out must be 8-byte aligned");
     hipLaunchKernelGGL(oct::windows_kernel, dim3((n + 255) / 256, count), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
@@ -821,11 +821,11 @@ int oct_opplan_set_window(oct_opplan* p, const double* window, void* stream) {
 
 int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                    int apod_size, float* out, long long out_stride, const float* bg, void* stream) {
+    if (n_files == 0) return OCT_OK;
     if (!p || !out || !bg) return fail(OCT_EINVAL, "null argument");
     int line0, nX;
     int rc = check_raw(raw, raw_dtype, n_files, lines, apod_mode, apod_size, 1, &line0, &nX);
     if (rc) return rc;
-    if (n_files == 0) return OCT_OK;
     if (out_stride < p->nzc) return fail(OCT_EINVAL, "out_stride %lld < bins %d", out_stride, p->nzc);
     oct::OpGemmArgs a;
     a.raw = raw;
@@ -1013,8 +1013,8 @@ int oct_cnn_adam(float* params, const float* grads, float* m, float* v, uint16_t
 
 int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride, int n_rows, int x0, int z0, int zpool,
                     int height, int width, float offset_db, float scale_db, uint16_t* out, int channels, void* stream) {
-    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (height <= 0 || width <= 0 || zpool <= 0 || x0 < 0 || z0 < 0 || x0 + width > n_x ||
         z0 + (long long)height * zpool > n_rows || n_rows > x_stride)
         return fail(OCT_EINVAL, "crop x [%d, %d) z [%d, %lld) outside (%d, %d)", x0, x0 + width, z0,
@@ -1039,8 +1039,8 @@ int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride,
 }
 
 int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, double* out, void* stream) {
-    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
+    if (!mag || !out) return fail(OCT_EINVAL, "null argument");
     if (n_x <= 0 || n_x > oct::kMedMaxX || n_z <= 0 || n_z > x_stride)
         return fail(OCT_EINVAL, "median shape (%d x %d, stride %lld; at most %d A-scans)", n_x, n_z, x_stride, oct::kMedMaxX);
     if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
@@ -1055,8 +1055,8 @@ int oct_depth_median(const float* mag, int n_bscans, int n_x, long long x_stride
 int oct_depth_median_opc(const float* mag, int n_bscans, int n_x, long long x_stride, int n_z, const int32_t* frame_idx,
                          const int32_t* shift, const int32_t* valid_lo, const int32_t* valid_hi, double* out,
                          void* stream) {
-    if (!mag || !out || !frame_idx || !shift || !valid_lo || !valid_hi) return fail(OCT_EINVAL, "null argument");
     if (n_bscans <= 0) return OCT_OK;
+    if (!mag || !out || !frame_idx || !shift || !valid_lo || !valid_hi) return fail(OCT_EINVAL, "null argument");
     if (n_x <= 0 || n_x > oct::kMedMaxX || n_z <= 0 || n_z > x_stride)
         return fail(OCT_EINVAL, "median shape (%d x %d, stride %lld; at most %d A-scans)", n_x, n_z, x_stride, oct::kMedMaxX);
     if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
