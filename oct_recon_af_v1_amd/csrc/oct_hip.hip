@@ -88,6 +88,15 @@ int launch_spectral(const oct_plan* p, const oct::SpectralArgs& a, int kind, hip
 }
 
 template <typename RawT>
+void launch_opgemm_rf(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
+    // background rows through LDS when a workgroup's A-scans lie in at most two files
+    if (a.nX >= oct::kOpCols && a.N == oct::kOpBgN)
+        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, true>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, false>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <typename RawT>
 int launch_background(const void* raw, float* bg, int B, int lines, int N, int apod, int mode, int group,
                       hipStream_t s) {
     const long long n = (long long)B * N;
@@ -847,10 +856,21 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     if (nblk > (1LL << 30)) return fail(OCT_EINVAL, "batch too large");
     const int grid = (int)((nblk + 7) / 8 * 8);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    switch (raw_dtype) {
-        case OCT_RAW_I16: hipLaunchKernelGGL(oct::opgemm_kernel<int16_t>, dim3(grid), dim3(256), 0, s, a); break;
-        case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
-        default: hipLaunchKernelGGL(oct::opgemm_kernel<float>, dim3(grid), dim3(256), 0, s, a); break;
+    // OCT_OPGEMM=classic selects the kernel that stages the data fragments through LDS (same
+    // results), for comparisons
+    const char* sel = std::getenv("OCT_OPGEMM");
+    if (sel && std::strcmp(sel, "classic") == 0) {
+        switch (raw_dtype) {
+            case OCT_RAW_I16: hipLaunchKernelGGL(oct::opgemm_kernel<int16_t>, dim3(grid), dim3(256), 0, s, a); break;
+            case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
+            default: hipLaunchKernelGGL(oct::opgemm_kernel<float>, dim3(grid), dim3(256), 0, s, a); break;
+        }
+    } else {
+        switch (raw_dtype) {
+            case OCT_RAW_I16: launch_opgemm_rf<int16_t>(a, grid, s); break;
+            case OCT_RAW_U16: launch_opgemm_rf<uint16_t>(a, grid, s); break;
+            default: launch_opgemm_rf<float>(a, grid, s); break;
+        }
     }
     HIP_TRY(hipGetLastError());
     return OCT_OK;

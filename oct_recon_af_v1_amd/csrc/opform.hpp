@@ -101,6 +101,7 @@ constexpr int kOpCols = 256;       // A-scans per workgroup (4 waves x 64)
 constexpr int kOpRows = 128;       // operator rows per workgroup (4 tiles of 32 = 64 z bins)
 constexpr int kOpK = 32;           // k per stage
 constexpr int kOpPad = 40;         // LDS row stride in halves (80 B)
+constexpr int kOpBgN = 2048;       // spectrum length of the background rows staged in LDS
 
 struct OpGemmArgs {
     const void* raw;
@@ -269,6 +270,212 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
     }
 
     // ---- epilogue: |X| for 8 bins per lane per tile
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        if (pc >= a.M) continue;
+        float* orow_out = a.out + pc * a.out_stride;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);         // 0..15
+                const int zl = (row0 / 32 + r) * 16 + m;
+                if (zl < a.nzc) {
+                    const float re = acc[r][c][v], im = acc[r][c][v + 8];
+                    orow_out[zl] = __builtin_sqrtf(re * re + im * im) * a.inv_scale;
+                }
+            }
+        }
+    }
+}
+
+// opgemm_kernel with the data fragments kept in registers.  Each lane owns one A-scan of its
+// wave's 64: it converts its own K step (raw - bg -> fp16 hi / lo, k 0..31 as four groups of 8),
+// and v_permlane32_swap turns the pair (k 8j..8j+7, k 8j+8..8j+15) into the B fragments of both
+// 32-column tiles: afterwards the first register holds, in lanes 0-31, k 8j.. of columns 0-31
+// and, in lanes 32-63, k 8j+8.. of columns 0-31 (from lanes 0-31); the second the same for
+// columns 32-63.  These are exactly opgemm_kernel's fragments, and the products run in the same
+// order, so the results are the same bits.  Only the operator rows go through LDS, double-
+// buffered with one workgroup barrier per K step; the next step's samples and operator rows are
+// loaded into registers during the current step's products.  kLdsBg (files of at least 256
+// A-scans, so a workgroup's A-scans lie in at most two files): the two background rows are
+// read once into LDS instead of by every lane at every step (half the vector-memory traffic).
+// LDS 2 x 20 KB (+ 16 KB).
+__device__ __forceinline__ void frag_swap(half8_t& x, half8_t& y) {
+    typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+    u32x4_t xv = __builtin_bit_cast(u32x4_t, x), yv = __builtin_bit_cast(u32x4_t, y);
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const auto r = __builtin_amdgcn_permlane32_swap(xv[d], yv[d], false, false);
+        xv[d] = r[0];
+        yv[d] = r[1];
+    }
+    x = __builtin_bit_cast(half8_t, xv);
+    y = __builtin_bit_cast(half8_t, yv);
+}
+
+template <typename RawT, bool kLdsBg>
+__global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
+    __shared__ __align__(16) _Float16 ahi[2][kOpRows * kOpPad];
+    __shared__ __align__(16) _Float16 alo[2][kOpRows * kOpPad];
+    __shared__ __align__(16) float bgs[kLdsBg ? 2 : 1][kLdsBg ? kOpBgN : 4];   // background rows of the block's files
+
+    const int nblk = a.row_blocks * a.col_blocks;
+    int rb, cb;
+    {
+        const int id = blockIdx.x;
+        const int xcd = id % 8;
+        const int slot = id / 8;
+        const int per_xcd = (nblk + 7) / 8;
+        const int lin = xcd * per_xcd + slot;
+        if (lin >= nblk) return;                                 // whole workgroup
+        cb = lin / a.row_blocks;
+        rb = lin - cb * a.row_blocks;
+    }
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const long long col0 = (long long)cb * kOpCols;
+    const int row0 = rb * kOpRows;
+    const int nk = a.N / kOpK;
+
+    const long long p = col0 + tid;                              // this lane's A-scan
+    const bool pvalid = p < a.M;
+    const RawT* rrow = nullptr;
+    const float* brow = nullptr;
+    const long long fb0 = col0 / a.nX;                           // first file of the block
+    int fsel = 0;                                                // kLdsBg: this lane's file - fb0 (0 or 1)
+    if (pvalid) {
+        const long long b = p / a.nX;
+        const long long x = p - b * a.nX;
+        rrow = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N;
+        brow = a.bg + b * a.N;
+        fsel = (int)(b - fb0);
+    }
+    if constexpr (kLdsBg) {
+        // nX >= kOpCols: the block's A-scans lie in files fb0 and fb0 + 1 (if present)
+        const long long nfiles = (a.M + a.nX - 1) / a.nX;
+        for (int e = tid; e < 2 * (kOpBgN / 4); e += 256) {
+            const int f = e / (kOpBgN / 4), q = e - f * (kOpBgN / 4);
+            float4 v = {0.f, 0.f, 0.f, 0.f};
+            if (fb0 + f < nfiles) v = *reinterpret_cast<const float4*>(a.bg + (fb0 + f) * a.N + 4 * q);
+            *reinterpret_cast<float4*>(&bgs[f][4 * q]) = v;
+        }
+    }
+    const int orow = tid >> 1;                                   // operator row staged by this thread
+    const int ohalf = (tid & 1) * 16;
+    const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
+    const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
+    constexpr int kRawVec = 16 / sizeof(RawT);
+    constexpr int kRawLd = kOpK / kRawVec;
+    uint4 rv[kRawLd];
+    half8_t oh[2], ol[2];
+    half8_t dh[kOpK / 8], dl[kOpK / 8];                          // this lane's step: k 8g .. 8g + 7
+    auto fetch = [&](int k0) {
+        if (pvalid) {
+#pragma unroll
+            for (int g = 0; g < kRawLd; ++g) rv[g] = *reinterpret_cast<const uint4*>(rrow + k0 + kRawVec * g);
+        }
+        const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
+        const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
+        oh[0] = sh[0];
+        oh[1] = sh[1];
+        ol[0] = sl[0];
+        ol[1] = sl[1];
+    };
+    auto convert = [&](int k0) {
+#pragma unroll
+        for (int g = 0; g < kOpK / 8; ++g) {
+            half8_t h = {}, l = {};
+            if (pvalid) {
+                float f[8];
+                if constexpr (sizeof(RawT) == 2) {
+                    unpack16<RawT>(rv[g], f);
+                } else {
+                    float f0[4], f1[4];
+                    unpack16<RawT>(rv[2 * g], f0);
+                    unpack16<RawT>(rv[2 * g + 1], f1);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        f[e] = f0[e];
+                        f[4 + e] = f1[e];
+                    }
+                }
+                float4 b0, b1;
+                if constexpr (kLdsBg) {
+                    b0 = *reinterpret_cast<const float4*>(&bgs[fsel][k0 + 8 * g]);
+                    b1 = *reinterpret_cast<const float4*>(&bgs[fsel][k0 + 8 * g + 4]);
+                } else {
+                    b0 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g);
+                    b1 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g + 4);
+                }
+                const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float d = f[e] - bb[e];
+                    const _Float16 hv = (_Float16)d;
+                    h[e] = hv;
+                    l[e] = (_Float16)(d - (float)hv);
+                }
+            }
+            dh[g] = h;
+            dl[g] = l;
+        }
+    };
+    auto stage_op = [&](int sb) {
+        half8_t* th = reinterpret_cast<half8_t*>(&ahi[sb][orow * kOpPad + ohalf]);
+        half8_t* tl = reinterpret_cast<half8_t*>(&alo[sb][orow * kOpPad + ohalf]);
+        th[0] = oh[0];
+        th[1] = oh[1];
+        tl[0] = ol[0];
+        tl[1] = ol[1];
+    };
+
+    f32x16_t acc[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[r][c][v] = 0.f;
+
+    fetch(0);
+    stage_op(0);
+    if constexpr (kLdsBg) __syncthreads();                       // background rows staged
+    convert(0);
+    if (nk > 1) fetch(kOpK);
+    __syncthreads();
+    for (int it = 0; it < nk; ++it) {
+        const int sb = it & 1;
+#pragma unroll
+        for (int ks = 0; ks < kOpK; ks += 16) {
+            const int kk = ks + 8 * (lane >> 5);
+            half8_t bh[2] = {dh[ks / 8], dh[ks / 8 + 1]};
+            half8_t bl[2] = {dl[ks / 8], dl[ks / 8 + 1]};
+            frag_swap(bh[0], bh[1]);
+            frag_swap(bl[0], bl[1]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = r * 32 + (lane & 31);
+                const half8_t ah = *reinterpret_cast<const half8_t*>(&ahi[sb][row * kOpPad + kk]);
+                const half8_t al = *reinterpret_cast<const half8_t*>(&alo[sb][row * kOpPad + kk]);
+#pragma unroll
+                for (int c = 0; c < 2; ++c) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[c], acc[r][c], 0, 0, 0);
+                }
+            }
+        }
+        if (it + 1 < nk) {                                       // stage step it + 1
+            stage_op(sb ^ 1);                                    // read by nobody in this step
+            convert((it + 1) * kOpK);
+            if (it + 2 < nk) fetch((it + 2) * kOpK);
+        }
+        __syncthreads();
+    }
+
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);

@@ -108,6 +108,14 @@ class Reconstructor:
         self._tile_mag = {}
         self._partial = None
         self._keep = None
+        # volume rows: each row's stitch runs on its own stream so the next row's spectral work
+        # overlaps it; the tile magnitudes alternate between two buffer sets by row parity
+        self.overlap_rows = True
+        self._stitch_stream = None
+        self._stitch_done = {}              # parity -> event after the last stitch reading that set
+        self._row_seq = 0
+        self._frames_dev = {}               # frame lists on the device (no upload per row)
+        self._table_pin = [None, None]      # pinned staging of device tile tables, alternating
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
 
     # ------------------------------------------------------------------ data access
@@ -255,13 +263,18 @@ class Reconstructor:
         return (self.cfg.form == "mfma" and self.concurrent_windows and self.nb == 1 and self.na == 1
                 and self.sp.ascan_binning == 1)
 
-    def _row_magnitudes(self, yi, frames, cancel=None):
-        """Spectral stage for every tile of row yi into per-tile buffers (nF, nX, rows)."""
+    def _row_magnitudes(self, yi, frames, cancel=None, parity=None):
+        """Spectral stage for every tile of row yi into per-tile buffers (nF, nX, rows).  With a
+        parity (0 / 1) the buffers are that parity's set, written only after the last stitch
+        that read the set has finished."""
         torch = self.torch
         frames = np.asarray(frames)
         out = []
         side = self._side_streams()
-        main = torch.cuda.current_stream(self.xp.device) if side else None
+        main = torch.cuda.current_stream(self.xp.device) if (side or parity is not None) else None
+        reuse = self._stitch_done.get(parity) if parity is not None else None
+        if reuse is not None and not side:
+            main.wait_event(reuse)
         start = None
         used = {}
         for (xi, zi) in self.scan.tiles_in_row(yi):
@@ -269,7 +282,7 @@ class Reconstructor:
             if st.empty:
                 continue
             lo, hi = self._tile_rows(st)
-            key = (xi, zi)
+            key = (xi, zi) if parity is None else (xi, zi, parity)
             buf = self._tile_mag.get(key)
             ctx = None
             if side:
@@ -281,6 +294,8 @@ class Reconstructor:
                     s_ = self._streams[(lo, hi)] = torch.cuda.Stream(device=self.xp.device)
                 if (lo, hi) not in used:
                     s_.wait_event(start)
+                    if reuse is not None:
+                        s_.wait_event(reuse)
                     used[(lo, hi)] = s_
                 ctx = torch.cuda.stream(s_)
                 ctx.__enter__()
@@ -310,6 +325,9 @@ class Reconstructor:
             nX = self._positions(raw.shape[1])
             shape = (nF, nX, hi - lo)
             if buf is None or tuple(buf.shape) != shape:
+                old = self._tile_mag.get(key)
+                if old is not None and self._stitch_stream is not None:
+                    old.record_stream(self._stitch_stream)   # a stitch may still read it
                 buf = self.torch.empty(shape, dtype=self.torch.float32, device=self.xp.device)
                 if main is not None:
                     buf.record_stream(main)
@@ -317,8 +335,10 @@ class Reconstructor:
             self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
         return buf
 
-    def _stitch_row(self, tiles, frames, db, plane0, partial):
-        """One fused stitch + finalise launch for a row (oct_stitch_row)."""
+    def _stitch_row(self, tiles, frames, db, plane0, partial, stream=None):
+        """One fused stitch + finalise launch for a row (oct_stitch_row), on the current stream
+        or, given `stream`, on that stream once the current stream's work so far is done (the
+        row's magnitudes and descriptors); returns the event recorded after it there."""
         _, nz_out, nx_out = self.n_out
         arr = (_lib.RowTile * max(1, len(tiles)))()
         nX = None
@@ -340,29 +360,74 @@ class Reconstructor:
             t.x_cols = st.x_cols.data_ptr()
             t.x_w = st.x_w.data_ptr()
             t.fz = st.fz.data_ptr()
-        fr = self.torch.as_tensor(np.asarray(frames, np.int32), device=self.xp.device)
+        fkey = np.asarray(frames, np.int32).tobytes()
+        fr = self._frames_dev.get(fkey)
+        if fr is None:
+            fr = self._frames_dev[fkey] = self.torch.as_tensor(np.asarray(frames, np.int32), device=self.xp.device)
         st0 = tiles[0][0] if tiles else None
         shift = vlo = vhi = None
         if st0 is not None and st0.shift is not None:
             if nX != st0.n_x_opc:
                 raise ValueError("optical-path correction grid does not match the tile's A-scans")
-            if int(fr.min()) < 0 or int(fr.max()) >= st0.n_y:
+            fa = np.asarray(frames)
+            if fa.size and (int(fa.min()) < 0 or int(fa.max()) >= st0.n_y):
                 raise ValueError("frame index outside the optical-path correction grid")
             shift, vlo, vhi = st0.shift.data_ptr(), st0.vlo.data_ptr(), st0.vhi.data_ptr()
         thr = float(np.float32(min_weight_threshold(np.float32)))
         table = None
         if len(tiles) > _ROW_ARG_TILES:              # descriptors through a device table
-            host = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
-            table = self.torch.from_numpy(host).to(self.xp.device)
+            table = self._upload_table(arr)
+        sh = self.xp.stream()
+        done = None
+        if stream is not None:
+            ready = self.torch.cuda.Event()
+            ready.record(self.torch.cuda.current_stream(self.xp.device))
+            stream.wait_event(ready)
+            sh = stream.cuda_stream
+            for t in (fr, table, db, partial):
+                if t is not None:
+                    t.record_stream(stream)
+        if table is not None:
             _lib.check(_lib.lib().oct_stitch_row_table(arr, table.data_ptr(), len(tiles), len(frames), nX or 1,
                                                        self.sp.nz, fr.data_ptr(), shift, vlo, vhi, nz_out, nx_out,
-                                                       thr, db.data_ptr(), plane0, partial.data_ptr(),
-                                                       self.xp.stream()), "oct_stitch_row_table")
+                                                       thr, db.data_ptr(), plane0, partial.data_ptr(), sh),
+                       "oct_stitch_row_table")
         else:
             _lib.check(_lib.lib().oct_stitch_row(arr, len(tiles), len(frames), nX or 1, self.sp.nz, fr.data_ptr(),
                                                  shift, vlo, vhi, nz_out, nx_out, thr, db.data_ptr(), plane0,
-                                                 partial.data_ptr(), self.xp.stream()), "oct_stitch_row")
+                                                 partial.data_ptr(), sh), "oct_stitch_row")
+        if stream is not None:
+            done = self.torch.cuda.Event()
+            done.record(stream)
         self._keep = (arr, fr, table)
+        return done
+
+    def _upload_table(self, arr):
+        """Tile descriptors -> a new device table, copied asynchronously on the current stream
+        from one of two pinned staging buffers (one is refilled only after its last copy ended)."""
+        torch = self.torch
+        nbytes = ctypes.sizeof(arr)
+        slot = self._row_seq % 2
+        pin, ev = self._table_pin[slot] or (None, None)
+        if pin is None or pin.numel() < nbytes:
+            pin = torch.empty(max(nbytes, 4096), dtype=torch.uint8, pin_memory=True)
+        elif ev is not None:
+            ev.synchronize()
+        ctypes.memmove(pin.data_ptr(), ctypes.addressof(arr), nbytes)
+        table = torch.empty(nbytes, dtype=torch.uint8, device=self.xp.device)
+        table.copy_(pin[:nbytes], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.xp.device))
+        self._table_pin[slot] = (pin, ev)
+        return table
+
+    def _join_stitches(self):
+        """The current stream waits for every row stitch issued so far (the stitch stream runs
+        them in order, so for its last one)."""
+        if self._stitch_stream is not None:
+            last = self.torch.cuda.Event()
+            last.record(self._stitch_stream)
+            self.torch.cuda.current_stream(self.xp.device).wait_event(last)
 
     def _row_partials(self, n_planes):
         _, nz_out, nx_out = self.n_out
@@ -386,6 +451,7 @@ class Reconstructor:
         _, nz_out, nx_out = self.n_out
         try:
             if self.cfg.fused_rows:
+                self._join_stitches()              # volume rows may still use the partials
                 db = self.xp.empty((len(frames), nz_out, nx_out))
                 partial, nb = self._row_partials(len(frames))
                 self._stitch_row(self._row_magnitudes(yi, frames, cancel), frames, db, 0, partial)
@@ -433,13 +499,24 @@ class Reconstructor:
         n_planes = len(vol["rows"]) * ny
         partial, _ = self._row_partials(n_planes)
         frames = np.arange(ny)
-        self._stitch_row(self._row_magnitudes(vol["rows"][k], frames, cancel), frames, vol["db"], k * ny, partial)
+        if not self.overlap_rows:
+            self._stitch_row(self._row_magnitudes(vol["rows"][k], frames, cancel), frames, vol["db"], k * ny, partial)
+            return
+        parity = self._row_seq % 2
+        self._row_seq += 1
+        if self._stitch_stream is None:
+            self._stitch_stream = self.torch.cuda.Stream(device=self.xp.device)
+        tiles = self._row_magnitudes(vol["rows"][k], frames, cancel, parity=parity)
+        self._stitch_done[parity] = self._stitch_row(tiles, frames, vol["db"], k * ny, partial,
+                                                     stream=self._stitch_stream)
 
     def finish_volume(self, vol):
-        """Clims and codes of a volume whose rows were all made by volume_row."""
+        """Clims and codes of a volume whose rows were all made by volume_row (vol["db"] is
+        complete, on the current stream, from here on)."""
         ny = self.scan.n_y_px
         n_planes = len(vol["rows"]) * ny
         partial, nb = self._row_partials(n_planes)
+        self._join_stitches()
         self._restore_windows()
         pcl, cl = self._clim(partial, n_planes, nb)
         self.fin.quantize(vol["db"], cl, pcl, legacy_double=self.cfg.legacy_double_quantization, out=vol["bits"])

@@ -82,3 +82,22 @@ def test_opform_crop_and_dtypes(xp):
     assert np.abs(full - o).max() <= 1e-5 * o.max()
     with pytest.raises(ValueError):
         sp.run_operator(xp.asarray(raw.astype(np.float32)))
+
+
+@pytest.mark.parametrize("dtype", ["uint16", "int16"])
+def test_register_fragment_gemm_equals_lds_fragment_gemm(xp, dtype, monkeypatch):
+    """The GEMM that forms the data fragments in registers (v_permlane32_swap) gives the bits of
+    the one that stages them through LDS (OCT_OPGEMM=classic): same fragments, same products in
+    the same order per accumulator.  Ragged batch, cropped and full operators."""
+    from oct_recon_af_v1_amd import synthetic as SY
+    print(f"case: register vs LDS fragments ({dtype})", flush=True)
+    lam, bref, sp = _setup(xp, 6e-5)
+    raw = SY.make_bscans(3, 301, seed=9).numpy().astype(dtype)
+    d = xp.asarray(raw)
+    out = {}
+    for sel in ("classic", "default"):
+        monkeypatch.setenv("OCT_OPGEMM", sel)
+        out[sel] = (sp.run_operator(d).cpu().numpy(), sp.run_operator(d, 211, 70).cpu().numpy())
+    monkeypatch.delenv("OCT_OPGEMM")
+    for a, b in zip(out["classic"], out["default"]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
