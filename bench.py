@@ -406,7 +406,7 @@ def main():
             "rows": [ulo, uhi], "rows_per_tile": out_rows, "operator_rows_padded": rpad,
             "operator_scale_log2": op.scale_log2,
             "roofline": {"bound": "mfma", "achieved": tfl, "peak": 2500.0, "unit": "TFLOP/s", "frac": tfl / 2500.0,
-                         "kernel": "opgemm_kernel<uint16_t>", "flops_per_ascan": 3 * 2 * rpad * 2048,
+                         "kernel": "opgemm_rf_kernel<uint16_t,true>", "flops_per_ascan": 3 * 2 * rpad * 2048,
                          "avg_launch_ms": avg_m, "ascans_per_launch": per_m, "busy_ms_per_step": busy_ms,
                          "launches_overlap": True,
                          "hbm_GBps": ascans * (2 * 2048 + 4 * out_rows) / (busy_ms * 1e-3) / 1e9},
