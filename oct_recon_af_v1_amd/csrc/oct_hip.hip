@@ -99,11 +99,17 @@ void launch_opgemm_rf(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
 template <typename RawT>
 int launch_background(const void* raw, float* bg, int B, int lines, int N, int apod, int mode, int group,
                       hipStream_t s) {
-    const long long n = (long long)B * N;
     const int threads = 256;
-    const long long blocks = (n + threads - 1) / threads;
-    hipLaunchKernelGGL(oct::background_kernel<RawT>, dim3((unsigned)blocks), dim3(threads), 0, s,
-                       reinterpret_cast<const RawT*>(raw), bg, B, lines, N, apod, mode, group);
+    constexpr int VEC = 16 / sizeof(RawT);
+    if (mode == 0 && N % VEC == 0 && (reinterpret_cast<uintptr_t>(bg) & 15) == 0) {
+        const long long n = (long long)B * (N / VEC);
+        hipLaunchKernelGGL(oct::background_lines_kernel<RawT>, dim3((unsigned)((n + threads - 1) / threads)),
+                           dim3(threads), 0, s, reinterpret_cast<const RawT*>(raw), bg, B, lines, N, apod);
+    } else {
+        const long long n = (long long)B * N;
+        hipLaunchKernelGGL(oct::background_kernel<RawT>, dim3((unsigned)((n + threads - 1) / threads)), dim3(threads),
+                           0, s, reinterpret_cast<const RawT*>(raw), bg, B, lines, N, apod, mode, group);
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

@@ -412,4 +412,36 @@ __global__ void background_kernel(const RawT* __restrict__ raw, float* __restric
     }
 }
 
+// Mode 0 of background_kernel with 16-byte loads: one thread per 16 / sizeof(RawT) consecutive
+// samples of a file, the line loop unrolled so that several lines' loads are in flight.  Same
+// float64 sums in line order, so the same values.  Needs N % (16 / sizeof(RawT)) == 0 and 16-byte
+// aligned raw and bg.
+template <typename RawT>
+__global__ __launch_bounds__(256) void background_lines_kernel(const RawT* __restrict__ raw, float* __restrict__ bg,
+                                                               int B, int lines, int N, int apod) {
+    constexpr int VEC = 16 / sizeof(RawT);
+    const int nv = N / VEC;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)B * nv) return;
+    const int b = (int)(t / nv);
+    const int j0 = (int)(t - (long long)b * nv) * VEC;
+    const RawT* r = raw + (long long)b * lines * N + j0;
+    double s[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) s[e] = 0.0;
+#pragma unroll 5
+    for (int l = 0; l < apod; ++l) {
+        float f[VEC];
+        unpack16<RawT>(*reinterpret_cast<const uint4*>(r + (long long)l * N), f);
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) s[e] += (double)f[e];
+    }
+    float o[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) o[e] = (float)(s[e] / apod);
+    float* dst = bg + (long long)b * N + j0;
+#pragma unroll
+    for (int e = 0; e < VEC; e += 4) *reinterpret_cast<float4*>(dst + e) = float4{o[e], o[e + 1], o[e + 2], o[e + 3]};
+}
+
 }  // namespace oct

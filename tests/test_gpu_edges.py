@@ -115,3 +115,20 @@ def test_rejected_arguments_leave_device_usable(xp):
     o = R.magnitude_fp64(raw.numpy(), 25, R.spectral_geometry(lam, bref, 1.33))
     g = sp.run_operator(raw.to(xp.device)).cpu().numpy()
     assert np.abs(g - o).max() <= 1e-5 * o.max()
+
+
+@pytest.mark.parametrize("dtype", ["uint16", "int16", "float32"])
+def test_background_rows_exact(xp, dtype):
+    """Background rows ("lines" mode: mean of the first apod A-lines in float64, stored float32)
+    equal numpy's: integer sums are exact in float64; float32 raw is summed in line order."""
+    print(f"case: background rows ({dtype})", flush=True)
+    torch = xp.torch
+    _, _, sp = _setup(xp, 0.0)
+    rng = np.random.default_rng(8)
+    raw = rng.integers(0, 4096, (5, 40, 2048)).astype(dtype)
+    if dtype == "float32":
+        raw = raw + rng.random(raw.shape, dtype=np.float32)
+    got = sp.background(xp.asarray(raw)).cpu().numpy()
+    ref = (np.add.accumulate(raw[:, :25].astype(np.float64), axis=1)[:, -1] / 25).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
