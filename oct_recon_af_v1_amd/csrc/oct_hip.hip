@@ -504,7 +504,7 @@ static int stitch_row_impl(const oct_row_tile* tiles, const oct_row_tile* dev_ti
     a.blocks_per_plane = oct_stitch_row_blocks(n_out_z, n_out_x);
     if (n_frames > 65535) return fail(OCT_EINVAL, "%d planes in one call (at most 65535)", n_frames);
     a.nplanes = n_frames;
-    const long long total = (long long)a.blocks_per_plane * n_frames;
+    const long long total = (long long)a.blocks_per_plane * ((n_frames + oct::kRowPlanes - 1) / oct::kRowPlanes);
     if (total > (1LL << 30)) return fail(OCT_EINVAL, "too many planes in one call");
     dim3 grid((unsigned)((total + 7) / 8 * 8));            // whole multiple of the 8 XCDs
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -132,8 +132,15 @@ constexpr int kRowPasses = kRowC / (kRowWaves * kRowLanesC);  // column passes p
 
 // kShift: optical-path shift present (kernels without it carry no per-x lookups);
 // kTable: tile descriptors read from the device table a.tt instead of the argument block.
+// A workgroup makes its block of kRowPlanes consecutive planes: the tap loads of a tile (depth
+// rows and weights, focus factors, column taps) are made once for all of them, and the planes'
+// magnitude loads are independent, so each dependent load round has kRowPlanes times as many
+// loads in flight.  Per voxel the summation order is unchanged.
+constexpr int kRowPlanes = 4;
+
 template <bool kShift, bool kTable>
 __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
+    constexpr int P = kRowPlanes;
     __shared__ float tile[kRowZ][kRowC + 1];
     __shared__ int smin[kRowWaves], smax[kRowWaves];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -141,20 +148,26 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
     // on XCD i % 8.  Each XCD takes a contiguous range of the work list, ordered depth block
     // fastest, so the depth blocks of one column range share that XCD's L2.
     const int gx = (a.nOutX + kRowC - 1) / kRowC, gz = (a.nOutZ + kRowZ - 1) / kRowZ;
-    const long long total = (long long)gx * gz * a.nplanes;
+    const int ngroups = (a.nplanes + P - 1) / P;
+    const long long total = (long long)gx * gz * ngroups;
     const long long per_xcd = (long long)gridDim.x / 8;
     const long long w = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (w >= total) return;                                // uniform per workgroup
     const int zb = (int)(w % gz);
     const int xb = (int)((w / gz) % gx);
-    const int f = (int)(w / ((long long)gz * gx));
+    const int f0 = (int)(w / ((long long)gz * gx)) * P;
+    const int np = min(P, a.nplanes - f0);                 // planes of this block (uniform)
     const int zo = zb * kRowZ + (lane % kRowZ);
     const int cb0 = xb * kRowC;
-    const int fb = a.frame_idx[f];
     const bool zok = zo < a.nOutZ;
-    float num[kRowPasses], den[kRowPasses];
+    int fp[P];                                             // plane -> frame (clamped past the last)
 #pragma unroll
-    for (int q = 0; q < kRowPasses; ++q) num[q] = den[q] = 0.f;
+    for (int p = 0; p < P; ++p) fp[p] = f0 + min(p, np - 1);
+    float num[kRowPasses][P], den[kRowPasses][P];
+#pragma unroll
+    for (int q = 0; q < kRowPasses; ++q)
+#pragma unroll
+        for (int p = 0; p < P; ++p) num[q][p] = den[q][p] = 0.f;
     // Tiles outer (in order), columns inner: per voxel the same summation order as the
     // per-tile path (tile, then x tap, then z tap).  The body is branch-free so that the loads
     // of all columns issue together: taps outside a tile or of zero weight read an in-range
@@ -167,7 +180,7 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
         const float2 ww = reinterpret_cast<const float2*>(t.zw)[zc];
         const int r0 = rr.x, r1 = rr.y;
         const float wz0 = zok ? ww.x : 0.f, wz1 = zok ? ww.y : 0.f;
-        const float f0 = t.fz[min(max(r0, 0), a.nZ - 1)], f1 = t.fz[min(max(r1, 0), a.nZ - 1)];
+        const float f0w = t.fz[min(max(r0, 0), a.nZ - 1)], f1w = t.fz[min(max(r1, 0), a.nZ - 1)];
         const int slo = max(t.mrow0, 0), shi = min(t.mrow0 + (int)t.mstride, a.nZ) - 1;   // computed rows
 #pragma unroll
         for (int q = 0; q < kRowPasses; ++q) {
@@ -181,65 +194,74 @@ __global__ __launch_bounds__(64 * kRowWaves) void stitch_row_kernel(RowArgs a) {
             for (int tx = 0; tx < 2; ++tx) {
                 const float wx = in ? (tx ? xw.y : xw.x) : 0.f;
                 const int x = tx ? xr.y : xr.x;
-                int sh = 0, lo = 0, hi = a.nZ - 1;
-                if constexpr (kShift) {
-                    const long long o = (long long)fb * a.nX + x;
-                    sh = a.shift[o];
-                    lo = a.vlo[o];
-                    hi = a.vhi[o];
-                }
-                const float* mrow = t.mag + ((long long)f * a.nX + x) * t.mstride - t.mrow0;
-                const float fw0 = (r0 >= lo && r0 <= hi) ? f0 : 0.f;
-                const float fw1 = (r1 >= lo && r1 <= hi) ? f1 : 0.f;
-                const int s0 = min(max(min(max(r0 + sh, 0), a.nZ - 1), slo), shi);
-                const int s1 = min(max(min(max(r1 + sh, 0), a.nZ - 1), slo), shi);
-                float sa = __builtin_fmaf(mrow[s0] * fw0, wz0, 0.f);
-                float sf = __builtin_fmaf(fw0, wz0, 0.f);
-                sa = __builtin_fmaf(mrow[s1] * fw1, wz1, sa);
-                sf = __builtin_fmaf(fw1, wz1, sf);
-                num[q] = __builtin_fmaf(sa, wx, num[q]);
-                den[q] = __builtin_fmaf(sf, wx, den[q]);
-            }
-        }
-    }
-    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
 #pragma unroll
-    for (int q = 0; q < kRowPasses; ++q) {
-        const int cl0 = (q * kRowWaves + wave) * kRowLanesC + lane / kRowZ;
-        if (zok && cb0 + cl0 < a.nOutX) {
-            const float v = (den[q] >= a.thr) ? 20.0f * log10f(num[q] / den[q]) : __builtin_nanf("");
-            tile[lane % kRowZ][cl0] = v;
-            if (__builtin_isfinite(v)) {
-                const int kk = fkey(v);
-                kmin = min(kmin, kk);
-                kmax = max(kmax, kk);
+                for (int p = 0; p < P; ++p) {
+                    int sh = 0, lo = 0, hi = a.nZ - 1;
+                    if constexpr (kShift) {
+                        const long long o = (long long)a.frame_idx[fp[p]] * a.nX + x;
+                        sh = a.shift[o];
+                        lo = a.vlo[o];
+                        hi = a.vhi[o];
+                    }
+                    const float* mrow = t.mag + ((long long)fp[p] * a.nX + x) * t.mstride - t.mrow0;
+                    const float fw0 = (r0 >= lo && r0 <= hi) ? f0w : 0.f;
+                    const float fw1 = (r1 >= lo && r1 <= hi) ? f1w : 0.f;
+                    const int s0 = min(max(min(max(r0 + sh, 0), a.nZ - 1), slo), shi);
+                    const int s1 = min(max(min(max(r1 + sh, 0), a.nZ - 1), slo), shi);
+                    float sa = __builtin_fmaf(mrow[s0] * fw0, wz0, 0.f);
+                    float sf = __builtin_fmaf(fw0, wz0, 0.f);
+                    sa = __builtin_fmaf(mrow[s1] * fw1, wz1, sa);
+                    sf = __builtin_fmaf(fw1, wz1, sf);
+                    num[q][p] = __builtin_fmaf(sa, wx, num[q][p]);
+                    den[q][p] = __builtin_fmaf(sf, wx, den[q][p]);
+                }
             }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        kmin = min(kmin, __shfl_xor(kmin, off));
-        kmax = max(kmax, __shfl_xor(kmax, off));
-    }
-    if (lane == 0) {
-        smin[wave] = kmin;
-        smax[wave] = kmax;
-    }
-    __syncthreads();
-    // coalesced stores: consecutive threads write consecutive columns of one output row
-    for (int e = threadIdx.x; e < kRowZ * kRowC; e += 64 * kRowWaves) {
-        const int zr = e / kRowC, cc = e - zr * kRowC;
-        const int z = zb * kRowZ + zr, c = cb0 + cc;
-        if (z < a.nOutZ && c < a.nOutX) a.db[((long long)(a.plane0 + f) * a.nOutZ + z) * a.nOutX + c] = tile[zr][cc];
-    }
-    if (threadIdx.x == 0) {
-        int mn = smin[0], mx = smax[0];
-        for (int w = 1; w < kRowWaves; ++w) {
-            mn = min(mn, smin[w]);
-            mx = max(mx, smax[w]);
+    // finalise the block's planes one at a time through the LDS tile (coalesced stores)
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (p >= np) break;                                // uniform per workgroup
+        int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+#pragma unroll
+        for (int q = 0; q < kRowPasses; ++q) {
+            const int cl0 = (q * kRowWaves + wave) * kRowLanesC + lane / kRowZ;
+            if (zok && cb0 + cl0 < a.nOutX) {
+                const float v = (den[q][p] >= a.thr) ? 20.0f * log10f(num[q][p] / den[q][p]) : __builtin_nanf("");
+                tile[lane % kRowZ][cl0] = v;
+                if (__builtin_isfinite(v)) {
+                    const int kk = fkey(v);
+                    kmin = min(kmin, kk);
+                    kmax = max(kmax, kk);
+                }
+            }
         }
-        const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + (long long)zb * gx + xb) * 2;
-        a.partial[o] = mn;
-        a.partial[o + 1] = mx;
+        for (int off = 32; off > 0; off >>= 1) {
+            kmin = min(kmin, __shfl_xor(kmin, off));
+            kmax = max(kmax, __shfl_xor(kmax, off));
+        }
+        if (lane == 0) {
+            smin[wave] = kmin;
+            smax[wave] = kmax;
+        }
+        __syncthreads();
+        const int f = f0 + p;
+        for (int e = threadIdx.x; e < kRowZ * kRowC; e += 64 * kRowWaves) {
+            const int zr = e / kRowC, cc = e - zr * kRowC;
+            const int z = zb * kRowZ + zr, c = cb0 + cc;
+            if (z < a.nOutZ && c < a.nOutX) a.db[((long long)(a.plane0 + f) * a.nOutZ + z) * a.nOutX + c] = tile[zr][cc];
+        }
+        if (threadIdx.x == 0) {
+            int mn = smin[0], mx = smax[0];
+            for (int w2 = 1; w2 < kRowWaves; ++w2) {
+                mn = min(mn, smin[w2]);
+                mx = max(mx, smax[w2]);
+            }
+            const long long o = ((long long)(a.plane0 + f) * a.blocks_per_plane + (long long)zb * gx + xb) * 2;
+            a.partial[o] = mn;
+            a.partial[o + 1] = mx;
+        }
+        __syncthreads();                                   // tile and smin reused by the next plane
     }
 }
 __device__ __forceinline__ int fkey(float f) {
