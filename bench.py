@@ -390,7 +390,8 @@ def main():
         within = float((d <= 1).float().mean().item())
         # one operator per focus depth's row window; flops and output rows averaged over tiles
         wins = [rec._tile_rows(st) for st in rec.stitchers.values() if not st.empty]
-        rpads = [((2 * ((hi - lo + 15) // 16 * 16) + 127) // 128) * 128 for lo, hi in wins]
+        rpads = [2 * ((hi - lo + 15) // 16 * 16) if 2 * ((hi - lo + 15) // 16 * 16) <= 128
+                 else ((2 * ((hi - lo + 15) // 16 * 16) + 127) // 128) * 128 for lo, hi in wins]
         rpad = float(np.mean(rpads))
         out_rows = float(np.mean([hi - lo for lo, hi in wins]))
         op = rec.sp.operator_plan(*((wins[0][0], wins[0][1] - wins[0][0])))

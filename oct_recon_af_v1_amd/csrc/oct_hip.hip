@@ -87,13 +87,23 @@ int launch_spectral(const oct_plan* p, const oct::SpectralArgs& a, int kind, hip
     return launch_spectral_w<RawT, 16, 8>(p, a, kind, s);
 }
 
-template <typename RawT>
-void launch_opgemm_rf(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
+template <typename RawT, int RT>
+void launch_opgemm_rf_rt(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
     // background rows through LDS when a workgroup's A-scans lie in at most two files
     if (a.nX >= oct::kOpCols && a.N == oct::kOpBgN)
-        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, true>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, true, RT>), dim3(grid), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, false>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, false, RT>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <typename RawT>
+void launch_opgemm_rf(const oct::OpGemmArgs& a, int rt, int grid, hipStream_t s) {
+    switch (rt) {
+        case 1: launch_opgemm_rf_rt<RawT, 1>(a, grid, s); break;
+        case 2: launch_opgemm_rf_rt<RawT, 2>(a, grid, s); break;
+        case 3: launch_opgemm_rf_rt<RawT, 3>(a, grid, s); break;
+        default: launch_opgemm_rf_rt<RawT, 4>(a, grid, s); break;
+    }
 }
 
 template <typename RawT>
@@ -737,7 +747,10 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     const double bound = amax_win / N * (colsum + usum / N);
     const int sa = (int)std::floor(std::log2(16384.0 / bound));
     const int nzc_pad = (nzc + 15) / 16 * 16;
-    const int r_pad = ((2 * nzc_pad + oct::kOpRows - 1) / oct::kOpRows) * oct::kOpRows;
+    // rows: 16 bins' real then imaginary parts per block of 32; up to 128 rows unpadded (the
+    // GEMM takes 1-4 row tiles per workgroup), beyond that a multiple of 128
+    const int r_pad = 2 * nzc_pad <= oct::kOpRows ? 2 * nzc_pad
+                                                  : ((2 * nzc_pad + oct::kOpRows - 1) / oct::kOpRows) * oct::kOpRows;
 
     int prev = 0;
     HIP_TRY(hipGetDevice(&prev));
@@ -854,7 +867,8 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     a.nX = nX;
     a.N = p->n;
     a.nzc = p->nzc;
-    a.row_blocks = p->r_pad / oct::kOpRows;
+    const int rt = std::min(4, p->r_pad / 32);                // row tiles of 32 per workgroup
+    a.row_blocks = p->r_pad / (32 * rt);
     a.col_blocks = (int)((a.M + oct::kOpCols - 1) / oct::kOpCols);
     a.out_stride = out_stride;
     a.inv_scale = (float)std::ldexp(1.0, -p->sa);
@@ -865,7 +879,7 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     // OCT_OPGEMM=classic selects the kernel that stages the data fragments through LDS (same
     // results), for comparisons
     const char* sel = std::getenv("OCT_OPGEMM");
-    if (sel && std::strcmp(sel, "classic") == 0) {
+    if (sel && std::strcmp(sel, "classic") == 0 && p->r_pad % oct::kOpRows == 0) {
         switch (raw_dtype) {
             case OCT_RAW_I16: hipLaunchKernelGGL(oct::opgemm_kernel<int16_t>, dim3(grid), dim3(256), 0, s, a); break;
             case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
@@ -873,9 +887,9 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
         }
     } else {
         switch (raw_dtype) {
-            case OCT_RAW_I16: launch_opgemm_rf<int16_t>(a, grid, s); break;
-            case OCT_RAW_U16: launch_opgemm_rf<uint16_t>(a, grid, s); break;
-            default: launch_opgemm_rf<float>(a, grid, s); break;
+            case OCT_RAW_I16: launch_opgemm_rf<int16_t>(a, rt, grid, s); break;
+            case OCT_RAW_U16: launch_opgemm_rf<uint16_t>(a, rt, grid, s); break;
+            default: launch_opgemm_rf<float>(a, rt, grid, s); break;
         }
     }
     HIP_TRY(hipGetLastError());

@@ -298,7 +298,8 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
 // columns 32-63.  These are exactly opgemm_kernel's fragments, and the products run in the same
 // order, so the results are the same bits.  Only the operator rows go through LDS, double-
 // buffered with one workgroup barrier per K step; the next step's samples and operator rows are
-// loaded into registers during the current step's products.  kLdsBg (files of at least 256
+// loaded into registers during the current step's products.  RT row tiles of 32 per workgroup
+// (1-4: operators of up to 128 rows are not padded to a whole 128).  kLdsBg (files of at least 256
 // A-scans, so a workgroup's A-scans lie in at most two files): the two background rows are
 // read once into LDS instead of by every lane at every step (half the vector-memory traffic).
 // LDS 2 x 20 KB (+ 16 KB).
@@ -315,10 +316,11 @@ __device__ __forceinline__ void frag_swap(half8_t& x, half8_t& y) {
     y = __builtin_bit_cast(half8_t, yv);
 }
 
-template <typename RawT, bool kLdsBg>
+template <typename RawT, bool kLdsBg, int RT>
 __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
-    __shared__ __align__(16) _Float16 ahi[2][kOpRows * kOpPad];
-    __shared__ __align__(16) _Float16 alo[2][kOpRows * kOpPad];
+    constexpr int kR = 32 * RT;                                  // operator rows per workgroup
+    __shared__ __align__(16) _Float16 ahi[2][kR * kOpPad];
+    __shared__ __align__(16) _Float16 alo[2][kR * kOpPad];
     __shared__ __align__(16) float bgs[kLdsBg ? 2 : 1][kLdsBg ? kOpBgN : 4];   // background rows of the block's files
 
     const int nblk = a.row_blocks * a.col_blocks;
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     const int lane = tid & 63;
     const int wave = tid >> 6;
     const long long col0 = (long long)cb * kOpCols;
-    const int row0 = rb * kOpRows;
+    const int row0 = rb * kR;
     const int nk = a.N / kOpK;
 
     const long long p = col0 + tid;                              // this lane's A-scan
@@ -363,7 +365,8 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
             *reinterpret_cast<float4*>(&bgs[f][4 * q]) = v;
         }
     }
-    const int orow = tid >> 1;                                   // operator row staged by this thread
+    const bool ostage = tid < 2 * kR;                            // stages operator row tid / 2 (whole waves)
+    const int orow = ostage ? tid >> 1 : 0;
     const int ohalf = (tid & 1) * 16;
     const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
     const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
@@ -377,12 +380,14 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
 #pragma unroll
             for (int g = 0; g < kRawLd; ++g) rv[g] = *reinterpret_cast<const uint4*>(rrow + k0 + kRawVec * g);
         }
-        const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
-        const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
-        oh[0] = sh[0];
-        oh[1] = sh[1];
-        ol[0] = sl[0];
-        ol[1] = sl[1];
+        if (ostage) {
+            const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
+            const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
+            oh[0] = sh[0];
+            oh[1] = sh[1];
+            ol[0] = sl[0];
+            ol[1] = sl[1];
+        }
     };
     auto convert = [&](int k0) {
 #pragma unroll
@@ -424,6 +429,7 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
         }
     };
     auto stage_op = [&](int sb) {
+        if (!ostage) return;
         half8_t* th = reinterpret_cast<half8_t*>(&ahi[sb][orow * kOpPad + ohalf]);
         half8_t* tl = reinterpret_cast<half8_t*>(&alo[sb][orow * kOpPad + ohalf]);
         th[0] = oh[0];
@@ -432,9 +438,9 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
         tl[1] = ol[1];
     };
 
-    f32x16_t acc[4][2];
+    f32x16_t acc[RT][2];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
@@ -456,7 +462,7 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
             frag_swap(bh[0], bh[1]);
             frag_swap(bl[0], bl[1]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
+            for (int r = 0; r < RT; ++r) {
                 const int row = r * 32 + (lane & 31);
                 const half8_t ah = *reinterpret_cast<const half8_t*>(&ahi[sb][row * kOpPad + kk]);
                 const half8_t al = *reinterpret_cast<const half8_t*>(&alo[sb][row * kOpPad + kk]);
@@ -482,7 +488,7 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
         if (pc >= a.M) continue;
         float* orow_out = a.out + pc * a.out_stride;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < RT; ++r) {
 #pragma unroll
             for (int v = 0; v < 8; ++v) {
                 const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);         // 0..15

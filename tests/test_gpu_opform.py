@@ -101,3 +101,20 @@ def test_register_fragment_gemm_equals_lds_fragment_gemm(xp, dtype, monkeypatch)
     monkeypatch.delenv("OCT_OPGEMM")
     for a, b in zip(out["classic"], out["default"]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("z0,nzc", [(400, 10), (180, 20), (300, 33), (512, 64), (211, 70)])
+def test_narrow_operators_vs_oracle(xp, z0, nzc):
+    """Operators of 32, 64, 96 and 128 rows are applied with 1-4 row tiles per workgroup (no
+    padding to 128 rows); 70 bins take two 128-row blocks.  Each within 1e-5 of the largest
+    float64 oracle value on its bins."""
+    from oct_recon_af_v1_amd import synthetic as SY
+    print(f"case: operator bins [{z0}, {z0 + nzc})", flush=True)
+    lam, bref, sp = _setup(xp, -3e-5)
+    raw = SY.make_bscans(2, 280, seed=12).numpy()
+    geo = R.spectral_geometry(lam, bref, 1.33)
+    o = R.magnitude_fp64(raw, 25, geo)
+    g = sp.run_operator(xp.asarray(raw), z0, nzc).cpu().numpy()
+    assert g.shape == (2, 280, nzc)
+    assert np.abs(g - o[..., z0:z0 + nzc]).max() <= 1e-5 * o.max()
+
