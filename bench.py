@@ -46,6 +46,14 @@ def profiled_value(name: str, key: str = "traffic_bytes_per_ascan"):
     return None
 
 
+def operator_rows(nzc: int) -> int:
+    """Rows of the fp16 operator the GEMM applies for nzc bins: 32-row tiles of 16 bins (real,
+    imaginary), grouped 1-4 per workgroup as oct_opplan_create groups them."""
+    tiles = 2 * ((nzc + 15) // 16 * 16) // 32
+    rt = tiles if tiles <= 4 else (3 if (tiles + 2) // 3 * 3 < (tiles + 3) // 4 * 4 else 4)
+    return (tiles + rt - 1) // rt * rt * 32
+
+
 # VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
 VALU_PEAK_GIPS = 256 * 4 * 2.4 / 4
 
@@ -390,8 +398,7 @@ def main():
         within = float((d <= 1).float().mean().item())
         # one operator per focus depth's row window; flops and output rows averaged over tiles
         wins = [rec._tile_rows(st) for st in rec.stitchers.values() if not st.empty]
-        rpads = [2 * ((hi - lo + 15) // 16 * 16) if 2 * ((hi - lo + 15) // 16 * 16) <= 128
-                 else ((2 * ((hi - lo + 15) // 16 * 16) + 127) // 128) * 128 for lo, hi in wins]
+        rpads = [operator_rows(hi - lo) for lo, hi in wins]
         rpad = float(np.mean(rpads))
         out_rows = float(np.mean([hi - lo for lo, hi in wins]))
         op = rec.sp.operator_plan(*((wins[0][0], wins[0][1] - wins[0][0])))

@@ -684,7 +684,7 @@ int oct_dispersion_windows(const double* betas, const float* head, int head_stri
 struct oct_opplan {
     int device;
     int n;
-    int z0, nzc, r_pad, nzc_pad;
+    int z0, nzc, r_pad, nzc_pad, rt;   // rt: operator row tiles of 32 per GEMM workgroup
     int sa;
     int tt;                       // transposed-band width
     double max_win;               // largest |window| of the creation window (sets sa)
@@ -747,10 +747,11 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     const double bound = amax_win / N * (colsum + usum / N);
     const int sa = (int)std::floor(std::log2(16384.0 / bound));
     const int nzc_pad = (nzc + 15) / 16 * 16;
-    // rows: 16 bins' real then imaginary parts per block of 32; up to 128 rows unpadded (the
-    // GEMM takes 1-4 row tiles per workgroup), beyond that a multiple of 128
-    const int r_pad = 2 * nzc_pad <= oct::kOpRows ? 2 * nzc_pad
-                                                  : ((2 * nzc_pad + oct::kOpRows - 1) / oct::kOpRows) * oct::kOpRows;
+    // rows: 16 bins' real then imaginary parts per tile of 32 rows.  The GEMM takes rt = 1-4 row
+    // tiles per workgroup: all of them up to 4 tiles, else 4, or 3 where that pads fewer tiles
+    const int tiles = 2 * nzc_pad / 32;
+    const int rt = tiles <= 4 ? tiles : ((tiles + 2) / 3 * 3 < (tiles + 3) / 4 * 4 ? 3 : 4);
+    const int r_pad = (tiles + rt - 1) / rt * rt * 32;
 
     int prev = 0;
     HIP_TRY(hipGetDevice(&prev));
@@ -761,6 +762,7 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     p->z0 = z0;
     p->nzc = nzc;
     p->r_pad = r_pad;
+    p->rt = rt;
     p->nzc_pad = nzc_pad;
     p->sa = sa;
     p->tt = TT;
@@ -867,7 +869,7 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     a.nX = nX;
     a.N = p->n;
     a.nzc = p->nzc;
-    const int rt = std::min(4, p->r_pad / 32);                // row tiles of 32 per workgroup
+    const int rt = p->rt;                                     // row tiles of 32 per workgroup
     a.row_blocks = p->r_pad / (32 * rt);
     a.col_blocks = (int)((a.M + oct::kOpCols - 1) / oct::kOpCols);
     a.out_stride = out_stride;

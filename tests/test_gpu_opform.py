@@ -97,17 +97,17 @@ def test_register_fragment_gemm_equals_lds_fragment_gemm(xp, dtype, monkeypatch)
     out = {}
     for sel in ("classic", "default"):
         monkeypatch.setenv("OCT_OPGEMM", sel)
-        out[sel] = (sp.run_operator(d).cpu().numpy(), sp.run_operator(d, 211, 70).cpu().numpy())
+        out[sel] = (sp.run_operator(d).cpu().numpy(), sp.run_operator(d, 211, 120).cpu().numpy())
     monkeypatch.delenv("OCT_OPGEMM")
     for a, b in zip(out["classic"], out["default"]):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("z0,nzc", [(400, 10), (180, 20), (300, 33), (512, 64), (211, 70)])
+@pytest.mark.parametrize("z0,nzc", [(400, 10), (180, 20), (300, 33), (512, 64), (211, 70), (90, 95), (600, 120)])
 def test_narrow_operators_vs_oracle(xp, z0, nzc):
     """Operators of 32, 64, 96 and 128 rows are applied with 1-4 row tiles per workgroup (no
-    padding to 128 rows); 70 bins take two 128-row blocks.  Each within 1e-5 of the largest
-    float64 oracle value on its bins."""
+    padding to 128 rows); 70 and 95 bins take two workgroup row blocks of 3 tiles, 120 bins two
+    of 4.  Each within 1e-5 of the largest float64 oracle value on its bins."""
     from oct_recon_af_v1_amd import synthetic as SY
     print(f"case: operator bins [{z0}, {z0 + nzc})", flush=True)
     lam, bref, sp = _setup(xp, -3e-5)
