@@ -290,32 +290,16 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
     }
 }
 
-// opgemm_kernel with the data fragments kept in registers.  Each lane owns one A-scan of its
-// wave's 64: it converts its own K step (raw - bg -> fp16 hi / lo, k 0..31 as four groups of 8),
-// and v_permlane32_swap turns the pair (k 8j..8j+7, k 8j+8..8j+15) into the B fragments of both
-// 32-column tiles: afterwards the first register holds, in lanes 0-31, k 8j.. of columns 0-31
-// and, in lanes 32-63, k 8j+8.. of columns 0-31 (from lanes 0-31); the second the same for
-// columns 32-63.  These are exactly opgemm_kernel's fragments, and the products run in the same
-// order, so the results are the same bits.  Only the operator rows go through LDS, double-
-// buffered with one workgroup barrier per K step; the next step's samples and operator rows are
-// loaded into registers during the current step's products.  RT row tiles of 32 per workgroup
-// (1-4: operators of up to 128 rows are not padded to a whole 128).  kLdsBg (files of at least 256
-// A-scans, so a workgroup's A-scans lie in at most two files): the two background rows are
-// read once into LDS instead of by every lane at every step (half the vector-memory traffic).
-// LDS 2 x 20 KB (+ 16 KB).
-__device__ __forceinline__ void frag_swap(half8_t& x, half8_t& y) {
-    typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
-    u32x4_t xv = __builtin_bit_cast(u32x4_t, x), yv = __builtin_bit_cast(u32x4_t, y);
-#pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const auto r = __builtin_amdgcn_permlane32_swap(xv[d], yv[d], false, false);
-        xv[d] = r[0];
-        yv[d] = r[1];
-    }
-    x = __builtin_bit_cast(half8_t, xv);
-    y = __builtin_bit_cast(half8_t, yv);
-}
-
+// opgemm_kernel with the data fragments kept in registers.  Each lane loads and converts the
+// samples of its own B fragments directly (raw - bg -> fp16 hi / lo): for the wave's two 32-column
+// tiles c, lanes 0-31 take samples k..k+7 and lanes 32-63 k+8..k+15 of A-scans 32 c + (lane & 31).
+// These are exactly opgemm_kernel's fragments, and the products run in the same order, so the
+// results are the same bits.  Only the operator rows go through LDS, double-buffered with one
+// workgroup barrier per K step; the next step's samples and operator rows are loaded into
+// registers during the current step's products.  RT row tiles of 32 per workgroup (1-4).
+// kLdsBg (files of at least 256 A-scans, so a workgroup's A-scans lie in at most two files): the
+// two background rows are read once into LDS instead of by every lane at every step.
+// LDS 2 x RT x 5 KB (+ 16 KB).
 template <typename RawT, bool kLdsBg, int RT>
 __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     constexpr int kR = 32 * RT;                                  // operator rows per workgroup
@@ -342,18 +326,24 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     const int row0 = rb * kR;
     const int nk = a.N / kOpK;
 
-    const long long p = col0 + tid;                              // this lane's A-scan
-    const bool pvalid = p < a.M;
-    const RawT* rrow = nullptr;
-    const float* brow = nullptr;
+    // This lane's data fragments: A-scans pc[c] = col0 + 64 wave + 32 c + (lane & 31), c = 0, 1,
+    // samples k0 + ks + ko .. + 7 (ko = 8 (lane >> 5)): the B-fragment layout of the matrix cores,
+    // loaded directly (a wave's load touches 32 A-scan rows, two lanes per 32-byte segment).
+    const int ko = 8 * (lane >> 5);
     const long long fb0 = col0 / a.nX;                           // first file of the block
-    int fsel = 0;                                                // kLdsBg: this lane's file - fb0 (0 or 1)
-    if (pvalid) {
-        const long long b = p / a.nX;
-        const long long x = p - b * a.nX;
-        rrow = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N;
-        brow = a.bg + b * a.N;
-        fsel = (int)(b - fb0);
+    bool pvalid[2];
+    const RawT* rrow[2];
+    const float* brow[2];
+    int fsel[2];                                                 // kLdsBg: file - fb0 (0 or 1)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        pvalid[c] = pc < a.M;
+        const long long b = pvalid[c] ? pc / a.nX : fb0;
+        const long long x = pvalid[c] ? pc - b * a.nX : 0;
+        rrow[c] = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N + ko;
+        brow[c] = a.bg + b * a.N + ko;
+        fsel[c] = (int)(b - fb0);
     }
     if constexpr (kLdsBg) {
         // nX >= kOpCols: the block's A-scans lie in files fb0 and fb0 + 1 (if present)
@@ -371,14 +361,20 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
     const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
     constexpr int kRawVec = 16 / sizeof(RawT);
-    constexpr int kRawLd = kOpK / kRawVec;
-    uint4 rv[kRawLd];
+    constexpr int kFragLd = 8 / kRawVec;                         // 16-byte loads per fragment
+    uint4 rv[2][kOpK / 16][kFragLd];                             // [c][ks / 16]
     half8_t oh[2], ol[2];
-    half8_t dh[kOpK / 8], dl[kOpK / 8];                          // this lane's step: k 8g .. 8g + 7
+    half8_t dh[kOpK / 16][2], dl[kOpK / 16][2];                  // fragments [ks / 16][c]
     auto fetch = [&](int k0) {
-        if (pvalid) {
 #pragma unroll
-            for (int g = 0; g < kRawLd; ++g) rv[g] = *reinterpret_cast<const uint4*>(rrow + k0 + kRawVec * g);
+        for (int c = 0; c < 2; ++c) {
+            if (pvalid[c]) {
+#pragma unroll
+                for (int j = 0; j < kOpK / 16; ++j)
+#pragma unroll
+                    for (int q = 0; q < kFragLd; ++q)
+                        rv[c][j][q] = *reinterpret_cast<const uint4*>(rrow[c] + k0 + 16 * j + kRawVec * q);
+            }
         }
         if (ostage) {
             const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
@@ -391,41 +387,44 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     };
     auto convert = [&](int k0) {
 #pragma unroll
-        for (int g = 0; g < kOpK / 8; ++g) {
-            half8_t h = {}, l = {};
-            if (pvalid) {
-                float f[8];
-                if constexpr (sizeof(RawT) == 2) {
-                    unpack16<RawT>(rv[g], f);
-                } else {
-                    float f0[4], f1[4];
-                    unpack16<RawT>(rv[2 * g], f0);
-                    unpack16<RawT>(rv[2 * g + 1], f1);
+        for (int c = 0; c < 2; ++c) {
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        f[e] = f0[e];
-                        f[4 + e] = f1[e];
+            for (int j = 0; j < kOpK / 16; ++j) {
+                half8_t h = {}, l = {};
+                if (pvalid[c]) {
+                    float f[8];
+                    if constexpr (sizeof(RawT) == 2) {
+                        unpack16<RawT>(rv[c][j][0], f);
+                    } else {
+                        float f0[4], f1[4];
+                        unpack16<RawT>(rv[c][j][0], f0);
+                        unpack16<RawT>(rv[c][j][1], f1);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            f[e] = f0[e];
+                            f[4 + e] = f1[e];
+                        }
+                    }
+                    float4 b0, b1;
+                    if constexpr (kLdsBg) {
+                        b0 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + 16 * j + ko]);
+                        b1 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + 16 * j + ko + 4]);
+                    } else {
+                        b0 = *reinterpret_cast<const float4*>(brow[c] + k0 + 16 * j);
+                        b1 = *reinterpret_cast<const float4*>(brow[c] + k0 + 16 * j + 4);
+                    }
+                    const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const float d = f[e] - bb[e];
+                        const _Float16 hv = (_Float16)d;
+                        h[e] = hv;
+                        l[e] = (_Float16)(d - (float)hv);
                     }
                 }
-                float4 b0, b1;
-                if constexpr (kLdsBg) {
-                    b0 = *reinterpret_cast<const float4*>(&bgs[fsel][k0 + 8 * g]);
-                    b1 = *reinterpret_cast<const float4*>(&bgs[fsel][k0 + 8 * g + 4]);
-                } else {
-                    b0 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g);
-                    b1 = *reinterpret_cast<const float4*>(brow + k0 + 8 * g + 4);
-                }
-                const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const float d = f[e] - bb[e];
-                    const _Float16 hv = (_Float16)d;
-                    h[e] = hv;
-                    l[e] = (_Float16)(d - (float)hv);
-                }
+                dh[j][c] = h;
+                dl[j][c] = l;
             }
-            dh[g] = h;
-            dl[g] = l;
         }
     };
     auto stage_op = [&](int sb) {
@@ -457,10 +456,8 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
 #pragma unroll
         for (int ks = 0; ks < kOpK; ks += 16) {
             const int kk = ks + 8 * (lane >> 5);
-            half8_t bh[2] = {dh[ks / 8], dh[ks / 8 + 1]};
-            half8_t bl[2] = {dl[ks / 8], dl[ks / 8 + 1]};
-            frag_swap(bh[0], bh[1]);
-            frag_swap(bl[0], bl[1]);
+            const half8_t bh[2] = {dh[ks / 16][0], dh[ks / 16][1]};
+            const half8_t bl[2] = {dl[ks / 16][0], dl[ks / 16][1]};
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
                 const int row = r * 32 + (lane & 31);
