@@ -46,11 +46,17 @@ def profiled_value(name: str, key: str = "traffic_bytes_per_ascan"):
     return None
 
 
+def operator_row_tiles(nzc: int) -> int:
+    """32-row tiles per GEMM workgroup for an operator of nzc bins (as oct_opplan_create chooses)."""
+    tiles = 2 * ((nzc + 15) // 16 * 16) // 32
+    return tiles if tiles <= 4 else (3 if (tiles + 2) // 3 * 3 < (tiles + 3) // 4 * 4 else 4)
+
+
 def operator_rows(nzc: int) -> int:
     """Rows of the fp16 operator the GEMM applies for nzc bins: 32-row tiles of 16 bins (real,
-    imaginary), grouped 1-4 per workgroup as oct_opplan_create groups them."""
+    imaginary), padded to whole groups of operator_row_tiles(nzc)."""
     tiles = 2 * ((nzc + 15) // 16 * 16) // 32
-    rt = tiles if tiles <= 4 else (3 if (tiles + 2) // 3 * 3 < (tiles + 3) // 4 * 4 else 4)
+    rt = operator_row_tiles(nzc)
     return (tiles + rt - 1) // rt * rt * 32
 
 
@@ -414,7 +420,8 @@ def main():
             "rows": [ulo, uhi], "rows_per_tile": out_rows, "operator_rows_padded": rpad,
             "operator_scale_log2": op.scale_log2,
             "roofline": {"bound": "mfma", "achieved": tfl, "peak": 2500.0, "unit": "TFLOP/s", "frac": tfl / 2500.0,
-                         "kernel": "opgemm_rf_kernel<uint16_t,true>", "flops_per_ascan": 3 * 2 * rpad * 2048,
+                         "kernel": f"opgemm_rf_kernel<uint16_t,true,{operator_row_tiles(wins[0][1] - wins[0][0])}>",
+                         "flops_per_ascan": 3 * 2 * rpad * 2048,
                          "avg_launch_ms": avg_m, "ascans_per_launch": per_m, "busy_ms_per_step": busy_ms,
                          "launches_overlap": True,
                          "hbm_GBps": ascans * (2 * 2048 + 4 * out_rows) / (busy_ms * 1e-3) / 1e9},
