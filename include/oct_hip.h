@@ -136,7 +136,7 @@ int oct_stitch_row_table(const oct_row_tile* tiles, const oct_row_tile* dev_tile
                          int n_z, const int32_t* frame_idx, const int32_t* shift, const int32_t* valid_lo,
                          const int32_t* valid_hi, int n_out_z, int n_out_x, float thr, float* db, int plane0,
                          int* partial, void* stream);
-/* plane clims (n_planes, 2) and the global clim (2) from row partials */
+/* plane clims (n_planes, 2) and the global clim (2) from row partials (8-byte aligned) */
 int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
                     void* stream);
 

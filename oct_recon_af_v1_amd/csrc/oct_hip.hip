@@ -550,6 +550,7 @@ int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, floa
                     void* stream) {
     if (!partial || !plane_clim || !clim) return fail(OCT_EINVAL, "null argument");
     if (n_planes <= 0 || blocks_per_plane <= 0) return fail(OCT_EINVAL, "empty volume");
+    if (reinterpret_cast<uintptr_t>(partial) & 7) return fail(OCT_EINVAL, "partials must be 8-byte aligned");
     hipLaunchKernelGGL(oct::clim_kernel, dim3(1), dim3(oct::kFinThreads), 0, reinterpret_cast<hipStream_t>(stream),
                        partial, blocks_per_plane, n_planes, plane_clim, clim);
     HIP_TRY(hipGetLastError());
@@ -573,6 +574,7 @@ int oct_finalize(const float* num, const float* den, float* db, int n_planes, lo
     const size_t need = oct_finalize_workspace_bytes(n_planes, plane_elems);
     if (!workspace || workspace_bytes < need)
         return fail(OCT_EINVAL, "workspace of %zu bytes needed, %zu given", need, workspace_bytes);
+    if (reinterpret_cast<uintptr_t>(workspace) & 7) return fail(OCT_EINVAL, "workspace must be 8-byte aligned");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int nb = finalize_blocks(plane_elems);
     int* partial = reinterpret_cast<int*>(workspace);

@@ -329,9 +329,12 @@ __global__ __launch_bounds__(kFinThreads) void clim_kernel(const int* __restrict
     int gmin = 0x7FFFFFFF, gmax = (int)0x80000000;
     for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
         int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
-        for (int i = 0; i < nblk; ++i) {
-            kmin = min(kmin, partial[((long long)p * nblk + i) * 2]);
-            kmax = max(kmax, partial[((long long)p * nblk + i) * 2 + 1]);
+        const int2* pp = reinterpret_cast<const int2*>(partial) + (long long)p * nblk;
+#pragma unroll 8
+        for (int i = 0; i < nblk; ++i) {                     // independent loads: overlapped
+            const int2 v = pp[i];
+            kmin = min(kmin, v.x);
+            kmax = max(kmax, v.y);
         }
         const bool any = kmin <= kmax;
         plane_clim[2 * p] = any ? fkey_inv(kmin) : __builtin_nanf("");
