@@ -332,6 +332,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="run a 1 GiB device copy first")
     ap.add_argument("--no-config2", action="store_true", help="skip the 200 000-A-scan all-bins measurement")
     ap.add_argument("--no-estimator", action="store_true", help="skip the learned-estimator measurement")
+    ap.add_argument("--side-streams", type=int, default=None,
+                    help="streams for the focus depths' operator-form launches (default: the pipeline's)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: multi-GPU runs are not part of this build "
@@ -395,6 +397,8 @@ def main():
         bits_plain = vol["bits"].clone()
         clim_plain = vol["clim"].clone()
         rec.cfg.form = "mfma"
+        if args.side_streams:
+            rec.max_side_streams = args.side_streams
         busy_m = []
         el_m, ms_m = time_steps(rec, vol, args.steps, args.warmup, busy_m)
         rec.cfg.form = "plain"

@@ -102,6 +102,7 @@ class Reconstructor:
         self._tile_betas = {}
         self._streams = {}
         self.concurrent_windows = True
+        self.max_side_streams = 4
         if tile_betas is not None:
             self.set_tile_betas(tile_betas)
         self._mag = None
@@ -258,7 +259,8 @@ class Reconstructor:
     def _side_streams(self):
         """Operator-form launches of different row windows run on their own streams: one tile's
         GEMM (a few hundred workgroups) does not fill the chip, tiles of other focus depths do.
-        Tiles of one window share a stream (their operator may be rebuilt per tile).  Off when
+        Tiles of one window share a stream (their operator may be rebuilt per tile); windows share
+        max_side_streams streams in turn (2, 3 and 4 measured the same on the tiled workload).  Off when
         binning or repeats share the processor's staging buffers."""
         return (self.cfg.form == "mfma" and self.concurrent_windows and self.nb == 1 and self.na == 1
                 and self.sp.ascan_binning == 1)
@@ -290,8 +292,13 @@ class Reconstructor:
                     start = torch.cuda.Event()
                     start.record(main)
                 s_ = self._streams.get((lo, hi))
-                if s_ is None:
-                    s_ = self._streams[(lo, hi)] = torch.cuda.Stream(device=self.xp.device)
+                if s_ is None:                      # windows share max_side_streams streams, in turn
+                    k = len(self._streams)
+                    if k < self.max_side_streams:
+                        s_ = torch.cuda.Stream(device=self.xp.device)
+                    else:
+                        s_ = list(self._streams.values())[k % self.max_side_streams]
+                    self._streams[(lo, hi)] = s_
                 if (lo, hi) not in used:
                     s_.wait_event(start)
                     if reuse is not None:
