@@ -424,7 +424,8 @@ def main():
             "rows": [ulo, uhi], "rows_per_tile": out_rows, "operator_rows_padded": rpad,
             "operator_scale_log2": op.scale_log2,
             "roofline": {"bound": "mfma", "achieved": tfl, "peak": 2500.0, "unit": "TFLOP/s", "frac": tfl / 2500.0,
-                         "kernel": f"opgemm_rf_kernel<uint16_t,true,{operator_row_tiles(wins[0][1] - wins[0][0])}>",
+                         "kernel": "opgemm_rf_kernel<uint16_t,true,RT>, RT " +
+                                   "/".join(str(t) for t in sorted({operator_row_tiles(hi - lo) for lo, hi in wins})),
                          "flops_per_ascan": 3 * 2 * rpad * 2048,
                          "avg_launch_ms": avg_m, "ascans_per_launch": per_m, "busy_ms_per_step": busy_ms,
                          "launches_overlap": True,
