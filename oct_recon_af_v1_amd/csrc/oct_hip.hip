@@ -591,10 +591,18 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
                  const float* plane_clim, int legacy_double, void* stream) {
     if (n_planes <= 0 || plane_elems <= 0) return OCT_OK;
     if (!db || !bits || !clim || (legacy_double && !plane_clim)) return fail(OCT_EINVAL, "null argument");
-    const long long n = (long long)n_planes * plane_elems;
-    const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 16);
-    hipLaunchKernelGGL(oct::quantize_kernel, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), db,
-                       bits, plane_elems, n_planes, clim, plane_clim, legacy_double);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const long long gx = (plane_elems + 1023) / 1024;
+    if (plane_elems % 4 == 0 && n_planes <= 65535 && gx <= 0x7FFFFFFFLL && (reinterpret_cast<uintptr_t>(db) & 15) == 0 &&
+        (reinterpret_cast<uintptr_t>(bits) & 7) == 0) {
+        hipLaunchKernelGGL(oct::quantize4_kernel, dim3((unsigned)gx, (unsigned)n_planes), dim3(256), 0, s, db, bits,
+                           plane_elems, clim, plane_clim, legacy_double);
+    } else {
+        const long long n = (long long)n_planes * plane_elems;
+        const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 16);
+        hipLaunchKernelGGL(oct::quantize_kernel, dim3(blocks), dim3(256), 0, s, db, bits, plane_elems, n_planes, clim,
+                           plane_clim, legacy_double);
+    }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

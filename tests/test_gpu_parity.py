@@ -229,6 +229,14 @@ def test_finalize_clim_quantize_exact(xp):
         bits = fin.quantize(db, cl, pcl, legacy_double=legacy).cpu().numpy()
         obits = R.quantise_volume(dbh, (float(gmin), float(gmax)), legacy, pcs)
         assert np.array_equal(bits, obits)
+    # planes whose size is not a multiple of 4 take the element-wise kernel
+    num3, den3 = num[:2, :41, :35].copy(), den[:2, :41, :35].copy()
+    db3, pcl3, cl3 = fin.finalize(xp.asarray(num3), xp.asarray(den3))
+    dbh3 = db3.cpu().numpy()
+    g3min, g3max, pcs3 = R.volume_clim(dbh3)
+    for legacy in (False, True):
+        bits = fin.quantize(db3, cl3, pcl3, legacy_double=legacy).cpu().numpy()
+        assert np.array_equal(bits, R.quantise_volume(dbh3, (float(g3min), float(g3max)), legacy, pcs3))
     # spectral-only finalise (no denominator)
     db2, _, _ = fin.finalize(xp.asarray(num), None)
     with np.errstate(divide="ignore"):
