@@ -194,7 +194,7 @@ def copy_calibration(gib: float = 1.0):
     del a, b
 
 
-def config2(xp, args):
+def all_bins_form(xp, args):
     """Operator-form precision workload: 200 000 A-scans (200 B-scans x 1000 + 25 background
     lines), reflectivity over 60 dB, all 1024 depth bins; MFMA form vs plain FP32 timing and
     agreement with the float64 oracle on a sample of 2 B-scans."""
@@ -330,7 +330,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the operator-form measurements")
     ap.add_argument("--copy-calibration", action="store_true", help="run a 1 GiB device copy first")
-    ap.add_argument("--no-config2", action="store_true", help="skip the 200 000-A-scan all-bins measurement")
+    ap.add_argument("--no-all-bins", action="store_true", help="skip the 200 000-A-scan all-bins measurement")
     ap.add_argument("--no-estimator", action="store_true", help="skip the learned-estimator measurement")
     ap.add_argument("--side-streams", type=int, default=None,
                     help="streams for the focus depths' operator-form launches (default: the pipeline's)")
@@ -434,9 +434,9 @@ def main():
         log(f"mfma: step {el_m / args.steps * 1e3:.2f} ms; opgemm avg {avg_m:.3f} ms per launch, busy {busy_ms:.2f} ms per step; "
             f"within 1 LSB of plain {within:.6f}")
         del bits_plain
-        if not args.no_config2:
-            res["config2_precision_form"] = config2(xp, args)
-            log(f"config2: {res['config2_precision_form']}")
+        if not args.no_all_bins:
+            res["all_bins_precision_form"] = all_bins_form(xp, args)
+            log(f"all bins: {res['all_bins_precision_form']}")
     if not args.no_estimator:
         res["learned_estimator"] = estimator_bench(xp, args)
         log(f"estimator: {res['learned_estimator']}")
