@@ -40,11 +40,11 @@ def _proc(xp, lam, beta_ref, apod=25, mode="lines", group=1):
     return sg, HipSpectralProcessor(sg, apod, xp, apod_mode=mode, apod_group=group)
 
 
-def test_magnitude_config0_lsb_parity(xp):
+def test_magnitude_one_tile_lsb_parity(xp):
     """20 B-scans x 500 A-scans x 2048 px uint16 (+25 background lines), beta ~ N(0, 5e-5^2)."""
     from oct_recon_af_v1_amd import synthetic as SY
     from oct_recon_af_v1_amd.geometry import beta_from_index_units
-    _say("case: config0 magnitude, seed 0")
+    _say("case: one-tile magnitude (20 x 500 A-scans), seed 0")
     spec = SY.SpectrumSpec()
     lam = spec.lambda_nm()
     beta = float(np.random.default_rng(0).normal(0, 5e-5))
