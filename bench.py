@@ -47,17 +47,18 @@ def profiled_value(name: str, key: str = "traffic_bytes_per_ascan"):
 
 
 def operator_row_tiles(nzc: int) -> int:
-    """32-row tiles per GEMM workgroup for an operator of nzc bins (as oct_opplan_create chooses)."""
-    tiles = 2 * ((nzc + 15) // 16 * 16) // 32
-    return tiles if tiles <= 4 else (3 if (tiles + 2) // 3 * 3 < (tiles + 3) // 4 * 4 else 4)
+    """32-row tiles per GEMM workgroup for an operator of nzc bins (oct_opplan_rows)."""
+    import ctypes
+    from oct_recon_af_v1_amd import _lib
+    rt = ctypes.c_int(0)
+    _lib.lib().oct_opplan_rows(int(nzc), ctypes.byref(rt))
+    return rt.value
 
 
 def operator_rows(nzc: int) -> int:
-    """Rows of the fp16 operator the GEMM applies for nzc bins: 32-row tiles of 16 bins (real,
-    imaginary), padded to whole groups of operator_row_tiles(nzc)."""
-    tiles = 2 * ((nzc + 15) // 16 * 16) // 32
-    rt = operator_row_tiles(nzc)
-    return (tiles + rt - 1) // rt * rt * 32
+    """Rows of the fp16 operator the GEMM applies for nzc bins (oct_opplan_rows)."""
+    from oct_recon_af_v1_amd import _lib
+    return int(_lib.lib().oct_opplan_rows(int(nzc), None))
 
 
 # VALU issue peak without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles

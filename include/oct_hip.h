@@ -238,6 +238,11 @@ int oct_opplan_destroy(oct_opplan* plan);
  * octrecon/pipeline.py:155-190). */
 int oct_opplan_set_window(oct_opplan* plan, const double* window, void* stream);
 int oct_opplan_scale_bound(const oct_opplan* plan, double* max_window_abs);
+/* Rows of the fp16 operator oct_opplan_create allocates and the GEMM applies for nzc bins (32-row
+ * tiles of 16 bins' real and imaginary parts, grouped 1-4 per GEMM workgroup; the grouping into
+ * *row_tiles when not NULL).  Host arithmetic only: no device is touched.  Returns the row count,
+ * or OCT_EINVAL for nzc < 1.  No counterpart in the reference (a sizing query of this library). */
+int oct_opplan_rows(int nzc, int* row_tiles);
 /* magnitudes |A (raw - bg)| of bins [z0, z0+nzc) -> out (M, out_stride) float32 */
 int oct_opspectral(const oct_opplan* plan, const void* raw, int raw_dtype, int n_files, int lines, int apod_mode,
                    int apod_size, float* out, long long out_stride, const float* bg, void* stream);

@@ -64,6 +64,7 @@ SIGNATURES = {
     "oct_packbits_sizes": (_i, [_vp, _ll, _i, _vp, _vp]),
     "oct_packbits_write": (_i, [_vp, _ll, _i, _vp, _vp, _vp]),
     "oct_opplan_scale_bound": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
+    "oct_opplan_rows": (_i, [_i, ctypes.POINTER(ctypes.c_int)]),
     "oct_opspectral": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_plan_use_window": (_i, [_vp, _vp]),
     "oct_topfrac_metric_workspace_bytes": (_sz, [_i, _i, _i, _i]),

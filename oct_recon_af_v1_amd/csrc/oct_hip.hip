@@ -709,6 +709,16 @@ struct oct_opplan {
     double2* d_c;                 // (nzc) rank-1 term, rebuilt with each window
 };
 
+int oct_opplan_rows(int nzc, int* row_tiles) {
+    if (nzc < 1) return fail(OCT_EINVAL, "%d bins", nzc);
+    // rows: 16 bins' real then imaginary parts per tile of 32 rows.  The GEMM takes rt = 1-4 row
+    // tiles per workgroup: all of them up to 4 tiles, else 4, or 3 where that pads fewer tiles
+    const int tiles = 2 * ((nzc + 15) / 16 * 16) / 32;
+    const int rt = tiles <= 4 ? tiles : ((tiles + 2) / 3 * 3 < (tiles + 3) / 4 * 4 ? 3 : 4);
+    if (row_tiles) *row_tiles = rt;
+    return (tiles + rt - 1) / rt * rt * 32;
+}
+
 int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, const int32_t* band_start,
                       const double* band_w, const double* window, int z0, int nzc, int* scale_log2,
                       double* op64_host) {
@@ -757,11 +767,8 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     const double bound = amax_win / N * (colsum + usum / N);
     const int sa = (int)std::floor(std::log2(16384.0 / bound));
     const int nzc_pad = (nzc + 15) / 16 * 16;
-    // rows: 16 bins' real then imaginary parts per tile of 32 rows.  The GEMM takes rt = 1-4 row
-    // tiles per workgroup: all of them up to 4 tiles, else 4, or 3 where that pads fewer tiles
-    const int tiles = 2 * nzc_pad / 32;
-    const int rt = tiles <= 4 ? tiles : ((tiles + 2) / 3 * 3 < (tiles + 3) / 4 * 4 ? 3 : 4);
-    const int r_pad = (tiles + rt - 1) / rt * rt * 32;
+    int rt = 0;
+    const int r_pad = oct_opplan_rows(nzc, &rt);
 
     int prev = 0;
     HIP_TRY(hipGetDevice(&prev));

@@ -131,6 +131,26 @@ def test_library_exports_every_declared_symbol():
     assert declared == set(_lib.SIGNATURES)
 
 
+def test_operator_row_grouping():
+    """oct_opplan_rows (host arithmetic, no device): 32-row tiles of 16 bins' real and imaginary
+    parts, all in one GEMM workgroup up to 4 tiles, else groups of 4, or of 3 where that pads
+    fewer tiles; rejects nzc < 1."""
+    from oct_recon_af_v1_amd import _lib
+    L = _lib.lib()
+    cases = {1: (32, 1), 16: (32, 1), 17: (64, 2), 33: (96, 3), 64: (128, 4), 70: (192, 3), 95: (192, 3),
+             100: (256, 4), 120: (256, 4), 150: (384, 4), 1024: (2048, 4)}
+    for nzc, (rows, rt) in cases.items():
+        got = ctypes.c_int(-1)
+        assert (L.oct_opplan_rows(nzc, ctypes.byref(got)), got.value) == (rows, rt), nzc
+    for nzc in range(1, 1025):
+        got = ctypes.c_int(-1)
+        rows = L.oct_opplan_rows(nzc, ctypes.byref(got))
+        tiles = 2 * ((nzc + 15) // 16 * 16) // 32
+        assert rows % (32 * got.value) == 0 and rows >= 32 * tiles and rows - 32 * tiles < 32 * got.value
+        assert got.value == tiles if tiles <= 4 else got.value in (3, 4)
+    assert L.oct_opplan_rows(0, None) == -1                     # OCT_EINVAL
+
+
 def test_generator_shape_and_range():
     raw = SY.make_bscans(2, 8, seed=5)
     a = raw.numpy()
