@@ -152,7 +152,12 @@ class Reconstructor:
             op = self.sp.operator_plan(lo, hi - lo)
             if i is not None and tw["max_abs"] > op.max_window_abs() * (1 + 1e-12):
                 raise ValueError("tile window exceeds the operator's scale bound")
-            ptr = None if i is None else tw["f64"][i].data_ptr()
+            ptr = None
+            if i is not None:
+                ptr = tw["f64"][i].data_ptr()
+                # the operator build may run on a side stream that the current stream never
+                # waits for: keep the windows' memory until it has read them
+                tw["f64"].record_stream(self.torch.cuda.current_stream(self.xp.device))
             op.set_window(ptr, self.xp.stream())
         else:
             ptr = None if i is None else tw["f32"][i].data_ptr()
@@ -265,10 +270,13 @@ class Reconstructor:
         return (self.cfg.form == "mfma" and self.concurrent_windows and self.nb == 1 and self.na == 1
                 and self.sp.ascan_binning == 1)
 
-    def _row_magnitudes(self, yi, frames, cancel=None, parity=None):
+    def _row_magnitudes(self, yi, frames, cancel=None, parity=None, events=None):
         """Spectral stage for every tile of row yi into per-tile buffers (nF, nX, rows).  With a
         parity (0 / 1) the buffers are that parity's set, written only after the last stitch
-        that read the set has finished."""
+        that read the set has finished.  Tiles on side streams end in events that the current
+        stream waits for, or, given the list `events`, that are appended to it for the consumer
+        to wait for: then the next row's side work on a stream starts as soon as that stream's
+        tiles of this row are done, not when the slowest stream's are."""
         torch = self.torch
         frames = np.asarray(frames)
         out = []
@@ -315,7 +323,10 @@ class Reconstructor:
         for s_ in used.values():
             ev = torch.cuda.Event()
             ev.record(s_)
-            main.wait_event(ev)
+            if events is None:
+                main.wait_event(ev)
+            else:
+                events.append(ev)
         return out
 
     def _tile_spectral(self, xi, yi, zi, frames, lo, hi, buf, key, cancel, main=None):
@@ -342,10 +353,11 @@ class Reconstructor:
             self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
         return buf
 
-    def _stitch_row(self, tiles, frames, db, plane0, partial, stream=None):
+    def _stitch_row(self, tiles, frames, db, plane0, partial, stream=None, after=()):
         """One fused stitch + finalise launch for a row (oct_stitch_row), on the current stream
-        or, given `stream`, on that stream once the current stream's work so far is done (the
-        row's magnitudes and descriptors); returns the event recorded after it there."""
+        or, given `stream`, on that stream once the current stream's work so far (descriptors)
+        and the events `after` (the row's magnitudes) are done; returns the event recorded
+        after it there."""
         _, nz_out, nx_out = self.n_out
         arr = (_lib.RowTile * max(1, len(tiles)))()
         nX = None
@@ -390,6 +402,8 @@ class Reconstructor:
             ready = self.torch.cuda.Event()
             ready.record(self.torch.cuda.current_stream(self.xp.device))
             stream.wait_event(ready)
+            for ev in after:
+                stream.wait_event(ev)
             sh = stream.cuda_stream
             for t in (fr, table, db, partial):
                 if t is not None:
@@ -513,9 +527,10 @@ class Reconstructor:
         self._row_seq += 1
         if self._stitch_stream is None:
             self._stitch_stream = self.torch.cuda.Stream(device=self.xp.device)
-        tiles = self._row_magnitudes(vol["rows"][k], frames, cancel, parity=parity)
+        ends = []
+        tiles = self._row_magnitudes(vol["rows"][k], frames, cancel, parity=parity, events=ends)
         self._stitch_done[parity] = self._stitch_row(tiles, frames, vol["db"], k * ny, partial,
-                                                     stream=self._stitch_stream)
+                                                     stream=self._stitch_stream, after=ends)
 
     def finish_volume(self, vol):
         """Clims and codes of a volume whose rows were all made by volume_row (vol["db"] is
