@@ -351,6 +351,9 @@ class Reconstructor:
                     buf.record_stream(main)
                 self._tile_mag[key] = buf
             self._magnitudes(raw, len(fr), lo, hi, buf[s:s + len(fr)])
+            if main is not None and raw.is_cuda:
+                # a side stream reads it and the caller's stream may never wait for that stream
+                raw.record_stream(self.torch.cuda.current_stream(self.xp.device))
         return buf
 
     def _stitch_row(self, tiles, frames, db, plane0, partial, stream=None, after=()):

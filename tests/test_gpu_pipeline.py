@@ -97,8 +97,9 @@ def test_small_tiled_volume_vs_oracle(xp):
 def test_overlapped_rows_equal_serial_rows(xp):
     """Volume rows whose stitches run on their own stream beside the next row's spectral work
     (tile magnitudes alternating between two buffer sets) give the same bits as rows run one
-    after the other, in both forms, also for a second volume that reuses the buffer sets and for
-    process_row called in between."""
+    after the other, in both forms, also for a second volume that reuses the buffer sets, for
+    process_row called in between, and for tiles that a source hands over as device copies whose
+    last reference the reconstructor drops while the MFMA form's side streams still read them."""
     print("case: overlapped vs serial volume rows", flush=True)
     rec, grid, lam, bref, tiles = _tiled(xp, 4, 64, seed=5)
     for form in ("plain", "mfma"):
@@ -110,6 +111,22 @@ def test_overlapped_rows_equal_serial_rows(xp):
             mid_db, _ = rec.process_row(1)
             got.setdefault(overlap, []).append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(),
                                                 mid_db.cpu().numpy()))
+        # tiles from a source that hands over device copies made beforehand on the caller's
+        # stream, whose last reference goes as soon as the tile's launches are issued
+        resident = rec.tiles
+        files = rec.frame_files(np.arange(grid.n_y_px))
+        staged = {k + (files.tobytes(),): v[files[0]:files[-1] + 1].clone() for k, v in resident.items()}
+
+        def source(xi, yi, zi, f):
+            return staged.pop((xi, yi, zi, np.asarray(f).tobytes()))
+
+        rec.tiles = source
+        rec.overlap_rows = True
+        try:
+            vol = rec.reconstruct_volume(rec.allocate_volume())
+            got[True].append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(), got[True][0][2]))
+        finally:
+            rec.tiles = resident
         ref_db, ref_bits, ref_mid = got[False][0]
         for db, bits, mid in got[True]:
             assert np.array_equal(db.view(np.uint32), ref_db.view(np.uint32)), form
