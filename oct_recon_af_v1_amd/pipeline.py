@@ -102,6 +102,7 @@ class Reconstructor:
         self._tile_betas = {}
         self._streams = {}
         self.concurrent_windows = True
+        self.concurrent_forms = ("mfma", "plain")
         self.max_side_streams = 4
         if tile_betas is not None:
             self.set_tile_betas(tile_betas)
@@ -160,7 +161,10 @@ class Reconstructor:
                 tw["f64"].record_stream(self.torch.cuda.current_stream(self.xp.device))
             op.set_window(ptr, self.xp.stream())
         else:
-            ptr = None if i is None else tw["f32"][i].data_ptr()
+            ptr = None
+            if i is not None:
+                ptr = tw["f32"][i].data_ptr()
+                tw["f32"].record_stream(self.torch.cuda.current_stream(self.xp.device))
             _lib.check(_lib.lib().oct_plan_use_window(self.sp.plan.handle, ptr), "oct_plan_use_window")
 
     def _restore_windows(self):
@@ -262,12 +266,15 @@ class Reconstructor:
         return 0, self.sp.nz
 
     def _side_streams(self):
-        """Operator-form launches of different row windows run on their own streams: one tile's
-        GEMM (a few hundred workgroups) does not fill the chip, tiles of other focus depths do.
-        Tiles of one window share a stream (their operator may be rebuilt per tile); windows share
-        max_side_streams streams in turn (2, 3 and 4 measured the same on the tiled workload).  Off when
-        binning or repeats share the processor's staging buffers."""
-        return (self.cfg.form == "mfma" and self.concurrent_windows and self.nb == 1 and self.na == 1
+        """Launches of different row windows (focus depths) run on their own streams, one set per
+        form.  Operator form: one tile's GEMM (a few hundred workgroups) does not fill the chip,
+        tiles of other focus depths do.  Plain form: the persistent spectral kernel's last groups
+        leave CUs idle, which the next depth's launch fills (2.4 % per tiled step).  Tiles of one
+        window share a stream (their operator may be rebuilt per tile); windows share
+        max_side_streams streams in turn (2, 3 and 4 measured the same on the tiled workload).
+        Forms outside concurrent_forms keep every launch on the caller's stream.  Off when binning
+        or repeats share the processor's staging buffers."""
+        return (self.cfg.form in self.concurrent_forms and self.concurrent_windows and self.nb == 1 and self.na == 1
                 and self.sp.ascan_binning == 1)
 
     def _row_magnitudes(self, yi, frames, cancel=None, parity=None, events=None):
@@ -299,14 +306,15 @@ class Reconstructor:
                 if start is None:
                     start = torch.cuda.Event()
                     start.record(main)
-                s_ = self._streams.get((lo, hi))
+                streams = self._streams.setdefault(self.cfg.form, {})   # each form its own set
+                s_ = streams.get((lo, hi))
                 if s_ is None:                      # windows share max_side_streams streams, in turn
-                    k = len(self._streams)
+                    k = len(streams)
                     if k < self.max_side_streams:
                         s_ = torch.cuda.Stream(device=self.xp.device)
                     else:
-                        s_ = list(self._streams.values())[k % self.max_side_streams]
-                    self._streams[(lo, hi)] = s_
+                        s_ = list(streams.values())[k % self.max_side_streams]
+                    streams[(lo, hi)] = s_
                 if (lo, hi) not in used:
                     s_.wait_event(start)
                     if reuse is not None:
