@@ -96,10 +96,11 @@ def test_small_tiled_volume_vs_oracle(xp):
 
 def test_overlapped_rows_equal_serial_rows(xp):
     """Volume rows whose stitches run on their own stream beside the next row's spectral work
-    (tile magnitudes alternating between two buffer sets) give the same bits as rows run one
-    after the other, in both forms, also for a second volume that reuses the buffer sets, for
-    process_row called in between, and for tiles that a source hands over as device copies whose
-    last reference the reconstructor drops while the MFMA form's side streams still read them."""
+    (tile magnitudes alternating between two buffer sets, each focus depth's tiles on a stream
+    of its own) give the same bits as rows run one after the other on the caller's stream alone,
+    in both forms; also for a second volume that reuses the buffer sets, for process_row called
+    in between, and for tiles that a source hands over as device copies whose last reference the
+    reconstructor drops while the side streams still read them."""
     print("case: overlapped vs serial volume rows", flush=True)
     rec, grid, lam, bref, tiles = _tiled(xp, 4, 64, seed=5)
     for form in ("plain", "mfma"):
@@ -107,6 +108,8 @@ def test_overlapped_rows_equal_serial_rows(xp):
         got = {}
         for overlap in (False, True, True):
             rec.overlap_rows = overlap
+            # the serial reference also keeps every tile's launch on the caller's stream
+            rec.concurrent_forms = ("mfma", "plain") if overlap else ()
             vol = rec.reconstruct_volume(rec.allocate_volume())
             mid_db, _ = rec.process_row(1)
             got.setdefault(overlap, []).append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(),
