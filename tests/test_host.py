@@ -158,3 +158,19 @@ def test_generator_shape_and_range():
     assert a.max() <= 4095
     b = SY.make_bscans(2, 8, seed=5).numpy()
     assert np.array_equal(a, b)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 (or a WORLD_SIZE above 1) exits with status 2 and
+    says why, before touching a device."""
+    import os
+    import subprocess
+    import sys
+    root = Path(__file__).resolve().parents[1]
+    for extra_args, env_ws in ((["--gpus", "2"], "1"), (["--gpus", "1"], "2")):
+        env = dict(os.environ, WORLD_SIZE=env_ws)
+        r = subprocess.run([sys.executable, str(root / "bench.py"), *extra_args], cwd=root, env=env,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 2, (extra_args, env_ws, r.stderr)
+        assert "exactly one GPU" in r.stderr
+        assert r.stdout == ""
