@@ -284,11 +284,16 @@ class InputBuilder:
         return out
 
     def _mag(self, n):
+        """Probe magnitudes of n images, one buffer per shape kept for the builder's lifetime: a
+        HIP graph captured for one batch shape holds this buffer's address, so it must survive
+        calls with other shapes (it is never freed or reallocated)."""
         sc = self.scorer
         shape = (n, self.spec.n_x_crop, sc.nz)
-        if getattr(self, "_magbuf", None) is None or tuple(self._magbuf.shape) != shape:
-            self._magbuf = sc.torch.empty(shape, dtype=sc.torch.float32, device=sc.xp.device)
-        return self._magbuf
+        bufs = self.__dict__.setdefault("_magbufs", {})
+        buf = bufs.get(shape)
+        if buf is None:
+            buf = bufs[shape] = sc.torch.empty(shape, dtype=sc.torch.float32, device=sc.xp.device)
+        return buf
 
 
 def focus_from_profiles(profiles, search_px=None, guess=None):

@@ -92,6 +92,26 @@ def test_graph_replays_equal_eager(xp, heldout):
         del junk
 
 
+def test_graph_replays_alternating_shapes(xp, heldout):
+    """Graphs captured for two batch shapes, replayed alternately: each replay must equal the eager
+    sequence, so capturing or running one shape never frees or moves a buffer that the other
+    shape's graph writes (the probe magnitudes, the network's activations)."""
+    from oct_recon_af_v1_amd import learned as LN
+    print("case: estimator graph replays, alternating batch shapes 8 / 24", flush=True)
+    spec, raw, beta, focus = heldout
+    est = LN.LearnedEstimator(xp, spec.lambda_nm(), spec.apod)
+    ref = LN.LearnedEstimator(xp, spec.lambda_nm(), spec.apod)
+    for rep, (i, n) in enumerate(((0, 8), (8, 24), (32, 8), (40, 24), (64, 8), (16, 24), (96, 8))):
+        t = raw[i:i + n]
+        g = est.predict(t, graph=True)
+        if rep % 2 == 0:
+            est.predict(raw[100:100 + 3 + rep])           # eager work of a third shape between replays
+        e = ref.predict(t)
+        xp.synchronize()
+        for k in ("beta_idx", "focus", "sigma_beta", "sigma_focus", "beta_stage1"):
+            assert np.array_equal(g[k], e[k]), (rep, n, k)
+
+
 def test_learned_estimator_speed(xp, heldout):
     """Estimation time per tile of 8 B-scans: network vs the restated reference search."""
     from oct_recon_af_v1_amd import learned as LN
