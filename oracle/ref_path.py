@@ -16,9 +16,10 @@ Third-party arithmetic used as in the reference: scipy.fft (pocketfft; scipy 1.1
 image) for the single-precision inverse FFT, numpy matmul (OpenBLAS) for the stitching
 products, scipy.interpolate.PchipInterpolator for lambda_eq.
 
-Pins (tests/test_oracle.py): the MATLAB planes of the reference's tests
-(tests/golden/legacy_base.npz, 1e-4 dB), a brute-force DFT, closed forms of the window and
-quantisation, MATLAB filter2 binning values, and the reference's own dispersion-window identity.
+Pins (tests/test_oracle.py): the MATLAB plane of the reference's tests for the Telesto synthetic
+scan (tests/golden/legacy_telesto.npz, 1e-4 dB), a brute-force DFT, closed forms of the window and
+quantisation, MATLAB filter2 binning values, the reference's own dispersion-window identity, and
+the dense float64 operator against the float64 spectral path.
 """
 from __future__ import annotations
 
