@@ -211,6 +211,16 @@ int oct_spectral_repeat(const oct_plan* plan, const void* raw, int raw_dtype, in
 size_t oct_topfrac_metric_workspace_bytes(int n_bscans, int n_x, int n_z, int z_min);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* workspace, size_t workspace_bytes, void* stream);
+/* No counterpart in the reference: oct_topfrac_metric for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight search loops, harnesses that forbid hipMalloc).
+ * The same kernels and results as oct_topfrac_metric, with the workspace taken from a 4 MiB
+ * device array that the library holds on each GPU until the process ends (enough for about 500
+ * B-scans per call); a call whose oct_topfrac_metric_workspace_bytes exceeds 4 MiB fails with
+ * OCT_EINVAL.  It allocates nothing and uses the device current at the call (the stream must
+ * belong to it).  All calls on one GPU share that array: calls that may overlap on one GPU
+ * (different streams, or threads) race and must be serialised by the caller. */
+int oct_topfrac_metric_noalloc(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
+                               void* stream);
 /* Candidate windows out (count, n, 2) float32: hann_over_n[j] exp(-i beta_b dk2[j]) in float64, with
  * beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref (device
  * arrays; `head` lets a network's output set the candidates without a host round trip). */

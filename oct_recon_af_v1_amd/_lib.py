@@ -69,6 +69,7 @@ SIGNATURES = {
     "oct_plan_use_window": (_i, [_vp, _vp]),
     "oct_topfrac_metric_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp, _sz, _vp]),
+    "oct_topfrac_metric_noalloc": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp]),
     "oct_plan_use_windows": (_i, [_vp, _vp, _ll, _ll]),
     "oct_spectral_repeat": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_depth_median": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp]),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -675,6 +676,57 @@ int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_m
                        dim3(oct::kTfThreads), 0, s, st, part, S, k, n_bscans, out);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation metric
+// Scratch of oct_topfrac_metric_noalloc: the histograms, selection states and float64 partials
+// of the metric, in a module-scope device array (every GPU the library runs on holds its own
+// copy until the process ends).  Host code never passes `scratch` itself to a kernel: its name
+// is a host-side stand-in, so the device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_dev.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int scratch_of_current_device(void** ptr) {
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(OCT_EINVAL, "device %d outside [0, %d)", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OCT_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    *ptr = g_scratch_dev[dev];
+    return OCT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int oct_topfrac_metric_noalloc(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
+                               void* stream) {
+    if (n_bscans <= 0) return OCT_OK;
+    const size_t need = oct_topfrac_metric_workspace_bytes(n_bscans, n_x, n_z, z_min);
+    if (need > kNoAllocScratchBytes)
+        return fail(OCT_EINVAL, "metric of %d B-scans x %d x %d needs %zu bytes of scratch, more than the %zu bytes "
+                    "of the no-allocation entry point (use oct_topfrac_metric with a workspace)", n_bscans, n_x, n_z,
+                    need, kNoAllocScratchBytes);
+    void* ws = nullptr;
+    const int rc = scratch_of_current_device(&ws);
+    if (rc != OCT_OK) return rc;
+    // the same kernels as oct_topfrac_metric; its first kernel clears the histograms and states
+    return oct_topfrac_metric(mag, n_bscans, n_x, n_z, z_min, frac, out, ws, kNoAllocScratchBytes, stream);
 }
 
 int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,
