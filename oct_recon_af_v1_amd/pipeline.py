@@ -119,6 +119,18 @@ class Reconstructor:
         self._frames_dev = {}               # frame lists on the device (no upload per row)
         self._table_pin = [None, None]      # pinned staging of device tile tables, alternating
         self.timers = {"spectral": 0.0, "stitch": 0.0, "finalise": 0.0}
+        self.prepare_operators()
+
+    def prepare_operators(self):
+        """Create the operator (MFMA) plan of every row window the tiles use, now rather than at a
+        tile's first launch: plan creation clears and builds on the device and synchronises, so
+        it must not run inside a row (where side streams are busy) or under graph capture.
+        Called at construction and again at the start of every volume and row (a no-op unless
+        spectral_rows changed)."""
+        for st in self.stitchers.values():
+            if not st.empty:
+                lo, hi = self._tile_rows(st)
+                self.sp.operator_plan(lo, hi - lo)
 
     # ------------------------------------------------------------------ data access
     def set_tile_betas(self, tile_betas, update: bool = False):
@@ -481,6 +493,7 @@ class Reconstructor:
         Returns (db, plane_clim) device tensors: (nF, nOutZ, nOutX) float32 and (nF, 2)."""
         frames = np.arange(self.scan.n_y_px) if frames is None else np.asarray(frames)
         _, nz_out, nx_out = self.n_out
+        self.prepare_operators()
         try:
             if self.cfg.fused_rows:
                 self._join_stitches()              # volume rows may still use the partials
@@ -560,6 +573,7 @@ class Reconstructor:
         """All requested y-tile rows into one device volume: dB, plane clims, global clim and
         uint16 codes (single finalise and quantise passes).  No host synchronisation."""
         ny = self.scan.n_y_px
+        self.prepare_operators()
         try:
             if self.cfg.fused_rows:
                 for k in range(len(vol["rows"])):

@@ -15,6 +15,7 @@ the readers take the sizes the caller's header gives.
 """
 from __future__ import annotations
 
+import os
 import re
 import threading
 from concurrent.futures import ThreadPoolExecutor
@@ -27,8 +28,15 @@ from . import _lib
 _SRR_NAME = re.compile(r"^Data_Y(\d{1,4})_YTotal(\d+)_B(\d{1,4})_BTotal(\d+)_(.+)\.srr$")
 
 
-def _readinto(path: Path, dst: memoryview, nbytes: int, offset: int = 0):
+def _readinto(path: Path, dst: memoryview, nbytes: int, offset: int = 0, whole: bool = False):
+    """nbytes of the file from `offset` into dst; IOError if the file holds fewer.  whole=True:
+    the file must hold exactly nbytes after the offset, counted in 2-byte samples (an SRR payload,
+    octrecon/io/srr.py read_file), so a longer or truncated file is an error too."""
     with open(path, "rb", buffering=0) as f:
+        if whole:
+            samples = (os.fstat(f.fileno()).st_size - offset) // 2
+            if samples != nbytes // 2:
+                raise IOError(f"{path}: expected {nbytes // 2} samples, got {samples}")
         if offset:
             f.seek(offset)
         got = f.readinto(dst)
@@ -153,7 +161,7 @@ class SrrTile:
         if mv.nbytes != n * self.payload_bytes:
             raise ValueError(f"destination holds {mv.nbytes} bytes, {n} files need {n * self.payload_bytes}")
         jobs = [(self.path(fi), mv[k * self.payload_bytes:(k + 1) * self.payload_bytes], self.payload_bytes,
-                 self.offset) for k, fi in enumerate(file_indices)]
+                 self.offset, True) for k, fi in enumerate(file_indices)]
         if pool is None:
             for j in jobs:
                 _readinto(*j)

@@ -64,6 +64,35 @@ def test_magnitude_one_tile_lsb_parity(xp):
          f"vs fp64 {f64:.6%} (max {m64}); cpu32 vs fp64 {c32:.6%}")
     assert rel < 2e-6
     assert f32 >= 0.9999 and f64 >= 0.9999
+    _explain_outliers(g, o32, o64, clim)
+
+
+def _codes(mag, clim):
+    with np.errstate(divide="ignore"):
+        return R.db_to_bits((20 * np.log10(np.asarray(mag, np.float32))).astype(np.float32), clim).astype(np.int64)
+
+
+def _explain_outliers(g, o32, o64, clim):
+    """Every voxel more than 1 LSB from the float32 CPU path or from float64 is a near-zero voxel
+    of its A-scan: |A| below 1e-3 of the A-scan's peak (60 dB down), where the dB scale turns an
+    absolute float32 rounding error into more than one code; and its error against float64 is
+    such a rounding error: at most 1e-6 of the A-scan's peak (the reference's own float32 CPU
+    path misses float64 by up to 2.8e-7 of the peak on this input, and by more than 1 LSB on
+    voxels up to 2e-4 of the peak; the reference states the same effect for its torch backend,
+    tests/test_torch_backend.py:62-64)."""
+    cg, c32, c64 = _codes(g, clim), _codes(o32, clim), _codes(o64, clim)
+    peak = o64.max(axis=-1, keepdims=True)
+    rel = o64 / peak
+    err = np.abs(g.astype(np.float64) - o64) / peak
+    out = (np.abs(cg - c32) > 1) | (np.abs(cg - c64) > 1)
+    cpu_out = np.abs(c32 - c64) > 1
+    _say(f"outliers: device {int(out.sum())} (of which the CPU path also misses float64: "
+         f"{int((out & cpu_out).sum())}), CPU path vs float64 {int(cpu_out.sum())}; device outliers at "
+         f"|A|/peak <= {rel[out].max() if out.any() else 0:.2e}, error/peak <= {err[out].max() if out.any() else 0:.2e}; "
+         f"largest error/peak anywhere {err.max():.2e}")
+    assert np.all(rel[out] < 1e-3)
+    assert np.all(err[out] <= 1e-6)
+    assert err.max() <= 1e-6
 
 
 def test_magnitude_dtypes_and_background_modes(xp):

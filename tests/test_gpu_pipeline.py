@@ -1,9 +1,11 @@
 """Tiled reconstruction on the hip device (fused per-row stitch + finalise, both spectral forms)
 against the oracle's restatement of Reconstructor.process_row.
 
-Tolerances: dB volumes quantised with one clim (the oracle's); at least 99.99 % of voxels
-within 1 LSB for the plain FP32 path and 99.9 % for the MFMA form; NaN masks identical; the
-fused row path equals the per-tile contribute path to 2e-6 dB.
+Tolerances: dB volumes quantised with one clim (the oracle's); against the float64 oracle, at
+least 99.99 % of voxels within 1 LSB for the plain FP32 path and 99.9 % for the MFMA form (the
+north star's bar), and the plain path also against the reference's float32 CPU path; NaN masks
+identical; the fused row path equals the per-tile contribute path to 2e-6 dB; every combination
+of overlapped rows and per-depth streams gives the same bits.
 """
 import numpy as np
 import pytest
@@ -48,7 +50,7 @@ def _tiled(xp, ny, nx, seed=3, bscan_dev=None):
     return rec, grid, lam, bref, tiles
 
 
-def _oracle_rows(rec, grid, lam, bref, tiles, rows, frames):
+def _oracle_rows(rec, grid, lam, bref, tiles, rows, frames, precision="float64"):
     geo = R.spectral_geometry(lam, bref, 1.33)
     ot = R.tiled_geometry(grid.x_centers_mm, grid.y_centers_mm, grid.z_depths_mm, grid.tile_range_x_mm,
                           grid.tile_range_y_mm, grid.n_x_px, grid.n_y_px, geo.z_um, 512.0, 1.0, crop=[0.0, 0.15])
@@ -61,7 +63,7 @@ def _oracle_rows(rec, grid, lam, bref, tiles, rows, frames):
             raws = [np.stack([tiles[(xi, yi, zi)][int(i)].cpu().numpy() for i in frames]) for xi, zi in ids]
         fr = np.arange(raws[0].shape[0]) if isinstance(frames, slice) else np.asarray(frames)
         out.append(R.reconstruct_row(raws, ids, fr, geo, ot, grid.x_centers_mm, grid.z_depths_mm, 512.0,
-                                     rec.cfg.focus_sigma, 25, None, "float32"))
+                                     rec.cfg.focus_sigma, 25, None, precision))
     return np.concatenate(out)
 
 
@@ -78,6 +80,7 @@ def test_small_tiled_volume_vs_oracle(xp):
     print("case: small tiled volume (3x3x4 tiles, 4 x 64 A-scans)", flush=True)
     rec, grid, lam, bref, tiles = _tiled(xp, 4, 64)
     ref = _oracle_rows(rec, grid, lam, bref, tiles, [0, 1, 2], slice(None))
+    ref32 = _oracle_rows(rec, grid, lam, bref, tiles, [0, 1, 2], slice(None), "float32")
     for form, bar in (("plain", 0.9999), ("mfma", 0.999)):
         rec.cfg.form = form
         res = rec.run()
@@ -86,8 +89,12 @@ def test_small_tiled_volume_vs_oracle(xp):
         assert db.shape == ref.shape
         assert np.array_equal(np.isfinite(db), np.isfinite(ref))
         frac, worst = _lsb(db, ref)
-        print(f"{form}: within 1 LSB of the oracle {frac:.6%} (max {worst})", flush=True)
+        frac32, worst32 = _lsb(db, ref32)
+        print(f"{form}: within 1 LSB of the float64 oracle {frac:.6%} (max {worst}); of the float32 CPU path "
+              f"{frac32:.6%} (max {worst32})", flush=True)
         assert frac >= bar
+        if form == "plain":
+            assert frac32 >= 0.9999
         # device clim and codes are exactly those of numpy on the device's own dB volume
         gmin, gmax, pcs = R.volume_clim(db)
         assert vol["clim"].cpu().numpy().tolist() == [np.float32(gmin), np.float32(gmax)]
@@ -103,17 +110,20 @@ def test_overlapped_rows_equal_serial_rows(xp):
     reconstructor drops while the side streams still read them."""
     print("case: overlapped vs serial volume rows", flush=True)
     rec, grid, lam, bref, tiles = _tiled(xp, 4, 64, seed=5)
+    combos = [(False, ()), (False, ("mfma", "plain")), (True, ()), (True, ("mfma", "plain")),
+              (True, ("mfma", "plain"))]
     for form in ("plain", "mfma"):
         rec.cfg.form = form
         got = {}
-        for overlap in (False, True, True):
+        for overlap, forms in combos:
+            print(f"  {form}: overlap_rows={overlap} concurrent_forms={forms}", flush=True)
             rec.overlap_rows = overlap
-            # the serial reference also keeps every tile's launch on the caller's stream
-            rec.concurrent_forms = ("mfma", "plain") if overlap else ()
+            rec.concurrent_forms = forms
             vol = rec.reconstruct_volume(rec.allocate_volume())
             mid_db, _ = rec.process_row(1)
-            got.setdefault(overlap, []).append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(),
-                                                mid_db.cpu().numpy()))
+            # the first combination is the serial reference: every launch on the caller's stream
+            got.setdefault(bool(overlap or forms), []).append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(),
+                                                              mid_db.cpu().numpy()))
         # tiles from a source that hands over device copies made beforehand on the caller's
         # stream, whose last reference goes as soon as the tile's launches are issued
         resident = rec.tiles
@@ -125,6 +135,7 @@ def test_overlapped_rows_equal_serial_rows(xp):
 
         rec.tiles = source
         rec.overlap_rows = True
+        rec.concurrent_forms = ("mfma", "plain")
         try:
             vol = rec.reconstruct_volume(rec.allocate_volume())
             got[True].append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy(), got[True][0][2]))
@@ -135,6 +146,42 @@ def test_overlapped_rows_equal_serial_rows(xp):
             assert np.array_equal(db.view(np.uint32), ref_db.view(np.uint32)), form
             assert np.array_equal(bits, ref_bits), form
             assert np.array_equal(mid.view(np.uint32), ref_mid.view(np.uint32)), form
+
+
+def test_tile_windows_same_bits_in_every_stream_arrangement(xp):
+    """Per-tile dispersion (tile_betas: the plain form points each launch at its tile's window,
+    the MFMA form rebuilds the depth's operator per tile) gives the same bits with every
+    combination of overlapped rows and per-depth streams as with every launch on the caller's
+    stream, in both forms; and a tile given the volume's own dispersion reconstructs as without
+    a tile window."""
+    print("case: per-tile windows, every stream arrangement", flush=True)
+    rec, grid, lam, bref, tiles = _tiled(xp, 4, 64, seed=6)
+    rng = np.random.default_rng(11)
+    betas = {k: bref * (1 + rng.uniform(-0.2, 0.2)) for k in tiles}
+    same = (1, 1, 2)
+    combos = [(False, ()), (False, ("mfma", "plain")), (True, ()), (True, ("mfma", "plain"))]
+    for form in ("plain", "mfma"):
+        rec.cfg.form = form
+        rec.set_tile_betas({})
+        rec.overlap_rows, rec.concurrent_forms = False, ()
+        plain_vol = rec.reconstruct_volume(rec.allocate_volume())["db"].cpu().numpy()
+        rec.set_tile_betas(betas)
+        outs = []
+        for overlap, forms in combos:
+            print(f"  {form}: overlap_rows={overlap} concurrent_forms={forms}", flush=True)
+            rec.overlap_rows, rec.concurrent_forms = overlap, forms
+            vol = rec.reconstruct_volume(rec.allocate_volume())
+            outs.append((vol["db"].cpu().numpy(), vol["bits"].cpu().numpy()))
+        for db, bits in outs[1:]:
+            assert np.array_equal(db.view(np.uint32), outs[0][0].view(np.uint32)), form
+            assert np.array_equal(bits, outs[0][1]), form
+        assert not np.array_equal(outs[0][0], plain_vol)          # the tile windows took effect
+        # one tile given the volume's own dispersion as its window: the same bits as no window
+        rec.set_tile_betas({same: bref})
+        rec.overlap_rows, rec.concurrent_forms = True, ("mfma", "plain")
+        only = rec.reconstruct_volume(rec.allocate_volume())["db"].cpu().numpy()
+        assert np.array_equal(only.view(np.uint32), plain_vol.view(np.uint32)), form
+    rec.set_tile_betas({})
 
 
 def test_fused_rows_equal_per_tile_path(xp):
@@ -166,10 +213,10 @@ def test_fused_rows_equal_per_tile_path(xp):
 
 def test_full_size_tiled_sampled_plane(xp):
     """The benchmark workload at full size (36 tiles x 100 x 1000 A-scans): one output plane of
-    every y row against the oracle's reconstruction of that plane, plus volume-wide properties,
-    in the plain form and then the MFMA form (at least 99.9 % of the sampled voxels within 1 LSB
-    of the float64 oracle, every code within 1 LSB of the plain volume's but for at most 1e-4 of
-    them)."""
+    every y row against the float64 oracle's reconstruction of that plane, plus volume-wide
+    properties, in the plain form (at least 99.99 % of the sampled voxels within 1 LSB) and then
+    the MFMA form (at least 99.9 %, and every code within 1 LSB of the plain volume's but for at
+    most 1e-4 of them)."""
     print("case: full-size tiled workload, sampled planes", flush=True)
     rec, grid, lam, bref, tiles = _tiled(xp, 100, 1000, seed=5)
     res = rec.run()

@@ -52,6 +52,18 @@ def test_srr_read_and_oracle(tmp_path, apod, scan):
     o = RI.read_srr(tmp_path, names)
     want = np.concatenate([payload[[5, 1], apod[0]:apod[1]], payload[[5, 1], scan[0]:scan[1]]], 1) & 0x0FFF
     assert np.array_equal(o, want.astype(np.int16))
+    # a payload one sample longer or shorter than size1 * scanend is an error, as in the
+    # reference's read_file (an odd trailing byte is not a sample)
+    p5 = t.path(5)
+    good = p5.read_bytes()
+    for bad in (good + b"\x00\x00", good[:-2], good + b"\x00"):
+        p5.write_bytes(bad)
+        if len(bad) == len(good) + 1:
+            t.read([5], np.empty((1, file_lines, 64), np.uint16))
+        else:
+            with pytest.raises(IOError):
+                t.read([5], np.empty((1, file_lines, 64), np.uint16))
+    p5.write_bytes(good)
 
 
 def test_srr_header_tokens():
