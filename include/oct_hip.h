@@ -211,6 +211,27 @@ int oct_spectral_repeat(const oct_plan* plan, const void* raw, int raw_dtype, in
 size_t oct_topfrac_metric_workspace_bytes(int n_bscans, int n_x, int n_z, int z_min);
 int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                        void* workspace, size_t workspace_bytes, void* stream);
+/* ---- the other sharpness metrics of the search (octrecon/estimation/dispersion.py:46-63 _metric)
+ * over I = |A|^2 (squared in float32, widened to float64) for all x and z >= z_min of each B-scan:
+ *   OCT_METRIC_TOP_PERCENTILE  log mean of the brightest 1 % (oct_topfrac_metric with frac 0.01)
+ *   OCT_METRIC_KURTOSIS        sum(I^2) / sum(I)^2
+ *   OCT_METRIC_ENTROPY         sum(p log(p + 1e-30)), p = I / sum(I) (the negative entropy)
+ *   OCT_METRIC_LEGACY_PEAK     max over z of the mean over x of log(|A| + 1e-12)
+ * out (n_bscans) float64; float64 sums with per-workgroup partials added in a fixed order, so
+ * results are deterministic; workspace of oct_sharpness_metric_workspace_bytes. */
+enum { OCT_METRIC_TOP_PERCENTILE = 0, OCT_METRIC_KURTOSIS = 1, OCT_METRIC_ENTROPY = 2, OCT_METRIC_LEGACY_PEAK = 3 };
+size_t oct_sharpness_metric_workspace_bytes(int metric, int n_bscans, int n_x, int n_z, int z_min);
+int oct_sharpness_metric(int metric, const float* mag, int n_bscans, int n_x, int n_z, int z_min, double* out,
+                         void* workspace, size_t workspace_bytes, void* stream);
+/* Exact order statistics per B-scan (the percentile and median of the B-scan signal score,
+ * octrecon/estimation/dispersion.py:145-150): out[b n_ranks + r] = the ranks[b n_ranks + r]-th
+ * largest (1-based, 1 <= rank <= n_x (n_z - z_min)) of the values over x and z >= z_min of B-scan
+ * b, in float64: |A|^2 squared in float32 when square != 0, else |A| (whose order is that of
+ * the float64 squares).  ranks: device int64 (n_bscans, n_ranks).  Radix selection as
+ * oct_topfrac_metric; workspace of oct_order_stats_workspace_bytes. */
+size_t oct_order_stats_workspace_bytes(int n_bscans, int n_ranks, int n_x, int n_z, int z_min);
+int oct_order_stats(const float* mag, int n_bscans, int n_x, int n_z, int z_min, int square, const long long* ranks,
+                    int n_ranks, double* out, void* workspace, size_t workspace_bytes, void* stream);
 /* No counterpart in the reference: oct_topfrac_metric for callers that cannot allocate device
  * memory (code under HIP graph capture, tight search loops, harnesses that forbid hipMalloc).
  * The same kernels and results as oct_topfrac_metric, with the workspace taken from a 4 MiB

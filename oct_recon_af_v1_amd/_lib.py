@@ -12,6 +12,7 @@ LIB_PATH = Path(__file__).resolve().parent / "liboct_hip.so"
 OCT_RAW_I16, OCT_RAW_U16, OCT_RAW_F32 = 0, 1, 2
 OCT_APOD_LINES, OCT_APOD_FRAME_MEAN = 0, 1
 OCT_OUT_MAG, OCT_OUT_COMPLEX, OCT_OUT_LINEARISED = 0, 1, 2
+OCT_METRICS = {"top_percentile": 0, "kurtosis": 1, "entropy": 2, "legacy_peak": 3}
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -70,6 +71,10 @@ SIGNATURES = {
     "oct_topfrac_metric_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "oct_topfrac_metric": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp, _sz, _vp]),
     "oct_topfrac_metric_noalloc": (_i, [_vp, _i, _i, _i, _i, ctypes.c_double, _vp, _vp]),
+    "oct_sharpness_metric_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "oct_sharpness_metric": (_i, [_i, _vp, _i, _i, _i, _i, _vp, _vp, _sz, _vp]),
+    "oct_order_stats_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "oct_order_stats": (_i, [_vp, _i, _i, _i, _i, _i, _vp, _i, _vp, _vp, _sz, _vp]),
     "oct_plan_use_windows": (_i, [_vp, _vp, _ll, _ll]),
     "oct_spectral_repeat": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _i, _vp, _ll, _vp, _vp]),
     "oct_depth_median": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp]),

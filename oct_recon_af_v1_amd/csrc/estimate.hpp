@@ -23,10 +23,13 @@ struct TopfracState {
     long long k_rem;      // rank (from the top, 1-based) of the threshold among keys sharing the prefix
 };
 
-__device__ __forceinline__ unsigned tf_key(const float* A, long long e, int w, int nz, int zmin) {
+// Selection key of element e (x-major over columns z >= zmin): the float32 bits of |A|^2 (the
+// top-fraction metric squares in float32) or of |A| itself (order statistics of |A|, whose
+// order is that of the float64 squares).  Non-negative floats order as their unsigned bits.
+__device__ __forceinline__ unsigned tf_key(const float* A, long long e, int w, int nz, int zmin, bool square) {
     const long long x = e / w, z = zmin + e - x * w;
     const float a = A[x * nz + z];
-    return __builtin_bit_cast(unsigned, a * a);
+    return __builtin_bit_cast(unsigned, square ? a * a : a);
 }
 
 __global__ __launch_bounds__(kTfThreads) void topfrac_init_kernel(unsigned* __restrict__ hist, TopfracState* st,
@@ -39,22 +42,24 @@ __global__ __launch_bounds__(kTfThreads) void topfrac_init_kernel(unsigned* __re
     }
 }
 
-// pass: bits [lo, lo + nbits) of keys whose bits above lo + nbits equal the prefix.  Grid (S, B).
+// pass: bits [lo, lo + nbits) of keys whose bits above lo + nbits equal the prefix.  Grid (S, V):
+// selection state b works on B-scan b / per (per > 1: several ranks of one B-scan).
 __global__ __launch_bounds__(kTfThreads) void topfrac_hist_kernel(const float* __restrict__ mag, int nX, int nz,
                                                                   int zmin, const TopfracState* __restrict__ st,
-                                                                  unsigned* __restrict__ hist, int lo, int nbits) {
+                                                                  unsigned* __restrict__ hist, int lo, int nbits,
+                                                                  int per, int square) {
     __shared__ unsigned h[kTfBins];
     const int b = blockIdx.y;
     for (int i = threadIdx.x; i < kTfBins; i += kTfThreads) h[i] = 0u;
     __syncthreads();
-    const float* A = mag + (long long)b * nX * nz;
+    const float* A = mag + (long long)(b / per) * nX * nz;
     const int w = nz - zmin;
     const long long M = (long long)nX * w;
     const int hi = lo + nbits;
     const unsigned prefix = st[b].prefix;
     const unsigned dmask = (1u << nbits) - 1u;
     for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
-        const unsigned u = tf_key(A, e, w, nz, zmin);
+        const unsigned u = tf_key(A, e, w, nz, zmin, square != 0);
         if (hi >= 32 || (u >> hi) == (prefix >> hi)) atomicAdd(&h[(u >> lo) & dmask], 1u);
     }
     __syncthreads();
@@ -124,7 +129,7 @@ __global__ __launch_bounds__(kTfThreads) void topfrac_sum_kernel(const float* __
     const unsigned thr = st[b].prefix;
     double sum = 0.0;
     for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
-        const unsigned u = tf_key(A, e, w, nz, zmin);
+        const unsigned u = tf_key(A, e, w, nz, zmin, true);
         if (u > thr) sum += (double)__builtin_bit_cast(float, u);
     }
     red[threadIdx.x] = sum;
@@ -146,6 +151,134 @@ __global__ __launch_bounds__(kTfThreads) void topfrac_final_kernel(const Topfrac
     for (int s = 0; s < S; ++s) total += part[(long long)b * S + s];
     total += (double)st[b].k_rem * (double)__builtin_bit_cast(float, st[b].prefix);
     out[b] = log(total / (double)kk);
+}
+
+// Order statistics: state v takes its own rank (1-based, from the top) ranks[v].
+__global__ __launch_bounds__(kTfThreads) void order_init_kernel(unsigned* __restrict__ hist, TopfracState* st,
+                                                                const long long* __restrict__ ranks, int V) {
+    const long long i = (long long)blockIdx.x * kTfThreads + threadIdx.x;
+    if (i < (long long)V * kTfBins) hist[i] = 0u;
+    if (i < V) {
+        st[i].prefix = 0u;
+        st[i].k_rem = ranks[i];
+    }
+}
+
+// out[v] = the selected key of state v as a float (|A|^2 in float32, or |A|), in float64.
+__global__ __launch_bounds__(kTfThreads) void order_final_kernel(const TopfracState* __restrict__ st, int V,
+                                                                 double* __restrict__ out) {
+    const int v = blockIdx.x * kTfThreads + threadIdx.x;
+    if (v < V) out[v] = (double)__builtin_bit_cast(float, st[v].prefix);
+}
+
+// Moment metrics (octrecon/estimation/dispersion.py:58-62), I = |A|^2 squared in float32 and
+// widened to float64, over x and z >= zmin.  Pass 1: per-workgroup partial sums of I and I^2,
+// part[(b S + s) 2 + {0, 1}].  Grid (S, B).
+__global__ __launch_bounds__(kTfThreads) void moments_kernel(const float* __restrict__ mag, int nX, int nz, int zmin,
+                                                             double* __restrict__ part) {
+    __shared__ double r1[kTfThreads], r2[kTfThreads];
+    const int b = blockIdx.y;
+    const float* A = mag + (long long)b * nX * nz;
+    const int w = nz - zmin;
+    const long long M = (long long)nX * w;
+    double s1 = 0.0, s2 = 0.0;
+    for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
+        const double I = (double)__builtin_bit_cast(float, tf_key(A, e, w, nz, zmin, true));
+        s1 += I;
+        s2 += I * I;
+    }
+    r1[threadIdx.x] = s1;
+    r2[threadIdx.x] = s2;
+    __syncthreads();
+    for (int h = kTfThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            r1[threadIdx.x] += r1[threadIdx.x + h];
+            r2[threadIdx.x] += r2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[((long long)b * gridDim.x + blockIdx.x) * 2] = r1[0];
+        part[((long long)b * gridDim.x + blockIdx.x) * 2 + 1] = r2[0];
+    }
+}
+
+// Pass 2 of the entropy metric: partial sums of p log(p + 1e-30), p = I / sum(I) (the total
+// from pass 1's partials, added in a fixed order by every workgroup).  ent[b S + s].  Grid (S, B).
+__global__ __launch_bounds__(kTfThreads) void entropy_kernel(const float* __restrict__ mag, int nX, int nz, int zmin,
+                                                             const double* __restrict__ part, double* __restrict__ ent) {
+    __shared__ double red[kTfThreads];
+    const int b = blockIdx.y;
+    const int S = gridDim.x;
+    double tot = 0.0;
+    for (int s = 0; s < S; ++s) tot += part[((long long)b * S + s) * 2];
+    const float* A = mag + (long long)b * nX * nz;
+    const int w = nz - zmin;
+    const long long M = (long long)nX * w;
+    double acc = 0.0;
+    for (long long e = (long long)blockIdx.x * kTfThreads + threadIdx.x; e < M; e += (long long)gridDim.x * kTfThreads) {
+        const double p = (double)__builtin_bit_cast(float, tf_key(A, e, w, nz, zmin, true)) / tot;
+        acc += p * log(p + 1e-30);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kTfThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ent[(long long)b * S + blockIdx.x] = red[0];
+}
+
+// kind 1 (kurtosis): sum I^2 / (sum I)^2; kind 2 (entropy): sum of the pass-2 partials.
+// Partials added in a fixed order.  Grid (ceil(B / 256)).
+__global__ __launch_bounds__(kTfThreads) void moments_final_kernel(const double* __restrict__ part,
+                                                                   const double* __restrict__ ent, int S, int B,
+                                                                   int kind, double* __restrict__ out) {
+    const int b = blockIdx.x * kTfThreads + threadIdx.x;
+    if (b >= B) return;
+    double s1 = 0.0, s2 = 0.0;
+    if (kind == 1) {
+        for (int s = 0; s < S; ++s) {
+            s1 += part[((long long)b * S + s) * 2];
+            s2 += part[((long long)b * S + s) * 2 + 1];
+        }
+        out[b] = s2 / (s1 * s1);
+    } else {
+        for (int s = 0; s < S; ++s) s1 += ent[(long long)b * S + s];
+        out[b] = s1;
+    }
+}
+
+// Legacy glass-slide metric (octrecon/estimation/dispersion.py:49-51): per depth z >= zmin the
+// mean over x of log(|A| + 1e-12) (the sum of float32 |A| + 1e-12 in float32, then log and mean
+// in float64), L[b, z - zmin]; then the maximum over z.  Grid (ceil(w / 256), B): one column per
+// thread, so the x loop reads along z across the workgroup.
+__global__ __launch_bounds__(kTfThreads) void logmean_kernel(const float* __restrict__ mag, int nX, int nz, int zmin,
+                                                             double* __restrict__ L) {
+    const int b = blockIdx.y;
+    const int w = nz - zmin;
+    const int z = blockIdx.x * kTfThreads + threadIdx.x;
+    if (z >= w) return;
+    const float* A = mag + (long long)b * nX * nz + zmin + z;
+    double acc = 0.0;
+    for (int x = 0; x < nX; ++x) acc += log((double)(A[(long long)x * nz] + 1e-12f));
+    L[(long long)b * w + z] = acc / (double)nX;
+}
+
+// out[b] = max_z L[b, z]; one workgroup per B-scan.
+__global__ __launch_bounds__(kTfThreads) void rowmax_kernel(const double* __restrict__ L, int w,
+                                                            double* __restrict__ out) {
+    __shared__ double red[kTfThreads];
+    const int b = blockIdx.x;
+    double m = -INFINITY;
+    for (int z = threadIdx.x; z < w; z += kTfThreads) m = fmax(m, L[(long long)b * w + z]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = kTfThreads / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = red[0];
 }
 
 // Dispersion search windows: out[b, j] = hann_n[j] exp(-i beta_b dk2[j]) (hann_n = Hann/rms/N,

@@ -667,7 +667,7 @@ int oct_topfrac_metric(const float* mag, int n_bscans, int n_x, int n_z, int z_m
     const int lo[3] = {20, 10, 0}, nb[3] = {11, 10, 10};
     for (int p = 0; p < 3; ++p) {
         hipLaunchKernelGGL(oct::topfrac_hist_kernel, dim3(S, n_bscans), dim3(oct::kTfThreads), 0, s, mag, n_x, n_z, z_min,
-                           st, hist, lo[p], nb[p]);
+                           st, hist, lo[p], nb[p], 1, 1);
         hipLaunchKernelGGL(oct::topfrac_select_kernel, dim3(n_bscans), dim3(oct::kTfThreads), 0, s, hist, st, lo[p], nb[p]);
     }
     hipLaunchKernelGGL(oct::topfrac_sum_kernel, dim3(S, n_bscans), dim3(oct::kTfThreads), 0, s, mag, n_x, n_z, z_min, st,
@@ -727,6 +727,89 @@ int oct_topfrac_metric_noalloc(const float* mag, int n_bscans, int n_x, int n_z,
     if (rc != OCT_OK) return rc;
     // the same kernels as oct_topfrac_metric; its first kernel clears the histograms and states
     return oct_topfrac_metric(mag, n_bscans, n_x, n_z, z_min, frac, out, ws, kNoAllocScratchBytes, stream);
+}
+
+size_t oct_order_stats_workspace_bytes(int n_bscans, int n_ranks, int n_x, int n_z, int z_min) {
+    if (n_bscans <= 0 || n_ranks <= 0 || n_x <= 0 || z_min < 0 || z_min >= n_z) return 0;
+    return (size_t)n_bscans * (size_t)n_ranks * (sizeof(unsigned) * oct::kTfBins + sizeof(oct::TopfracState));
+}
+
+int oct_order_stats(const float* mag, int n_bscans, int n_x, int n_z, int z_min, int square, const long long* ranks,
+                    int n_ranks, double* out, void* workspace, size_t workspace_bytes, void* stream) {
+    if (n_bscans <= 0 || n_ranks <= 0) return OCT_OK;
+    if (!mag || !ranks || !out || !workspace) return fail(OCT_EINVAL, "null argument");
+    if (n_x <= 0 || z_min < 0 || z_min >= n_z) return fail(OCT_EINVAL, "bad shape (%d, %d, zmin %d)", n_x, n_z, z_min);
+    const long long V = (long long)n_bscans * n_ranks;
+    if (V > 65535) return fail(OCT_EINVAL, "at most 65535 (B-scan, rank) pairs per call");
+    const size_t need = oct_order_stats_workspace_bytes(n_bscans, n_ranks, n_x, n_z, z_min);
+    if (workspace_bytes < need) return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 16) return fail(OCT_EINVAL, "workspace must be 16-byte aligned");
+    const long long M = (long long)n_x * (n_z - z_min);
+    const int S = topfrac_splits((int)V, M);
+    unsigned* hist = static_cast<unsigned*>(workspace);
+    auto* st = reinterpret_cast<oct::TopfracState*>(hist + (size_t)V * oct::kTfBins);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const long long ninit = V * oct::kTfBins;
+    hipLaunchKernelGGL(oct::order_init_kernel, dim3((unsigned)((ninit + oct::kTfThreads - 1) / oct::kTfThreads)),
+                       dim3(oct::kTfThreads), 0, s, hist, st, ranks, (int)V);
+    const int lo[3] = {20, 10, 0}, nb[3] = {11, 10, 10};
+    for (int p = 0; p < 3; ++p) {
+        hipLaunchKernelGGL(oct::topfrac_hist_kernel, dim3(S, (unsigned)V), dim3(oct::kTfThreads), 0, s, mag, n_x, n_z,
+                           z_min, st, hist, lo[p], nb[p], n_ranks, square ? 1 : 0);
+        hipLaunchKernelGGL(oct::topfrac_select_kernel, dim3((unsigned)V), dim3(oct::kTfThreads), 0, s, hist, st, lo[p],
+                           nb[p]);
+    }
+    hipLaunchKernelGGL(oct::order_final_kernel, dim3((unsigned)((V + oct::kTfThreads - 1) / oct::kTfThreads)),
+                       dim3(oct::kTfThreads), 0, s, st, (int)V, out);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+size_t oct_sharpness_metric_workspace_bytes(int metric, int n_bscans, int n_x, int n_z, int z_min) {
+    if (n_bscans <= 0 || n_x <= 0 || z_min < 0 || z_min >= n_z) return 0;
+    const long long M = (long long)n_x * (n_z - z_min);
+    switch (metric) {
+        case OCT_METRIC_TOP_PERCENTILE: return oct_topfrac_metric_workspace_bytes(n_bscans, n_x, n_z, z_min);
+        case OCT_METRIC_KURTOSIS:
+        case OCT_METRIC_ENTROPY: return sizeof(double) * 3 * (size_t)n_bscans * (size_t)topfrac_splits(n_bscans, M);
+        case OCT_METRIC_LEGACY_PEAK: return sizeof(double) * (size_t)n_bscans * (size_t)(n_z - z_min);
+        default: return 0;
+    }
+}
+
+int oct_sharpness_metric(int metric, const float* mag, int n_bscans, int n_x, int n_z, int z_min, double* out,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+    if (metric < OCT_METRIC_TOP_PERCENTILE || metric > OCT_METRIC_LEGACY_PEAK)
+        return fail(OCT_EINVAL, "unknown metric %d", metric);
+    if (metric == OCT_METRIC_TOP_PERCENTILE)
+        return oct_topfrac_metric(mag, n_bscans, n_x, n_z, z_min, 0.01, out, workspace, workspace_bytes, stream);
+    if (n_bscans <= 0) return OCT_OK;
+    if (!mag || !out || !workspace) return fail(OCT_EINVAL, "null argument");
+    if (n_x <= 0 || z_min < 0 || z_min >= n_z) return fail(OCT_EINVAL, "bad metric shape (%d, %d, zmin %d)", n_x, n_z, z_min);
+    if (n_bscans > 65535) return fail(OCT_EINVAL, "at most 65535 B-scans per call");
+    const size_t need = oct_sharpness_metric_workspace_bytes(metric, n_bscans, n_x, n_z, z_min);
+    if (workspace_bytes < need) return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes, need);
+    if (reinterpret_cast<uintptr_t>(workspace) % 16) return fail(OCT_EINVAL, "workspace must be 16-byte aligned");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    double* ws = static_cast<double*>(workspace);
+    const int T = oct::kTfThreads;
+    if (metric == OCT_METRIC_LEGACY_PEAK) {
+        const int w = n_z - z_min;
+        hipLaunchKernelGGL(oct::logmean_kernel, dim3((w + T - 1) / T, n_bscans), dim3(T), 0, s, mag, n_x, n_z, z_min, ws);
+        hipLaunchKernelGGL(oct::rowmax_kernel, dim3(n_bscans), dim3(T), 0, s, ws, w, out);
+    } else {
+        const long long M = (long long)n_x * (n_z - z_min);
+        const int S = topfrac_splits(n_bscans, M);
+        double* part = ws;
+        double* ent = ws + 2 * (size_t)n_bscans * S;
+        hipLaunchKernelGGL(oct::moments_kernel, dim3(S, n_bscans), dim3(T), 0, s, mag, n_x, n_z, z_min, part);
+        if (metric == OCT_METRIC_ENTROPY)
+            hipLaunchKernelGGL(oct::entropy_kernel, dim3(S, n_bscans), dim3(T), 0, s, mag, n_x, n_z, z_min, part, ent);
+        hipLaunchKernelGGL(oct::moments_final_kernel, dim3((n_bscans + T - 1) / T), dim3(T), 0, s, part, ent, S,
+                           n_bscans, metric == OCT_METRIC_KURTOSIS ? 1 : 2, out);
+    }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
 }
 
 int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,

@@ -19,10 +19,12 @@ import warnings
 import numpy as np
 
 from . import _lib
-from .estimation import depth_median, detect_peak_near
+from .estimation import depth_median
+from .focus import detect_peak_near, interface_from_profiles
 
 REJECT_FLOOR_PIX = 15          # smallest residual threshold of the outlier rejection (pixels)
 MAX_REJECT_ITERATIONS = 6
+THEIL_SEN_EFFICIENCY = 0.91    # relative efficiency of the Theil-Sen slope (standard errors)
 
 
 def _mround(x):
@@ -32,134 +34,153 @@ def _mround(x):
 
 
 def theil_sen(z, pix):
-    """(slope, intercept, has_slope): median of the slopes of all pairs at distinct z, then the
-    median of pix - slope z."""
+    """(slope, intercept, has_slope): the median of the slopes of every pair of points at distinct
+    depths, and the median of pix - slope z.  No pair at distinct depths: (nan, nan, False)."""
     z = np.asarray(z, float)
-    pix = np.asarray(pix, float)
-    i, j = np.triu_indices(len(z), 1)
-    dz = z[j] - z[i]
-    use = np.abs(dz) > 1e-9
-    if not use.any():
+    p = np.asarray(pix, float)
+    dz = np.subtract.outer(z, z)
+    pair = np.triu(np.abs(dz) > 1e-9, 1)
+    if not pair.any():
         return np.nan, np.nan, False
-    slope = float(np.median((pix[j][use] - pix[i][use]) / dz[use]))
-    return slope, float(np.median(pix - slope * z)), True
+    slope = float(np.median(np.subtract.outer(p, p)[pair] / dz[pair]))
+    return slope, float(np.median(p - slope * z)), True
+
+
+def _robust_line(z, pix, notes):
+    """Theil-Sen line through (z, pix) refitted after dropping points whose residual exceeds
+    max(3 x 1.4826 x MAD, REJECT_FLOOR_PIX), at most MAX_REJECT_ITERATIONS times and never below
+    3 points.  Returns (slope, intercept, kept indices, [(rejected index, residual)])."""
+    kept = np.arange(z.size)
+    dropped = []
+    if z.size == 1:
+        notes.append("one measurement: the focus is taken as constant")
+        return 0.0, float(pix[0]), kept, dropped
+    for attempt in range(MAX_REJECT_ITERATIONS + 1):
+        slope, icpt, ok = theil_sen(z[kept], pix[kept])
+        if not ok:
+            notes.append("every measurement at one stage depth: the focus is taken as constant")
+            return 0.0, float(np.median(pix[kept])), kept, dropped
+        if kept.size < 3 or attempt == MAX_REJECT_ITERATIONS:
+            break
+        res = pix[kept] - (slope * z[kept] + icpt)
+        spread = 1.4826 * np.median(np.abs(res - np.median(res)))
+        far = np.abs(res) > max(3 * spread, REJECT_FLOOR_PIX)
+        if not far.any():
+            break
+        dropped.extend(zip(kept[far].tolist(), res[far].tolist()))
+        kept = kept[~far]
+    return slope, icpt, kept, dropped
+
+
+def _line_quality(z, pix, slope, icpt):
+    """(standard error of the slope, R^2) of the line on its points; nan below 3 points (and
+    for a zero spread in z or pix)."""
+    n = z.size
+    if n < 3:
+        return np.nan, np.nan
+    res = pix - (slope * z + icpt)
+    sigma = np.sqrt(np.sum(res ** 2) / (n - 2))
+    sxx = np.sum((z - z.mean()) ** 2)
+    sst = np.sum((pix - pix.mean()) ** 2)
+    se = sigma / np.sqrt(THEIL_SEN_EFFICIENCY * sxx) if sxx > 0 else np.nan
+    r2 = 1 - np.sum(res ** 2) / sst if sst > 0 else np.nan
+    return se, r2
+
+
+def _regime(slope, used, se, ni, na, dz):
+    """Drift regime of a measured slope: 'suspicious-structure-clicks' when it is significantly
+    negative (the measurements followed a fixed structure: the tissue index cannot be below the
+    immersion medium's), 'no-measurable-drift' within 2 standard errors of 0,
+    'clamped-to-max' above the physical maximum, else 'normal'."""
+    if np.isnan(se) or se <= 0:
+        ratio = np.nan
+        structure = slope < -0.5 * (ni / na) / dz
+    else:
+        ratio = slope / se
+        structure = ratio < -3 and slope < -(0.1 * (ni / na) / dz)
+    if structure:
+        return "suspicious-structure-clicks"
+    if abs(ratio) < 2:                      # false for nan
+        return "no-measurable-drift"
+    if used != slope and slope > 0:
+        return "clamped-to-max"
+    return "normal"
 
 
 def fit_drift(z_clicked_mm, focus_clicked_pix, z_all_mm, z_pixel_um, n_z, tissue_ri=1.4, immersion_ri=1.33,
               max_tissue_ri=1.55):
-    """Focus pixel (1-based int) at every depth of z_all_mm and diagnostics, from measured focus
-    pixels at stage depths z_clicked_mm (octrecon/estimation/drift_fit.py:46-300).
+    """Focus pixel (1-based int) at every depth of z_all_mm, and diagnostics, from focus pixels
+    measured at stage depths z_clicked_mm: the model of octrecon/estimation/drift_fit.py:46-235.
 
-    Slope limits: 0 .. (max_ri^2 - n_i^2) / (n_i n_a) / dz pixels per um; a significantly
-    negative measured slope means the clicks followed a fixed structure ("suspicious").
-    Diagnostics use the reference's keys (driftRegime, keptIdx, rejectedIdx, slopeUsed_pixPerUm,
-    driftSlope, tissueRI, seAtDepth_pix, ...)."""
+    Measurements with NaN or at stage z < 0 (above the tissue the focus does not move) are left
+    out.  A robust Theil-Sen line (_robust_line) is clamped to the physically possible slopes
+    0 .. (n_max^2 - n_i^2) / (n_i n_a) / dz pixels per um, classified (_regime), and evaluated
+    with no drift above z = 0; its standard error at each depth uses the pi / 2 variance of a
+    median and the Theil-Sen efficiency.  The diagnostics keep the reference's keys
+    (slopeMeasured_pixPerUm, slopeUsed_pixPerUm, driftSlope, intercept_pix, tissueRI, keptIdx,
+    rejectedIdx, rejectedReason, wasSlopeClamped, driftRegime, seSlope_pixPerUm, seTissueRI,
+    r2, seAtDepth_pix, maxMeasuredZ_mm, zPixelSize_um, messages)."""
     na, ni, nmax = float(tissue_ri), float(immersion_ri), float(max_tissue_ri)
     dz = float(z_pixel_um)
     nZ = int(_mround(n_z))
     if dz <= 0 or nZ < 1:
         raise ValueError("z_pixel_um must be > 0 and n_z >= 1")
     notes = []
-    zc = np.atleast_1d(np.asarray(z_clicked_mm, float)).ravel() * 1e3
+    z_um = np.atleast_1d(np.asarray(z_clicked_mm, float)).ravel() * 1e3
     pc = np.atleast_1d(np.asarray(focus_clicked_pix, float)).ravel()
-    nan = np.isnan(zc) | np.isnan(pc)
-    if nan.any():
-        notes.append(f"ignored {int(nan.sum())} measurement(s) with NaN values")
-    above = (zc < 0) & ~nan
+    missing = np.isnan(z_um) | np.isnan(pc)
+    above = ~missing & (z_um < 0)
+    if missing.any():
+        notes.append(f"{int(missing.sum())} measurement(s) with NaN left out")
     if above.any():
-        notes.append(f"ignored {int(above.sum())} measurement(s) at stage z < 0 (above the tissue the focus "
-                     "does not drift)")
-    use = np.flatnonzero(~nan & ~above)
-    z, pix = zc[use], pc[use]
-    if len(z) == 0:
+        notes.append(f"{int(above.sum())} measurement(s) above the tissue (stage z < 0) left out: the focus does "
+                     "not drift there")
+    src = np.flatnonzero(~missing & ~above)
+    if src.size == 0:
         raise ValueError("no usable focus measurement")
+    z, pix = z_um[src], pc[src]
 
+    slope, icpt, kept, dropped = _robust_line(z, pix, notes)
+    se, r2 = _line_quality(z[kept], pix[kept], slope, icpt)
     m_max = (nmax ** 2 - ni ** 2) / (ni * na) / dz
-    kept = np.arange(len(z))
-    out_idx, out_res = [], []
-    if len(z) == 1:
-        slope, icpt = 0.0, float(pix[0])
-        notes.append("one measurement: constant focus")
-    else:
-        for round_ in range(MAX_REJECT_ITERATIONS + 1):
-            slope, icpt, ok = theil_sen(z[kept], pix[kept])
-            if not ok:
-                slope, icpt = 0.0, float(np.median(pix[kept]))
-                notes.append("all measurements at one stage depth: constant focus")
-                break
-            if len(kept) < 3 or round_ == MAX_REJECT_ITERATIONS:
-                break
-            r = pix[kept] - (slope * z[kept] + icpt)
-            mad = 1.4826 * np.median(np.abs(r - np.median(r)))
-            bad = np.abs(r) > max(3.0 * mad, REJECT_FLOOR_PIX)
-            if not bad.any():
-                break
-            out_idx.extend(kept[bad].tolist())
-            out_res.extend(r[bad].tolist())
-            kept = kept[~bad]
-
-    nk = len(kept)
-    se_slope, r2 = np.nan, np.nan
-    if nk >= 3:
-        res = pix[kept] - (slope * z[kept] + icpt)
-        s_res = np.sqrt(np.sum(res ** 2) / (nk - 2))
-        sxx = np.sum((z[kept] - z[kept].mean()) ** 2)
-        if sxx > 0:
-            se_slope = s_res / np.sqrt(0.91 * sxx)          # Theil-Sen efficiency 0.91
-        sst = np.sum((pix[kept] - pix[kept].mean()) ** 2)
-        if sst > 0:
-            r2 = 1.0 - np.sum(res ** 2) / sst
-
     used = min(max(slope, 0.0), m_max)
-    clamped = used != slope
-    unit = (ni / na) / dz
-    if np.isnan(se_slope) or se_slope <= 0:
-        sig = np.nan
-        structure = slope < -0.5 * unit
-    else:
-        sig = slope / se_slope
-        structure = sig < -3 and slope < -0.1 * unit
-    if structure:
-        regime = "suspicious-structure-clicks"
-        notes.append("significantly negative drift: the measurements followed a fixed structure; slope set to 0")
-    elif abs(sig) < 2:
-        regime = "no-measurable-drift"
-    elif clamped and slope > 0:
-        regime = "clamped-to-max"
-        notes.append(f"drift above the physical maximum (tissue index {nmax}): clamped")
-    else:
-        regime = "normal"
-    if clamped:
+    regime = _regime(slope, used, se, ni, na, dz)
+    if used != slope:
         icpt = float(np.median(pix[kept] - used * z[kept]))
+    if regime == "suspicious-structure-clicks":
+        notes.append("the drift slope is significantly negative: the measurements followed a fixed structure, not "
+                     "the focus; slope set to 0")
+    elif regime == "clamped-to-max":
+        notes.append(f"the drift slope exceeds the maximum for tissue index {nmax}: clamped")
+    ri = np.nan if regime == "suspicious-structure-clicks" else float(np.sqrt(max(ni ** 2 + (slope * dz) * ni * na,
+                                                                                    ni ** 2)))
+    se_ri = (se * dz) * ni * na / (2 * ri) if (np.isfinite(ri) and np.isfinite(se) and ri > 0) else np.nan
+    reasons = [f"measurement at z = {z[k] * 1e-3:.3f} mm is {r:+.0f} pixels from the drift line" for k, r in dropped]
+    notes.extend("left out: " + r for r in reasons)
 
-    ns2 = ni ** 2 + slope * dz * ni * na
-    ri = np.nan if structure else float(np.sqrt(max(ns2, ni ** 2)))
-    se_ri = (se_slope * dz) * ni * na / (2 * ri) if (np.isfinite(ri) and np.isfinite(se_slope) and ri > 0) else np.nan
-    for k, rr in zip(out_idx, out_res):
-        notes.append(f"rejected the measurement at z = {z[k] * 1e-3:.3f} mm ({rr:+.0f} pixels from the line)")
-
-    za = np.atleast_1d(np.asarray(z_all_mm, float)).ravel() * 1e3
-    zeff = np.maximum(za, 0.0)
-    focus = _mround(used * zeff + icpt)
+    z_eval = np.maximum(np.atleast_1d(np.asarray(z_all_mm, float)).ravel() * 1e3, 0.0)
+    focus = _mround(used * z_eval + icpt)
     if ((focus < 1) | (focus > nZ)).any():
-        notes.append("focus clamped to the image at some depths")
+        notes.append("the focus leaves the image at some depths: clamped to it")
     focus = np.clip(focus, 1, nZ).astype(int)
+    nk = kept.size
     if nk >= 3:
-        res = pix[kept] - (used * z[kept] + icpt)
-        s_res = np.sqrt(np.sum(res ** 2) / (nk - 2))
-        zbar = z[kept].mean()
-        sxx = np.sum((z[kept] - zbar) ** 2)
-        se_depth = (s_res * np.sqrt((np.pi / 2) / nk + (zeff - zbar) ** 2 / (0.91 * sxx)) if sxx > 0
-                    else np.full(zeff.shape, s_res * np.sqrt(np.pi / 2) / np.sqrt(nk)))
+        zk = z[kept]
+        res = pix[kept] - (used * zk + icpt)
+        sigma = np.sqrt(np.sum(res ** 2) / (nk - 2))
+        zbar = zk.mean()
+        sxx = np.sum((zk - zbar) ** 2)
+        se_depth = (sigma * np.sqrt((np.pi / 2) / nk + (z_eval - zbar) ** 2 / (THEIL_SEN_EFFICIENCY * sxx)) if sxx > 0
+                    else sigma * np.sqrt(np.pi / 2) * np.ones_like(z_eval) / np.sqrt(nk))
     else:
-        se_depth = np.full(zeff.shape, np.nan)
+        se_depth = np.full(z_eval.shape, np.nan)
     diag = {
         "slopeMeasured_pixPerUm": float(slope), "slopeUsed_pixPerUm": float(used), "driftSlope": float(used * dz),
-        "intercept_pix": float(icpt), "tissueRI": ri, "keptIdx": use[kept].tolist(),
-        "rejectedIdx": use[np.asarray(out_idx, int)].tolist(), "wasSlopeClamped": bool(clamped),
-        "driftRegime": regime, "seSlope_pixPerUm": float(se_slope), "seTissueRI": float(se_ri), "r2": float(r2),
-        "seAtDepth_pix": se_depth.tolist(), "maxMeasuredZ_mm": float(np.max(z[kept]) * 1e-3),
-        "zPixelSize_um": dz, "messages": notes,
+        "intercept_pix": float(icpt), "tissueRI": ri, "keptIdx": src[kept].tolist(),
+        "rejectedIdx": [int(src[k]) for k, _ in dropped], "rejectedReason": reasons,
+        "wasSlopeClamped": bool(used != slope), "driftRegime": regime, "seSlope_pixPerUm": float(se),
+        "seTissueRI": float(se_ri), "r2": float(r2), "seAtDepth_pix": se_depth.tolist(),
+        "maxMeasuredZ_mm": float(np.max(z[kept]) * 1e-3), "zPixelSize_um": dz, "messages": notes,
     }
     return focus, diag
 
@@ -212,18 +233,6 @@ def depth_profiles(xp, mag, frames=None, opc=None):
     return out.cpu().numpy()
 
 
-def _interface_pix(med):
-    """Per frame the brightest pixel of the x-median profile below the top third (first of
-    equal maxima), 1-based; the median over frames (octrecon/estimation/focus.py:209-218)."""
-    nz = med.shape[-1]
-    vals = []
-    for m in med:
-        m = m.copy()
-        m[: int(_mround(nz / 3))] = np.nan
-        vals.append(int(np.nanargmax(m)) + 1 if np.isfinite(m).any() else 1)
-    return int(_mround(np.median(vals))), vals
-
-
 def measure_focus_drift(xp, magnitude_of, frames, z_depths_mm, prior_pix, z_pixel_um, n_z, opc=None, tissue_ri=1.4,
                         step_um=50.0, first_um=50.0, search_px=60, min_prominence_db=0.3, log=None):
     """Focus pixel (1-based) per focus depth from the central tile of each measured depth.
@@ -248,7 +257,7 @@ def measure_focus_drift(xp, magnitude_of, frames, z_depths_mm, prior_pix, z_pixe
             warnings.simplefilter("ignore", RuntimeWarning)
             prof = np.nanmean(med, axis=0)
         if abs(z) <= 0.5 * step_mm + 1e-9:
-            pix, frame_vals = _interface_pix(med)
+            pix, frame_vals = interface_from_profiles(med)
             rule, prom, acc = "interface", None, True
         else:
             pix, prom = detect_peak_near(prof, guess, search_px)

@@ -300,7 +300,7 @@ def focus_from_profiles(profiles, search_px=None, guess=None):
     """Focus (0-based depth bin) and prominence (dB) of each depth profile by the reference's
     confocal detector (octrecon/estimation/focus.py:191-206); nan where no peak is found.
     Without a guess the whole depth range is searched."""
-    from .estimation import detect_peak_near
+    from .focus import detect_peak_near
     P = np.asarray(profiles, float)
     n = P.shape[-1]
     out = np.full(P.shape[0], np.nan)

@@ -591,6 +591,21 @@ class Reconstructor:
         vol["plane_clim"], vol["clim"] = pcl, cl
         return vol
 
+    def bscan_magnitudes(self, xi, yi, zi, frames):
+        """|A| of every depth row of tile (xi, yi, zi)'s frames at the volume's dispersion (repeats
+        averaged, binning applied): a new device tensor (len(frames), nX, N/2) float32."""
+        nz = self.sp.nz
+        raw = self.tile_frames(xi, yi, zi, frames)
+        dst = self.torch.empty((len(frames), self._positions(raw.shape[1]), nz), dtype=self.torch.float32,
+                               device=self.xp.device)
+        return self._magnitudes(raw, len(frames), 0, nz, dst)
+
+    def opc_tables(self):
+        """The optical-path correction's device tables (shift, valid_lo, valid_hi), each (nY, nX)
+        int32 per tile frame and A-scan (the same for every tile), or None without a correction."""
+        st = next(iter(self.stitchers.values()))
+        return (st.shift, st.vlo, st.vhi) if st.shift is not None else None
+
     def measure_focus_drift(self, prior_pix, n_frames: int = 5, tissue_ri: float = 1.4, log=None, **kw):
         """Focus pixel per focus depth measured on the central tile (octrecon/estimation/focus.py:
         221-279): magnitudes at the volume's dispersion (repeats averaged), optical-path corrected
@@ -600,18 +615,9 @@ class Reconstructor:
         xi0 = int(np.argmin(np.abs(self.scan.x_centers_mm)))
         yi0 = int(np.argmin(np.abs(self.scan.y_centers_mm)))
         frames = central_frames(self.scan.n_y_px, n_frames)
-        nz = self.sp.nz
-
-        def magnitude_of(zi):
-            raw = self.tile_frames(xi0, yi0, zi, frames)
-            dst = self.torch.empty((len(frames), self._positions(raw.shape[1]), nz), dtype=self.torch.float32,
-                                   device=self.xp.device)
-            return self._magnitudes(raw, len(frames), 0, nz, dst)
-
-        st = self.stitchers[(xi0, 0)]
-        opc = (st.shift, st.vlo, st.vhi) if st.shift is not None else None
         dz_um = float(np.median(np.diff(self.sg.z_um)))
-        return measure_focus_drift(self.xp, magnitude_of, frames, self.scan.z_depths_mm, prior_pix, dz_um, nz, opc=opc,
+        return measure_focus_drift(self.xp, lambda zi: self.bscan_magnitudes(xi0, yi0, zi, frames), frames,
+                                   self.scan.z_depths_mm, prior_pix, dz_um, self.sp.nz, opc=self.opc_tables(),
                                    tissue_ri=tissue_ri, log=log, **kw)
 
     def run(self, rows=None):

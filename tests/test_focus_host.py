@@ -2,8 +2,8 @@
 (tests/test_estimation.py:159-164) and a synthetic confocal profile."""
 import numpy as np
 
-from oct_recon_af_v1_amd.estimation import (depth_profile, detect_peak_near, findpeaks_matlab, imgaussfilt_1d,
-                                            interface_pix)
+from oct_recon_af_v1_amd.focus import (depth_profile, detect_peak_near, findpeaks_matlab, imgaussfilt_1d,
+                                       interface_pix)
 
 
 def test_matlab_helpers_known_answers():
