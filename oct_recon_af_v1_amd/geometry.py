@@ -277,53 +277,71 @@ class TiledGeometry:
     focus_pix: np.ndarray
 
 
+def _tile_axis(offset: float, span: float, n: int) -> np.ndarray:
+    """Tile-local positions of n samples over `span`, centred on `offset`: the first n nodes of
+    MATLAB linspace(-1/2, 1/2, n + 1) scaled (a sample sits at the start of its pixel)."""
+    return offset + span * matlab_linspace(-0.5, 0.5, n + 1)[:-1]
+
+
+def _mosaic_axis(centres, tile: np.ndarray) -> np.ndarray:
+    """Output positions of one lateral axis: MATLAB colon over every tile's samples on the tile
+    pitch (min centre + first sample ... max centre + last sample, half a pitch of slack); one
+    tile keeps its own axis."""
+    if len(centres) == 1:
+        return tile.copy()
+    step = tile[1] - tile[0] if len(tile) > 1 else 0.0
+    return matlab_colon(centres.min() + tile[0], step, centres.max() + tile[-1] + step / 2)
+
+
+def _depth_axes(z_um_tile, focus: np.ndarray, z_depths: np.ndarray):
+    """(tile-local z in mm, 0 at the reference focus row; output z before resampling).  The
+    reference focus row is the common one, or the one of the depth nearest z = 0; without
+    gating (any NaN focus) the tile axis stays at the spectrometer's.  The output axis covers
+    every depth's tile on the tile pitch and is re-anchored so that its node nearest 0 is 0."""
+    tz = np.asarray(z_um_tile, float) / 1e3
+    if not np.isnan(focus).any():
+        row = focus[0] if np.all(focus == focus[0]) else focus[int(np.argmin(np.abs(z_depths)))]
+        tz = tz - tz[int(np.round(row)) - 1]
+    pitch = tz[1] - tz[0]
+    span = matlab_colon(z_depths.min() + tz[0], pitch, z_depths.max() + tz[-1] + pitch / 2)
+    anchor = int(np.argmin(np.abs(span)))
+    return tz, pitch * (np.arange(1, span.size + 1) - (anchor + 1))
+
+
 def build_tiled_geometry(sg: ScanGrid, z_um_tile, focus_pix, output_pixel_size_um,
                          crop_z_range_mm=None) -> TiledGeometry:
-    """Reference: octrecon/core/geometry.py:146-202."""
+    """Tile and output grids of a tiled scan (the model of octrecon/core/geometry.py:146-202):
+    lateral tile axes from the scan's ranges (with the galvo phase-delay shift of x when the scan
+    states no explicit x correction), the depth axes of _depth_axes, the output axes of
+    _mosaic_axis; an output pixel size must equal the scan's lateral pitch (rounded to 0.01 um)
+    and resamples z to it; crop_z_range_mm = [lo, hi] keeps the output depths inside it.  The
+    grids' float64 expressions are the reference's, so the grid nodes (and every interpolation
+    weight built on them) come out identical."""
     nd = len(sg.z_depths_mm)
     focus = np.asarray(focus_pix, float).ravel()
     if focus.size == 1:
         focus = np.full(nd, focus[0])
     if focus.size != nd:
         raise ValueError("focus_pix must be scalar or one value per zDepth")
-
-    tx = sg.x_offset + sg.tile_range_x_mm * matlab_linspace(-0.5, 0.5, sg.n_x_px + 1)[:-1]
-    ty = sg.y_offset + sg.tile_range_y_mm * matlab_linspace(-0.5, 0.5, sg.n_y_px + 1)[:-1]
-    if not sg.galvo_x_correction_given and sg.galvo_phase_delay_asamples != 0 \
-            and np.isfinite(sg.pixel_size_um):
-        tx = tx - sg.galvo_phase_delay_asamples * (sg.pixel_size_um - 1.0) * 1e-3
-
-    tz = np.asarray(z_um_tile, float) / 1e3
-    if not np.any(np.isnan(focus)):
-        ref = focus[0] if np.all(focus == focus[0]) else focus[int(np.argmin(np.abs(sg.z_depths_mm)))]
-        tz = tz - tz[int(np.round(ref)) - 1]
-
-    dx = tx[1] - tx[0]
-    dy = ty[1] - ty[0] if len(ty) > 1 else 0.0
-    dz = tz[1] - tz[0]
-    xc, yc, zd = sg.x_centers_mm, sg.y_centers_mm, sg.z_depths_mm
-    x_all = matlab_colon(xc.min() + tx[0], dx, xc.max() + tx[-1] + dx / 2)
-    y_all = matlab_colon(yc.min() + ty[0], dy, yc.max() + ty[-1] + dy / 2)
-    z_all = matlab_colon(zd.min() + tz[0], dz, zd.max() + tz[-1] + dz / 2)
-    zero = int(np.argmin(np.abs(z_all)))
-    z_all = dz * (np.arange(1, len(z_all) + 1) - (zero + 1))
-    if len(xc) == 1:
-        x_all = tx.copy()
-    if len(yc) == 1:
-        y_all = ty.copy()
-    z_full = z_all.copy()
-
+    tx = _tile_axis(sg.x_offset, sg.tile_range_x_mm, sg.n_x_px)
+    ty = _tile_axis(sg.y_offset, sg.tile_range_y_mm, sg.n_y_px)
+    delay = sg.galvo_phase_delay_asamples
+    if not sg.galvo_x_correction_given and delay != 0 and np.isfinite(sg.pixel_size_um):
+        tx = tx - delay * (sg.pixel_size_um - 1.0) * 1e-3
+    tz, z_full = _depth_axes(z_um_tile, focus, sg.z_depths_mm)
+    out_x = _mosaic_axis(sg.x_centers_mm, tx)
+    out_y = _mosaic_axis(sg.y_centers_mm, ty)
+    out_z = z_full
     if output_pixel_size_um is not None:
-        px = np.round(np.mean(np.diff(x_all)) * 1e3 * 100) / 100
-        py = np.round(np.mean(np.diff(y_all)) * 1e3 * 100) / 100 if len(y_all) >= 2 else px
+        px = np.round(np.mean(np.diff(out_x)) * 1e3 * 100) / 100
+        py = np.round(np.mean(np.diff(out_y)) * 1e3 * 100) / 100 if len(out_y) >= 2 else px
         if not (px == py == output_pixel_size_um):
-            raise ValueError(f"output pixel size {output_pixel_size_um} um must equal the scan pixel "
-                             f"size ({px}, {py}) um")
-        z_all = matlab_colon(z_all[0], output_pixel_size_um * 1e-3, z_all.max())
-
+            raise ValueError(f"output pixel size {output_pixel_size_um} um must equal the scan pixel size "
+                             f"({px}, {py}) um")
+        out_z = matlab_colon(z_full[0], output_pixel_size_um * 1e-3, z_full.max())
     if crop_z_range_mm is not None and len(crop_z_range_mm):
         lo, hi = crop_z_range_mm
-        z_all = z_all[~((z_all < lo) | (z_all > hi))]
-        if z_all.size == 0:
+        out_z = out_z[(out_z >= lo) & (out_z <= hi)]
+        if out_z.size == 0:
             raise ValueError("crop z range does not overlap the available z range")
-    return TiledGeometry(tx, ty, tz, x_all, y_all, z_all, z_full, focus)
+    return TiledGeometry(tx, ty, tz, out_x, out_y, out_z, z_full.copy(), focus)

@@ -81,15 +81,20 @@ class OpticalPathCorrection:
 
     @classmethod
     def build(cls, poly, tile_x_mm, tile_y_mm, tile_z_mm):
-        x = np.asarray(tile_x_mm) * 1e3
-        y = np.asarray(tile_y_mm) * 1e3
-        z = np.asarray(tile_z_mm) * 1e3
-        dz = (z[-1] - z[0]) / (len(z) - 1)
-        X, Y = np.meshgrid(x, y)
-        p = np.asarray(poly, float).ravel()
-        P = p[0] * X + p[1] * Y + p[2] * X ** 2 + p[3] * Y ** 2 + p[4] * X * Y
-        pdz = P / dz
-        return cls(np.floor(pdz + 0.5).astype(np.int64), pdz, len(z))
+        """Tables for the polynomial poly = (p1..p5) over the tile's (frame, A-scan) grid: the
+        path in depth pixels P / dz (dz the mean pitch of the tile's depth axis) and its nearest
+        row offset (halves round up).  The polynomial is evaluated term by term as written above,
+        in float64, so the offsets and the validity edges match the reference's bit for bit."""
+        c = np.asarray(poly, float).ravel()
+        if c.size != 5:
+            raise ValueError("the optical-path polynomial has 5 coefficients")
+        xu = np.asarray(tile_x_mm, float)[None, :] * 1e3          # (1, nX) um
+        yu = np.asarray(tile_y_mm, float)[:, None] * 1e3          # (nY, 1) um
+        zu = np.asarray(tile_z_mm, float) * 1e3
+        pitch = (zu[-1] - zu[0]) / (zu.size - 1)
+        path = c[0] * xu + c[1] * yu + c[2] * xu ** 2 + c[3] * yu ** 2 + c[4] * xu * yu
+        rows = path / pitch
+        return cls(np.floor(rows + 0.5).astype(np.int64), rows, zu.size)
 
     def valid_bounds(self):
         """Integer (lo, hi): row r is valid iff lo <= r <= hi, decided with the same float64

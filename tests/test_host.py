@@ -58,6 +58,34 @@ def test_tiled_geometry_matches_oracle():
     assert tg.out_z_mm.shape[0] == c["plane_db"].shape[0]
 
 
+@pytest.mark.parametrize("case", range(5))
+def test_tiled_geometry_and_opc_tables_equal_oracle_on_grids(case):
+    """Grids of several shapes (one tile per axis, uneven focus per depth, crop, y overlap-free
+    rows) and random optical-path polynomials: every grid node and every table entry identical
+    to the oracle's restatement (bit for bit)."""
+    rng = np.random.default_rng(case)
+    lam = SY.SpectrumSpec().lambda_nm()
+    sg = G.build_spectral_geometry(lam, 3e7, 1.33)
+    ntx, nty, nd = [(3, 3, 4), (1, 2, 3), (4, 1, 1), (2, 5, 6), (1, 1, 2)][case]
+    nx, ny = int(rng.integers(20, 200)), int(rng.integers(2, 30))
+    px_um = 1.0
+    xc = (np.arange(ntx) - (ntx - 1) / 2) * nx * 1e-3 * 0.9
+    yc = (np.arange(nty) - (nty - 1) / 2) * ny * 1e-3
+    zd = np.arange(nd) * 0.05 - 0.05 * (case % 2)
+    focus = 512.0 if case % 2 == 0 else np.linspace(480, 540, nd).round()
+    crop = [0.0, 0.15] if case in (0, 3) else None
+    grid = G.ScanGrid(xc, yc, zd, nx * 1e-3, ny * 1e-3, nx, ny, pixel_size_um=px_um)
+    tg = G.build_tiled_geometry(grid, sg.z_um, focus, px_um, crop)
+    ot = R.tiled_geometry(xc, yc, zd, nx * 1e-3, ny * 1e-3, nx, ny, sg.z_um, focus, px_um, crop=crop)
+    for a, b in ((tg.tile_x_mm, ot.tx), (tg.tile_y_mm, ot.ty), (tg.tile_z_mm, ot.tz), (tg.out_x_mm, ot.out_x),
+                 (tg.out_y_mm, ot.out_y), (tg.out_z_mm, ot.out_z)):
+        assert np.array_equal(a, b)
+    poly = rng.normal(0, [0.05, 0.05, 2e-4, 2e-4, 1e-4])
+    oc = S.OpticalPathCorrection.build(poly, tg.tile_x_mm, tg.tile_y_mm, tg.tile_z_mm)
+    shift, pdz, nz = R.opc_build(poly, tg.tile_x_mm, tg.tile_y_mm, tg.tile_z_mm)
+    assert np.array_equal(oc.shift, shift) and np.array_equal(oc.pdz, pdz) and oc.n_rows == nz
+
+
 def test_stitch_taps_equal_dense_operators():
     c = legacy_case.load("telesto")
     s = c["scan"]
