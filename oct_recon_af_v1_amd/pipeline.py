@@ -537,7 +537,9 @@ class Reconstructor:
     def volume_row(self, vol, k, cancel=None):
         """The k-th requested row of a volume (fused path): its planes of vol["db"] and its clim
         partials.  Rows may be given in any order and at any time before finish_volume; used to
-        reconstruct a streamed volume row by row as its tiles arrive."""
+        reconstruct a streamed volume row by row as its tiles arrive.  Returns an event that
+        completes once the device has finished with the row's raw tiles (its stitch, which
+        follows every spectral launch of the row)."""
         if not self.cfg.fused_rows:
             raise ValueError("row-by-row reconstruction needs the fused row path")
         ny = self.scan.n_y_px
@@ -546,7 +548,9 @@ class Reconstructor:
         frames = np.arange(ny)
         if not self.overlap_rows:
             self._stitch_row(self._row_magnitudes(vol["rows"][k], frames, cancel), frames, vol["db"], k * ny, partial)
-            return
+            done = self.torch.cuda.Event()
+            done.record(self.torch.cuda.current_stream(self.xp.device))
+            return done
         parity = self._row_seq % 2
         self._row_seq += 1
         if self._stitch_stream is None:
@@ -555,6 +559,7 @@ class Reconstructor:
         tiles = self._row_magnitudes(vol["rows"][k], frames, cancel, parity=parity, events=ends)
         self._stitch_done[parity] = self._stitch_row(tiles, frames, vol["db"], k * ny, partial,
                                                      stream=self._stitch_stream, after=ends)
+        return self._stitch_done[parity]
 
     def finish_volume(self, vol):
         """Clims and codes of a volume whose rows were all made by volume_row (vol["db"] is

@@ -3,8 +3,10 @@
 One process drives one GPU with three threads:
   producer  the host generator (liboct_hostgen, OpenMP, GIL released) fills a ring of pinned
             host tiles;
-  copier    each pinned tile goes to the volume's resident raw array in HBM with an
-            asynchronous copy on its own stream; the slot is handed back when the copy is done;
+  copier    each pinned tile goes to its row's slot of the raw row ring in HBM with an
+            asynchronous copy on its own stream; the pinned slot is handed back when the copy is
+            done; a row slot is refilled only after the device has finished reading the row
+            that held it (the copy stream waits for that row's events);
   main      as soon as a y-tile row has landed, the dispersion and focus of each of its tiles
             come from the learned estimator on `est_bscans` B-scans per tile (the sharpness
             search and confocal detector for the B-scans it is not confident about), and the
@@ -18,9 +20,12 @@ The reference reads raw files with io_threads reader threads and a bounded read-
 (octrecon/pipeline.py:197-212, 240-288), copies each batch with a pageable default-stream copy
 and runs spectral + stitching per batch; dispersion and focus are measured before the run
 (octrecon/estimation/dispersion.py:154-267, octrecon/estimation/focus.py:221-279) and one
-dispersion is applied to the whole volume.  The whole raw volume stays resident here (a 12 x 9
-x 8 tile volume of 40 x 1000 A-scans is 145 GB of uint16, half of an MI355X's HBM), so the
-reconstruction after estimation reads HBM, not the host link.
+dispersion is applied to the whole volume.  Here the raw tiles live in a ring of
+`resident_rows` y-tile rows in HBM (default: every row, so that reconstructing after the stream
+reads HBM, not the host link; a 12 x 9 x 8 tile volume of 40 x 1000 A-scans is 145 GB of uint16).
+With fewer resident rows the device holds that many rows of raw data at a time, so a volume of
+any size streams through; the plain FP32 validation then runs row by row beside the MFMA
+reconstruction, before the row's slot is refilled.
 
     python -m oct_recon_af_v1_amd.stream [--tiles-x 12 --tiles-y 9 --depths 8 --bscans 40 --ascans 1000]
 """
@@ -67,10 +72,11 @@ class StreamedVolume:
                  fspec: SY.FocusSpec = SY.FocusSpec(), hspec: SY.HostTileSpec = SY.HostTileSpec(), ring: int = 8,
                  gen_threads: int | None = None, estimator: str = "learned", est_bscans: int = 8,
                  form: str = "mfma", log=None, readers: dict | None = None, io_threads: int = 8,
-                 rowwise: bool = True):
+                 rowwise: bool = True, resident_rows: int | None = None):
         """readers: optional {(xi, yi, zi): rawio.OcityTile | rawio.SrrTile} streaming the tiles
         from their raw files (n_y files each) instead of the host generator; the generator's
-        per-tile dispersion / focus stay the 'given' estimates."""
+        per-tile dispersion / focus stay the 'given' estimates.  resident_rows: y-tile rows of
+        raw data held in HBM at once (default all; at least 1; rowwise=False needs all)."""
         if estimator not in ("learned", "given"):
             raise ValueError(f"estimator must be learned|given, got {estimator!r}")
         if ring < 2:
@@ -119,7 +125,16 @@ class StreamedVolume:
             self.io_pool = ThreadPoolExecutor(max(1, int(io_threads)))
         # files hold int16 counts (Thorlabs); the generator writes uint16
         self.raw_dtype = torch.int16 if readers is not None else torch.uint16
-        self.raw = torch.empty((n, ss.n_y, self.lines, self.n), dtype=self.raw_dtype, device=xp.device)
+        self.per_row = len(self.grid.tiles_in_row(0))
+        R = ss.tiles_y if resident_rows is None else int(resident_rows)
+        if not 1 <= R <= ss.tiles_y:
+            raise ValueError(f"resident_rows must lie in [1, {ss.tiles_y}]")
+        if not rowwise and R < ss.tiles_y:
+            raise ValueError("reconstructing after the stream needs every row resident")
+        self.resident_rows = R
+        self.slots = torch.empty((R, self.per_row, ss.n_y, self.lines, self.n), dtype=self.raw_dtype,
+                                 device=xp.device)
+        self.tile_map = {}                     # key -> the device tile of a row being reconstructed
         self.ring = [torch.empty(ss.n_y * slot_lines * self.n, dtype=self.raw_dtype, pin_memory=True)
                      for _ in range(ring)]
         self._payload = None
@@ -134,10 +149,18 @@ class StreamedVolume:
             self.est_x = 256                            # the network's crop width
             if ss.n_x < self.est_x:
                 raise ValueError(f"the estimator needs B-scans of at least {self.est_x} A-scans")
-            self.fallback = HipDispersionScorer(self.lam, self.raw[0, :1, :spec.apod + self.est_x], spec.apod, xp)
+            self.fallback = HipDispersionScorer(self.lam, self.slots[0, 0, :1, :spec.apod + self.est_x], spec.apod,
+                                                xp)
         self._est_buf = None
         self._err = None
         self.rowwise = bool(rowwise)
+        self._release = {}                     # row -> (threading.Event, [device events]) of its slot
+        self._release_lock = threading.Lock()
+
+    def tile_raw(self, i: int):
+        """Device raw tile i (n_y, lines, N): its row's slot of the ring (valid while that row is
+        resident)."""
+        return self.slots[(i // self.per_row) % self.resident_rows, i % self.per_row]
 
     # ------------------------------------------------------------------ threads
     def _guard(self, fn, *args):
@@ -185,10 +208,10 @@ class StreamedVolume:
                 self._payload = torch.empty(src.shape, dtype=torch.int16, device=self.xp.device)
             self._payload.copy_(src, non_blocking=True)
             _lib.check(_lib.lib().oct_srr_unpack(self._payload.data_ptr(), self.ss.n_y, r.file_lines, self.n, r.apod[0],
-                                                 r.apod[1], r.scan[0], r.scan[1], self.raw[i].data_ptr(),
+                                                 r.apod[1], r.scan[0], r.scan[1], self.tile_raw(i).data_ptr(),
                                                  stream.cuda_stream), "oct_srr_unpack")
         else:
-            self.raw[i].copy_(src, non_blocking=True)
+            self.tile_raw(i).copy_(src, non_blocking=True)
 
     def _copy(self, ready_q, free_q, landed_q, stats):
         """Issue each ready tile's copy at once (copies queue back to back on the copy stream),
@@ -198,16 +221,19 @@ class StreamedVolume:
         stream = torch.cuda.Stream(device=self.xp.device)
         pending = deque()                     # (tile, slot, start event, end event, queued behind)
         done = False
+        def report():
+            i, slot, e0, e1, prev = pending.popleft()
+            t = e0.elapsed_time(e1)                 # from the later of its record and the
+            if prev is not None:                    # previous copy's end: its own time
+                t = min(t, prev.elapsed_time(e1))
+            stats["h2d_s"] += t / 1e3
+            free_q.put(slot)
+            landed_q.put(i)
+
         with torch.cuda.stream(stream):
             while not done or pending:
                 while pending and pending[0][3].query():
-                    i, slot, e0, e1, prev = pending.popleft()
-                    t = e0.elapsed_time(e1)                 # from the later of its record and the
-                    if prev is not None:                    # previous copy's end: its own time
-                        t = min(t, prev.elapsed_time(e1))
-                    stats["h2d_s"] += t / 1e3
-                    free_q.put(slot)
-                    landed_q.put(i)
+                    report()
                 if done:
                     time.sleep(1e-4)          # the GIL stays free for the others
                     continue
@@ -219,6 +245,18 @@ class StreamedVolume:
                     done = True
                     continue
                 i, slot = item
+                row = i // self.per_row
+                if i % self.per_row == 0 and row >= self.resident_rows:
+                    # the row slot is refilled only after the device has read the row that held
+                    # it; the copies in flight (earlier rows) are reported first, since the main
+                    # thread queues a row (and frees its slot) only once all its tiles have landed
+                    while pending:
+                        pending[0][3].synchronize()
+                        report()
+                    flag, evs = self._release_of(row - self.resident_rows)
+                    flag.wait()
+                    for ev in evs:
+                        stream.wait_event(ev)
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 prev = pending[-1][3] if pending else None
@@ -227,6 +265,17 @@ class StreamedVolume:
                 e1.record(stream)
                 pending.append((i, slot, e0, e1, prev))
         landed_q.put(None)
+
+    def _release_of(self, row):
+        """(host flag, device events) that free row `row`'s slot: set by the main thread once the
+        row's reconstruction is queued (or on failure, with no events)."""
+        with self._release_lock:
+            return self._release.setdefault(row, (threading.Event(), []))
+
+    def _release_row(self, row, events):
+        flag, evs = self._release_of(row)
+        evs.extend(events)
+        flag.set()
 
     # ------------------------------------------------------------------ estimation
     def _gather(self, idx):
@@ -238,12 +287,12 @@ class StreamedVolume:
         if self._est_buf is None or tuple(self._est_buf.shape) != shape:
             self._est_buf = torch.empty(shape, dtype=self.raw_dtype, device=self.xp.device)
         buf = self._est_buf.view(torch.int16)             # uint16 bits moved as int16
-        rv = self.raw.view(torch.int16)
         fr = torch.as_tensor(np.round(np.linspace(0, self.ss.n_y - 1, E)).astype(np.int64), device=self.xp.device)
         x0 = a + (self.ss.n_x - w) // 2
         for k, i in enumerate(idx):
-            buf[k * E:(k + 1) * E, :a] = rv[i, :, :a].index_select(0, fr)
-            buf[k * E:(k + 1) * E, a:] = rv[i, :, x0:x0 + w].index_select(0, fr)
+            rv = self.tile_raw(i).view(torch.int16)
+            buf[k * E:(k + 1) * E, :a] = rv[:, :a].index_select(0, fr)
+            buf[k * E:(k + 1) * E, a:] = rv[:, x0:x0 + w].index_select(0, fr)
         return self._est_buf
 
     def _estimate(self, idx, stats):
@@ -288,10 +337,11 @@ class StreamedVolume:
         cfg = ReconConfig(dispersion_quadratic_term=b0, focus_positions=[float(f) for f in focus_per_depth],
                           focus_sigma=ss.focus_sigma_um / dz_um, crop_z_range_mm=list(ss.crop_z_mm),
                           output_pixel_size_um=1.0, n_medium=1.33, batch_frames=ss.n_y, form=form or self.form)
-        tiles = {k: self.raw[i] for i, k in enumerate(self.keys)}
         use = range(len(self.keys)) if idx is None else idx
+        for i in use:
+            self.tile_map[self.keys[i]] = self.tile_raw(i)
         betas = {self.keys[i]: float(tile_beta_idx[i]) * self.to_ref for i in use}
-        return Reconstructor(cfg, self.grid, self.lam, tiles, self.spec.apod, xp=self.xp, tile_betas=betas)
+        return Reconstructor(cfg, self.grid, self.lam, self.tile_map, self.spec.apod, xp=self.xp, tile_betas=betas)
 
     def focus_per_depth(self, tile_focus, idx=None):
         """1-based focus pixel per focus plane: median over the plane's tiles (0-based bins + 1),
@@ -327,7 +377,10 @@ class StreamedVolume:
         landed = np.zeros(n, bool)
         done_rows = set()
         t_est = t_rows = 0.0
-        rec = vol = fpd = None
+        rec = vol = fpd = ref = vref = None
+        # with part of the rows resident the plain validation must run row by row, before the
+        # row's slot is refilled
+        inline_check = validate and self.resident_rows < ss.tiles_y
         try:
             while True:
                 i = landed_q.get()
@@ -349,17 +402,31 @@ class StreamedVolume:
                         # reconstruct this row now, while the next rows stream; the focus per
                         # depth comes from the first row (every row holds all depths)
                         tr = time.perf_counter()
+                        for i in idx:
+                            self.tile_map[self.keys[i]] = self.tile_raw(i)
+                        betas = {self.keys[i]: float(tile_beta[i]) * self.to_ref for i in idx}
                         if rec is None:
                             fpd = self.focus_per_depth(tile_focus, idx)
                             rec = self.reconstructor(tile_beta, fpd, idx=idx)
                             vol = rec.allocate_volume()
+                            if inline_check:
+                                ref = self.reconstructor(tile_beta, fpd, form="plain", idx=idx)
+                                vref = ref.allocate_volume()
                         else:
-                            rec.set_tile_betas({self.keys[i]: float(tile_beta[i]) * self.to_ref for i in idx},
-                                               update=True)
-                        rec.volume_row(vol, yi)
+                            rec.set_tile_betas(betas, update=True)
+                            if ref is not None:
+                                ref.set_tile_betas(betas, update=True)
+                        evs = [rec.volume_row(vol, yi)]
+                        if ref is not None:
+                            evs.append(ref.volume_row(vref, yi))
+                        main_ev = torch.cuda.Event()          # the estimator's reads of the row
+                        main_ev.record(torch.cuda.current_stream(self.xp.device))
+                        self._release_row(yi, evs + [main_ev])
                         t_rows += time.perf_counter() - tr
         finally:
             free_q.put(None)
+            for yi in range(ss.tiles_y):                  # a failed run never leaves the copier waiting
+                self._release_of(yi)[0].set()
             for t in threads:
                 t.join(timeout=60)
         if self._err is not None:
@@ -376,7 +443,7 @@ class StreamedVolume:
         self.xp.synchronize()
         t1 = time.perf_counter()
         total = n * ss.n_y * ss.n_x
-        raw_bytes = float(self.raw.numel() * 2)
+        raw_bytes = float(n * ss.n_y * self.lines * self.n * 2)
         out = {
             "ascans": total, "tiles": n, "raw_GB": raw_bytes / 1e9,
             "seconds": t1 - t0, "ascans_per_s": total / (t1 - t0),
@@ -393,10 +460,16 @@ class StreamedVolume:
             "focus_err_bins_median": float(np.median(np.abs(tile_focus - self.truth["focus"]))),
             "focus_per_depth": fpd, "output_shape_yzx": list(vol["db"].shape), "form": self.form,
             "source": "files" if self.readers is not None else "host generator",
+            "resident_rows": self.resident_rows, "raw_resident_GB": float(self.slots.numel() * 2) / 1e9,
+            "validation": ("plain FP32 row by row beside the reconstruction (inside the timed run)" if inline_check
+                           else "plain FP32 volume after the run" if validate else None),
         }
         if validate:
-            ref = self.reconstructor(tile_beta, fpd, form="plain")
-            vref = ref.reconstruct_volume(ref.allocate_volume())
+            if inline_check:
+                vref = ref.finish_volume(vref)
+            else:
+                ref = self.reconstructor(tile_beta, fpd, form="plain")
+                vref = ref.reconstruct_volume(ref.allocate_volume())
             frac, mx = lsb_agreement(self.xp, vol["db"], vref["db"], vref["clim"])
             out["within_1lsb_of_plain"] = frac
             out["max_lsb_vs_plain"] = mx
@@ -441,13 +514,16 @@ def main(argv=None):
     ap.add_argument("--after-stream", action="store_true",
                     help="reconstruct once every tile has landed (focus per depth from all rows)")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--resident-rows", type=int, default=None, help="y-tile rows of raw data held in HBM")
     ap.add_argument("--tiff", default=None, help="also write the volume as the legacy BigTIFF (+ .json)")
     a = ap.parse_args(argv)
     xp = get_xp("hip")[0]
     ss = StreamSpec(tiles_x=a.tiles_x, tiles_y=a.tiles_y, depths=a.depths, n_y=a.bscans, n_x=a.ascans, seed=a.seed)
     sv = StreamedVolume(xp, ss, ring=a.ring, gen_threads=a.gen_threads, estimator=a.estimator, form=a.form,
-                        log=lambda *m: print(*m, flush=True), rowwise=not a.after_stream)
-    print(f"streamed volume: {ss.n_tiles} tiles, {sv.raw.numel() * 2 / 1e9:.1f} GB raw resident", flush=True)
+                        log=lambda *m: print(*m, flush=True), rowwise=not a.after_stream,
+                        resident_rows=a.resident_rows)
+    print(f"streamed volume: {ss.n_tiles} tiles, {sv.slots.numel() * 2 / 1e9:.1f} GB raw resident "
+          f"({sv.resident_rows} of {ss.tiles_y} rows)", flush=True)
     sv.warmup()
     r = sv.run(validate=not a.no_validate)
     vol = r.pop("volume")

@@ -8,7 +8,11 @@ Bars:
   * the MFMA volume has at least 99.9 % of voxels within 1 LSB of the plain FP32 volume (both
     quantised with the plain volume's clim);
   * with the learned estimator (search fallback), per-tile dispersion and per-depth focus land
-    within the estimator's tolerances of the generator's values (median over tiles).
+    within the estimator's tolerances of the generator's values (median over tiles);
+  * a volume streamed through a ring of fewer resident y-tile rows (the plain validation then
+    row by row) equals the all-resident volume bit for bit;
+  * the full configuration-4 volume (12 x 9 tiles x 8 depths of 40 x 1000 A-scans, 145 GB of
+    raw data) streamed through 2 resident rows meets the bars above.
 """
 import numpy as np
 import pytest
@@ -89,7 +93,7 @@ def test_streamed_volume_given_estimates(xp):
     for i in (0, len(sv.keys) - 1):
         host = np.empty((ss.n_y, sv.lines, sv.n), np.uint16)
         sv.gen.generate(host, ss.n_x, sv.truth["beta_idx"][i], sv.truth["focus"][i], int(sv.truth["seed"][i]))
-        dev = sv.raw[i].view(torch.int16).cpu().numpy().view(np.uint16)
+        dev = sv.tile_raw(i).view(torch.int16).cpu().numpy().view(np.uint16)
         assert np.array_equal(host, dev), i
     assert r["within_1lsb_of_plain"] >= 0.999
     sv3 = StreamedVolume(xp, ss, ring=3, gen_threads=8, estimator="given")
@@ -142,7 +146,7 @@ def test_streamed_volume_from_files(xp, tmp_path, kind):
     rng = np.random.default_rng(9)
     readers = {}
     for i, key in enumerate(gen.keys):
-        raw = gen.raw[i].view(xp.torch.int16).cpu().numpy()
+        raw = gen.tile_raw(i).view(xp.torch.int16).cpu().numpy()
         folder = tmp_path / f"t{i}"
         if kind == "ocity":
             rawfiles.write_ocity(folder, raw)
@@ -156,3 +160,51 @@ def test_streamed_volume_from_files(xp, tmp_path, kind):
     r = sv.run(validate=False)
     assert r["source"] == "files"
     assert np.array_equal(r["volume"]["db"].cpu().numpy(), ref, equal_nan=True)
+
+
+def test_bounded_residency_same_bits(xp):
+    """1 and 2 resident rows of a 4-row volume give the all-resident volume, bit for bit, with the
+    plain validation run row by row."""
+    from oct_recon_af_v1_amd.stream import StreamedVolume
+    _say("case: streamed volume 2x4x2 tiles, 1 / 2 / all rows resident")
+    ss = _small(tiles_y=4)
+    full = StreamedVolume(xp, ss, ring=3, gen_threads=8, estimator="given")
+    r = full.run(validate=True)
+    ref = r["volume"]["db"].cpu().numpy()
+    assert r["resident_rows"] == 4
+    del full
+    for rows in (1, 2):
+        sv = StreamedVolume(xp, ss, ring=3, gen_threads=8, estimator="given", resident_rows=rows)
+        got = sv.run(validate=True)
+        _say(f"{rows} resident rows: {got['raw_resident_GB']:.3f} GB, within 1 LSB of plain "
+             f"{got['within_1lsb_of_plain']:.6f}, {got['validation']}")
+        assert got["resident_rows"] == rows and sv.slots.shape[0] == rows
+        assert np.array_equal(got["volume"]["db"].cpu().numpy(), ref, equal_nan=True)
+        assert got["within_1lsb_of_plain"] == r["within_1lsb_of_plain"]
+    with pytest.raises(ValueError):
+        StreamedVolume(xp, ss, estimator="given", resident_rows=2, rowwise=False)
+
+
+def test_full_size_streamed_volume(xp):
+    """Configuration 4 at full size: 864 tiles, host-generated and streamed through pinned async
+    copies into a ring of 2 resident y-tile rows (about 32 GB instead of 145 GB), per-tile
+    dispersion and focus from the learned estimator (search fallback), MFMA reconstruction
+    row by row, the plain FP32 volume made beside it row by row as the referee."""
+    from oct_recon_af_v1_amd import learned as LN
+    from oct_recon_af_v1_amd.stream import StreamedVolume, StreamSpec
+    _say("case: full configuration-4 streamed volume (12 x 9 x 8 tiles, 40 x 1000 A-scans), 2 rows resident")
+    sv = StreamedVolume(xp, StreamSpec(), ring=8, estimator="learned", resident_rows=2)
+    sv.warmup()
+    r = sv.run(validate=True)
+    _say({k: v for k, v in r.items() if k not in ("volume", "focus_per_depth")})
+    _say(f"end to end {r['ascans_per_s'] / 1e6:.2f} M A-scans/s over {r['ascans'] / 1e6:.2f} M A-scans "
+         f"({r['raw_GB']:.0f} GB raw, {r['raw_resident_GB']:.1f} GB resident); fallback rate "
+         f"{r['fallback_rate_bscans']:.2%} of B-scans; within 1 LSB of plain {r['within_1lsb_of_plain']:.6%}")
+    assert r["ascans"] == 12 * 9 * 8 * 40 * 1000
+    assert r["within_1lsb_of_plain"] >= 0.999
+    assert r["beta_err_idx_median"] <= LN.BETA_TOL_IDX
+    assert r["focus_err_bins_median"] <= LN.FOCUS_TOL_BINS
+    fpd = np.asarray(r["focus_per_depth"]) - 1                  # 0-based, as the generator's focus
+    truth = [np.median(sv.truth["focus"][[i for i, k in enumerate(sv.keys) if k[2] == d]]) for d in range(8)]
+    _say(f"focus per depth: {np.round(fpd, 1).tolist()} vs generator {np.round(truth, 1).tolist()}")
+    assert np.max(np.abs(fpd - truth)) <= LN.FOCUS_TOL_BINS
