@@ -52,7 +52,7 @@ struct oct_plan {
     long long win_fstride;       // floats between per-file override windows (0: one window)
     long long win_rstride;       // floats between the override windows of consecutive batch repeats
     float2* tw;   // (32, 64) transform twiddles [k1][n2]
-    int waves;    // workgroup shape of the fused kernel: 8 (LDS twiddles) or 4
+    int waves;    // workgroup shape of the fused kernel (see oct_plan_create)
 };
 
 namespace {
@@ -78,8 +78,58 @@ int launch_spectral_w(const oct_plan* p, const oct::SpectralArgs& a, int kind, h
     return OCT_OK;
 }
 
+template <typename RawT, int TP, int W>
+int launch_spectral16(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
+    const long long ngroups = (a.M + W - 1) / W;
+    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus * (16 / W));
+    if (grid <= 0) return OCT_OK;
+    const dim3 threads(64 * W);
+    const int st2 = a.k2n == 0 ? 0 : (a.rows <= 32 ? 1 : 2);
+    switch (kind * 3 + st2) {
+        case OCT_OUT_MAG * 3 + 0:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_MAG, 0, W>), dim3(grid), threads, 0, s, a);
+            break;
+        case OCT_OUT_MAG * 3 + 1:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_MAG, 1, W>), dim3(grid), threads, 0, s, a);
+            break;
+        case OCT_OUT_MAG * 3 + 2:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_MAG, 2, W>), dim3(grid), threads, 0, s, a);
+            break;
+        case OCT_OUT_COMPLEX * 3 + 0:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_CPLX, 0, W>), dim3(grid), threads, 0, s, a);
+            break;
+        case OCT_OUT_COMPLEX * 3 + 1:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_CPLX, 1, W>), dim3(grid), threads, 0, s, a);
+            break;
+        case OCT_OUT_COMPLEX * 3 + 2:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_CPLX, 2, W>), dim3(grid), threads, 0, s, a);
+            break;
+        default:
+            hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_EQ, 0, W>), dim3(grid), threads, 0, s, a);
+            break;
+    }
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 template <typename RawT>
 int launch_spectral(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
+    if (p->waves == 16) {
+        if (p->taps == 12) return launch_spectral16<RawT, 12, 16>(p, a, kind, s);
+        return launch_spectral16<RawT, 16, 16>(p, a, kind, s);
+    }
+    if (p->waves == 28) {            // two 8-wave workgroups per CU
+        if (p->taps == 12) return launch_spectral16<RawT, 12, 8>(p, a, kind, s);
+        return launch_spectral16<RawT, 16, 8>(p, a, kind, s);
+    }
+    if (p->waves == 82) {            // eight 2-wave workgroups per CU
+        if (p->taps == 12) return launch_spectral16<RawT, 12, 2>(p, a, kind, s);
+        return launch_spectral16<RawT, 16, 2>(p, a, kind, s);
+    }
+    if (p->waves == 44) {            // four 4-wave workgroups per CU
+        if (p->taps == 12) return launch_spectral16<RawT, 12, 4>(p, a, kind, s);
+        return launch_spectral16<RawT, 16, 4>(p, a, kind, s);
+    }
     if (p->waves == 4) {
         if (p->taps == 12) return launch_spectral_w<RawT, 12, 4>(p, a, kind, s);
         return launch_spectral_w<RawT, 16, 4>(p, a, kind, s);
@@ -150,8 +200,12 @@ int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const i
     int cus = 0;
     hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
     p->cus = (e == hipSuccess && cus > 0) ? cus : 256;
+    // workgroup shape of the fused kernel (OCT_HIP_SPECTRAL_WAVES, for measurements): 44 = four
+    // 4-wave workgroups per CU (default, fastest), 28 = two of 8, 82 = eight of 2, 16 = one of 16
+    // (in-place regions); 8 and 4 = the earlier two-region forms
     const char* ev = std::getenv("OCT_HIP_SPECTRAL_WAVES");
-    p->waves = (ev && std::atoi(ev) == 4) ? 4 : 8;
+    const int wv = ev ? std::atoi(ev) : 44;
+    p->waves = (wv == 4 || wv == 8 || wv == 16 || wv == 28 || wv == 82) ? wv : 44;
     std::vector<float> twh(2 * 32 * 64);
     for (int k1 = 0; k1 < 32; ++k1)
         for (int n2 = 0; n2 < 64; ++n2) {

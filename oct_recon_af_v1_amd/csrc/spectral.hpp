@@ -318,6 +318,305 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
     }
 }
 
+// The same path with 16 A-scans per group and one 1024-thread workgroup per CU, laid out so that
+// 16 waves fit in LDS (4 waves per SIMD instead of 2): each A-scan owns one 8.25 KB region that
+// holds in turn
+//   d - m (phase A);
+//   the equispaced interferogram, written in place once every wave has read every region
+//   (phase B keeps its outputs in registers across a barrier);
+//   the transform's transpose, real parts then imaginary parts (32 x 66 floats each).
+// Phase B: wave w computes output samples i = lane + 64 (2 w + qq), qq < 2, of all 16 A-scans
+// (each tap table entry read once per 16 A-scans).  Phase D is that of spectral_kernel.  Three
+// workgroup barriers per group.  The launch bounds cap the kernel at 128 VGPRs.
+constexpr int kGroupWaves = 16;
+constexpr int kRegionF = 32 * kStride;   // 2112 floats >= N
+
+// STAGE2 (compiled apart: the union does not fit in 128 VGPRs):
+//   0  the full second stage (any rows);
+//   1  at most 32 kept rows within two 32-bin blocks (a.k2n > 0, a.rows <= 32): each k1 has at most
+//      one kept bin, so each lane pair forms the direct 32-term sum of that bin's block only;
+//   2  more than 32 kept rows within two blocks: the direct sums of both blocks.
+// W = 16: one workgroup per CU, twiddles in LDS; W = 8 / 4: two / four workgroups per CU (their
+// loads and barriers overlap the others' arithmetic), twiddles read from the L1-resident
+// [k1][lane] table.
+template <typename RawT, int TP, int KIND, int STAGE2, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void spectral16_kernel(SpectralArgs a) {
+    constexpr bool PRUNED = STAGE2 > 0;
+    constexpr int kWaves = W;
+    constexpr int kThreads = 64 * kWaves;
+    constexpr int kQW = kQ / kWaves;         // output columns per lane per A-scan
+    constexpr bool kLdsTw = (W == 16);
+    __shared__ __align__(16) float buf[kWaves][kRegionF];
+    __shared__ __align__(16) cf tws[kLdsTw ? kQ : 1][kLdsTw ? 64 : 1];
+    __shared__ float lmean[kWaves];
+    __shared__ cf tw2[2][kQ];
+    __shared__ cf tw64[2];
+
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+
+    if constexpr (KIND != KIND_EQ) {
+        if constexpr (kLdsTw) {
+            for (int e = threadIdx.x; e < kQ * 64; e += kThreads) {
+                const int k1 = e >> 6, n2 = e & 63;
+                double sn, cs;
+                sincospi((double)(n2 * k1) / (kN / 2), &sn, &cs);
+                tws[k1][n2] = cf{(float)cs, (float)sn};
+            }
+        }
+        if (PRUNED && threadIdx.x < 2 * kQ + 2) {
+            const int t = threadIdx.x < 2 * kQ ? threadIdx.x / kQ : threadIdx.x - 2 * kQ;
+            const int k2 = a.k2lo + t;
+            double sn, cs;
+            if (threadIdx.x < 2 * kQ) {
+                const int m = threadIdx.x % kQ;
+                sincospi(2.0 * (double)((m * k2) % kQ) / kQ, &sn, &cs);
+                tw2[t][m] = cf{(float)cs, (float)sn};
+            } else {
+                sincospi(2.0 * (double)k2 / 64.0, &sn, &cs);
+                tw64[t] = cf{(float)cs, (float)sn};
+            }
+        }
+        // made visible to all waves by the first barrier of the group loop
+    }
+
+    constexpr int VEC = 16 / sizeof(RawT);
+    constexpr int NLD = kN / (64 * VEC);
+    const long long ngroups = (a.M + kWaves - 1) / kWaves;
+
+    for (long long g = blockIdx.x; g < ngroups; g += gridDim.x) {
+        // lane-derived addresses and row tests are rebuilt in every group: the empty asm makes
+        // the lane index opaque, so the compiler cannot hoist dozens of them out of the loop
+        // (under the 128-VGPR cap they would be spilled)
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));
+        int row0 = a.row0, rows = a.rows;
+        asm volatile("" : "+s"(row0), "+s"(rows));
+        const long long p0 = g * kWaves;
+        const int nvalid = (int)min((long long)kWaves, a.M - p0);
+        const long long p = p0 + wave;
+        float* R = &buf[wave][0];
+        const unsigned fv = (unsigned)p / (unsigned)a.nX;          // virtual file
+        const unsigned rep = fv / (unsigned)a.raw_files;
+        const unsigned b = fv - rep * (unsigned)a.raw_files;
+        const unsigned x = (unsigned)p - fv * (unsigned)a.nX;
+
+        // ---- A: load, background, mean, d - m -> this A-scan's region
+        if (wave < nvalid) {
+            const RawT* r = reinterpret_cast<const RawT*>(a.raw) +
+                            (((long long)b * a.lines + a.line0 + x) * (long long)kN);
+            const float* bgr = a.bg + (long long)b * kN;
+            float d[NLD][VEC];
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int j0 = VEC * lane + 64 * VEC * k;
+                uint4 u = *reinterpret_cast<const uint4*>(r + j0);
+                float f[VEC];
+                unpack16<RawT>(u, f);
+#pragma unroll
+                for (int e4 = 0; e4 < VEC / 4; ++e4) {
+                    float4 bv = *reinterpret_cast<const float4*>(bgr + j0 + 4 * e4);
+                    d[k][4 * e4] = f[4 * e4] - bv.x;
+                    d[k][4 * e4 + 1] = f[4 * e4 + 1] - bv.y;
+                    d[k][4 * e4 + 2] = f[4 * e4 + 2] - bv.z;
+                    d[k][4 * e4 + 3] = f[4 * e4 + 3] - bv.w;
+                }
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) s += d[k][e];
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            const float m = s * (1.0f / kN);
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int j0 = VEC * lane + 64 * VEC * k;
+#pragma unroll
+                for (int e4 = 0; e4 < VEC / 4; ++e4) {
+                    float4 v{d[k][4 * e4] - m, d[k][4 * e4 + 1] - m, d[k][4 * e4 + 2] - m, d[k][4 * e4 + 3] - m};
+                    *reinterpret_cast<float4*>(R + j0 + 4 * e4) = v;
+                }
+            }
+            if (lane == 0) lmean[wave] = m;
+        }
+        __syncthreads();
+
+        // ---- B: banded sinc of this wave's output columns for all A-scans of the group, kept
+        //      in registers until every wave has read every region; the columns' taps are read
+        //      once per group (L1 / L2)
+        // (regions of A-scans past the end of the batch hold stale values: computed, never stored)
+        // columns in passes of at most 4, whose taps are held in registers
+        constexpr int kCP = kQW < 4 ? kQW : 4;
+        float res[kWaves][kQW];
+#pragma unroll
+        for (int cp = 0; cp < kQW / kCP; ++cp) {
+            int bs[kCP];
+            float wt[kCP][TP];
+#pragma unroll
+            for (int qq = 0; qq < kCP; ++qq) {
+                const int i = lane + 64 * (wave * kQW + cp * kCP + qq);
+                bs[qq] = a.band_start[i];
+                const float4* wp = reinterpret_cast<const float4*>(a.band_w + (long long)i * TP);
+#pragma unroll
+                for (int t4 = 0; t4 < TP / 4; ++t4) {
+                    const float4 v = wp[t4];
+                    wt[qq][4 * t4] = v.x;
+                    wt[qq][4 * t4 + 1] = v.y;
+                    wt[qq][4 * t4 + 2] = v.z;
+                    wt[qq][4 * t4 + 3] = v.w;
+                }
+            }
+#pragma unroll
+            for (int aa = 0; aa < kWaves; ++aa) {
+                const float* Da = &buf[aa][0];
+                const float ma = lmean[aa];
+#pragma unroll
+                for (int qq = 0; qq < kCP; ++qq) {
+                    const float* src = Da + bs[qq];
+                    float acc = 0.f;
+#pragma unroll
+                    for (int t = 0; t < TP; ++t) acc = __builtin_fmaf(wt[qq][t], src[t], acc);
+                    res[aa][cp * kCP + qq] = acc + ma;
+                    // materialise the sum here: otherwise the compiler sinks the taps' arithmetic
+                    // past the barrier into the conditional stores and keeps every loaded tap live
+                    asm volatile("" : "+v"(res[aa][cp * kCP + qq]));
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int aa = 0; aa < kWaves; ++aa) {
+            if (aa < nvalid) {
+#pragma unroll
+                for (int qq = 0; qq < kQW; ++qq) {
+                    const int i = lane + 64 * (wave * kQW + qq);
+                    if constexpr (KIND == KIND_EQ) {
+                        reinterpret_cast<float*>(a.out)[(p0 + aa) * a.out_stride + i] = res[aa][qq];
+                    } else {
+                        buf[aa][i] = res[aa][qq];
+                    }
+                }
+            }
+        }
+        if constexpr (KIND == KIND_EQ) continue;      // the next group's first barrier orders the reuse
+        __syncthreads();
+
+        // ---- D: 2048-point inverse transform of this wave's A-scan
+        if (wave < nvalid) {
+            const float2* wg = reinterpret_cast<const float2*>(a.win + (long long)rep * a.win_rstride +
+                                                               (long long)b * a.win_fstride);
+            cf v[kQ];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                const float e = R[lane + 64 * q];
+                const float2 w = wg[lane + 64 * q];
+                v[q] = cf{e * w.x, e * w.y};
+            }
+            wave_lds_sync();
+            dft_inv_dif<kQ>(v);                          // Y[k1] in v[bitrev(k1)]
+            static_for<1, kQ>([&](auto k1) {
+                constexpr int r = bitrev(decltype(k1)::value, 5);
+                if constexpr (kLdsTw) {
+                    v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
+                } else {
+                    const float2 t = a.tw[decltype(k1)::value * 64 + lane];
+                    v[r] = cmul(v[r], cf{t.x, t.y});
+                }
+            });
+            // transpose through the region: real parts, then imaginary parts
+            const int kk1 = lane >> 1;
+            const int h = lane & 1;
+            // (scheduling barriers keep each pass's stores and loads together: mixing them with
+            // the transform's arithmetic would hold both halves and the arithmetic's values live)
+            float vr[kQ];
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].x; });
+            wave_lds_sync();
+#pragma unroll
+            for (int m = 0; m < kQ; ++m) vr[m] = R[kk1 * kStride + 2 * m + h];
+            wave_lds_sync();
+            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].y; });
+            wave_lds_sync();
+#pragma unroll
+            for (int m = 0; m < kQ; ++m) v[m] = cf{vr[m], R[kk1 * kStride + 2 * m + h]};
+            __builtin_amdgcn_sched_barrier(0);
+            // this A-scan's output row, indexed by the depth bin k (k - row0 in [0, rows) kept)
+            const long long obase = p * a.out_stride - row0;
+            if constexpr (STAGE2 == 1) {
+                // the block of this lane pair's bin: k2lo unless k1 + 32 k2lo lies before row0
+                const int t = (a.k2n == 2 && kk1 + 32 * a.k2lo < row0) ? 1 : 0;
+                float ex = 0.f, ey = 0.f;
+#pragma unroll
+                for (int m = 0; m < kQ; ++m) {
+                    const cf w = tw2[t][m];
+                    ex = __builtin_fmaf(v[m].x, w.x, __builtin_fmaf(-v[m].y, w.y, ex));
+                    ey = __builtin_fmaf(v[m].x, w.y, __builtin_fmaf(v[m].y, w.x, ey));
+                }
+                cf e{ex, ey};
+                if (h) e = cmul(e, tw64[t]);
+                const cf o{e.x + swap_pair(e.x), e.y + swap_pair(e.y)};
+                const int k = kk1 + 32 * (a.k2lo + t);
+                if (h == 0 && (unsigned)(k - row0) < (unsigned)rows) {
+                    if constexpr (KIND == KIND_MAG) {
+                        reinterpret_cast<float*>(a.out)[obase + k] = __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
+                    } else {
+                        reinterpret_cast<cf*>(a.out)[obase + k] = o;
+                    }
+                }
+            } else if constexpr (STAGE2 == 2) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    if (t < a.k2n) {
+                        float ex = 0.f, ey = 0.f;
+#pragma unroll
+                        for (int m = 0; m < kQ; ++m) {
+                            const cf w = tw2[t][m];
+                            ex = __builtin_fmaf(v[m].x, w.x, __builtin_fmaf(-v[m].y, w.y, ex));
+                            ey = __builtin_fmaf(v[m].x, w.y, __builtin_fmaf(v[m].y, w.x, ey));
+                        }
+                        cf e{ex, ey};
+                        if (h) e = cmul(e, tw64[t]);
+                        const cf o{e.x + swap_pair(e.x), e.y + swap_pair(e.y)};
+                        const int k = kk1 + 32 * (a.k2lo + t);
+                        if (h == t && (unsigned)(k - row0) < (unsigned)rows) {
+                            if constexpr (KIND == KIND_MAG) {
+                                reinterpret_cast<float*>(a.out)[obase + k] = __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
+                            } else {
+                                reinterpret_cast<cf*>(a.out)[obase + k] = o;
+                            }
+                        }
+                    }
+                }
+            } else {
+                dft_inv_dif<kQ>(v);                      // E_h[k2] in v[bitrev(k2)]
+                if (h) {
+                    static_for<1, kQ>([&](auto k2) {
+                        constexpr int r = bitrev(decltype(k2)::value, 5);
+                        v[r] = twiddle<decltype(k2)::value, 64>(v[r]);
+                    });
+                }
+                static_for<0, 16>([&](auto jj) {
+                    constexpr int j = decltype(jj)::value;
+                    constexpr int rlo = bitrev(j, 5);
+                    constexpr int rhi = bitrev(16 + j, 5);
+                    const cf send = h ? v[rlo] : v[rhi];
+                    const cf keep = h ? v[rhi] : v[rlo];
+                    const cf o{keep.x + swap_pair(send.x), keep.y + swap_pair(send.y)};
+                    const int k = kk1 + 32 * (16 * h + j);
+                    if ((unsigned)(k - row0) < (unsigned)rows) {
+                        if constexpr (KIND == KIND_MAG) {
+                            reinterpret_cast<float*>(a.out)[obase + k] = __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
+                        } else {
+                            reinterpret_cast<cf*>(a.out)[obase + k] = o;
+                        }
+                    }
+                });
+            }
+        }
+        // no closing barrier: phase D touches only this wave's region, which only this wave
+        // rewrites (next phase A); other waves read it only after the next group's first barrier
+    }
+}
+
 // A-scan binning of the scan lines (Thorlabs AScanBinning; octrecon/core/spectral.py:20-46, the
 // legacy filter2(ones(1,B)/B) + decimation): kept line c averages scan lines
 // s0 + c B + t - left, t < B (zero outside), s0 = max(1, B / 2) - 1, left = (B + 1) / 2 - 1, as
