@@ -416,14 +416,16 @@ def main():
     # with the next launch), so the kernel's time is the union of the launch intervals
     busy_ms = busy_p[0] / args.steps
     eff_ms = busy_ms / (launches / args.steps)
-    lo, hi = rec.stitchers[(0, 0)].needed_rows() if args.spectral_rows == "needed" else (0, 1024)
-    rows = hi - lo
+    # every tile computes its stitcher's row window (wider where the optical-path shift spreads it)
+    wins = [rec._tile_rows(st) for st in rec.stitchers.values() if not st.empty]
+    lo, hi = min(w[0] for w in wins), max(w[1] for w in wins)
+    rows = float(np.mean([w[1] - w[0] for w in wins]))
     bytes_per_ascan = 2 * 2048 + 4 * rows
     achieved = per_launch * bytes_per_ascan / (eff_ms * 1e-3) / 1e9
     value = ascans * args.steps / el
     clim = vol["clim"].cpu().numpy().tolist()
     log(f"plain: step {el / args.steps * 1e3:.2f} ms; spectral launch avg {avg_ms:.3f} ms over {launches} launches "
-        f"({per_launch:.0f} A-scans, {rows} rows), busy {busy_ms:.2f} ms per step = {eff_ms:.3f} ms per launch; "
+        f"({per_launch:.0f} A-scans, {rows:.2f} rows on average), busy {busy_ms:.2f} ms per step = {eff_ms:.3f} ms per launch; "
         f"clim {clim}")
     tpa = profiled_value("spectral_kernel_tiled_fp32") if args.spectral_rows == "needed" else None
     ipa = profiled_value("spectral_kernel_tiled_fp32", "valu_instructions_per_ascan") \
@@ -442,13 +444,14 @@ def main():
                                f"{args.bscans} B-scans x {args.ascans} A-scans x 2048 px uint16 per tile "
                                "(+25 background lines), shared beta, sigma 30 um, n 1.33, 1 um output, "
                                "depth 0-0.15 mm; spectral + stitch + finalise + quantise",
-                   "ascans_per_step": ascans, "output_yzx": list(rec.n_out), "spectral_rows": [lo, hi]},
+                   "ascans_per_step": ascans, "output_yzx": list(rec.n_out), "spectral_rows": [lo, hi],
+                   "spectral_rows_per_tile": rows},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": tpa * per_launch if tpa else None,
                      "traffic_source": "profiles/spectral_kernel_tiled_fp32.md (rocprofv3 FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if tpa else None,
-                     "kernel": "spectral_kernel<uint16_t,12,MAG,8>",
+                     "kernel": "spectral16_kernel<uint16_t,12,MAG,1|2,4> (one or two second-stage blocks)",
                      "bound_note": "VALU-issue bound (non-packed FP32): the HBM fraction is reported as asked",
                      "bytes_per_ascan": bytes_per_ascan, "ascans_per_launch": per_launch,
                      "avg_launch_ms": avg_ms, "busy_ms_per_step": busy_ms, "busy_ms_per_launch": eff_ms,

@@ -1,6 +1,8 @@
-// Fused spectral kernel: raw spectra -> background subtraction -> mean removal -> banded sinc
+// Fused spectral kernels: raw spectra -> background subtraction -> mean removal -> banded sinc
 // k-linearisation -> complex window (Hann/rms x dispersion, 1/N folded in) -> 2048-point
 // inverse FFT -> depth crop -> |.| (or complex, or the linearised interferogram).
+// spectral_kernel (8 A-scans per 512-thread workgroup, 2 waves per SIMD) and spectral16_kernel
+// (W A-scans per W-wave workgroup, 4 waves per SIMD, the default) compute the same values.
 //
 // Reference: octrecon/core/spectral.py:136-153 (SpectralProcessor.magnitude),
 // octrecon/core/gpu_kernels.py:10-38 (per-sample fused pre-FFT), octrecon/core/cpu_kernels.py:12-41.
@@ -318,17 +320,15 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
     }
 }
 
-// The same path with 16 A-scans per group and one 1024-thread workgroup per CU, laid out so that
-// 16 waves fit in LDS (4 waves per SIMD instead of 2): each A-scan owns one 8.25 KB region that
-// holds in turn
+// The same path at 4 waves per SIMD (spectral_kernel runs 2): W A-scans per group and one W-wave
+// workgroup, 16 / W workgroups per CU.  Each A-scan owns one 8.25 KB region that holds in turn
 //   d - m (phase A);
 //   the equispaced interferogram, written in place once every wave has read every region
 //   (phase B keeps its outputs in registers across a barrier);
 //   the transform's transpose, real parts then imaginary parts (32 x 66 floats each).
-// Phase B: wave w computes output samples i = lane + 64 (2 w + qq), qq < 2, of all 16 A-scans
-// (each tap table entry read once per 16 A-scans).  Phase D is that of spectral_kernel.  Three
+// Phase B: wave w computes output samples i = lane + 64 (w 32/W + qq), qq < 32/W, of all W A-scans
+// (each tap table entry read once per W A-scans).  Phase D is that of spectral_kernel.  Three
 // workgroup barriers per group.  The launch bounds cap the kernel at 128 VGPRs.
-constexpr int kGroupWaves = 16;
 constexpr int kRegionF = 32 * kStride;   // 2112 floats >= N
 
 // STAGE2 (compiled apart: the union does not fit in 128 VGPRs):
