@@ -130,4 +130,29 @@ __device__ __forceinline__ float swap_pair(float v) {
     return __builtin_bit_cast(float, i);
 }
 
+// Cross-lane moves without LDS addresses: DPP (row_ror / quad_perm controls) and ds_swizzle
+// (bit-mask mode, xor within 32-lane halves).
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int XOR>
+__device__ __forceinline__ float swizzle_xor_f(float v) {
+    static_assert(XOR > 0 && XOR < 32, "ds_swizzle bit-mask mode works within 32 lanes");
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x1F | (XOR << 10)));
+}
+
+// Sum over the wave by the xor butterfly s += s[lane ^ m], m = 32, 16, ..., 1: every lane ends
+// with the same bits as the __shfl_xor loop (same pairs in the same order; addition commutes),
+// without its index arithmetic.  x32 = (lane ^ 32) * 4, the ds_bpermute address of the first step.
+__device__ __forceinline__ float wave_sum_xor(float s, int x32) {
+    s += __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(x32, __builtin_bit_cast(int, s)));
+    s += swizzle_xor_f<16>(s);
+    s += dpp_f<0x128>(s);          // row_ror:8 = lane ^ 8 within a row of 16
+    s += swizzle_xor_f<4>(s);
+    s += dpp_f<0x4E>(s);           // quad_perm [2,3,0,1] = lane ^ 2
+    s += dpp_f<0xB1>(s);           // quad_perm [1,0,3,2] = lane ^ 1
+    return s;
+}
+
 }  // namespace oct

@@ -84,7 +84,7 @@ int launch_spectral16(const oct_plan* p, const oct::SpectralArgs& a, int kind, h
     const int grid = (int)std::min<long long>(ngroups, (long long)p->cus * (16 / W));
     if (grid <= 0) return OCT_OK;
     const dim3 threads(64 * W);
-    const int st2 = a.k2n == 0 ? 0 : (a.rows <= 32 ? 1 : 2);
+    const int st2 = a.k2n == 0 ? 0 : (a.rows <= 32 + oct::kStage2Extra ? 1 : 2);
     switch (kind * 3 + st2) {
         case OCT_OUT_MAG * 3 + 0:
             hipLaunchKernelGGL((oct::spectral16_kernel<RawT, TP, oct::KIND_MAG, 0, W>), dim3(grid), threads, 0, s, a);

@@ -167,8 +167,7 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) s += d[k][e];
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            s = wave_sum_xor(s, (lane ^ 32) << 2);
             const float m = s * (1.0f / kN);
 #pragma unroll
             for (int k = 0; k < NLD; ++k) {
@@ -333,9 +332,12 @@ constexpr int kRegionF = 32 * kStride;   // 2112 floats >= N
 
 // STAGE2 (compiled apart: the union does not fit in 128 VGPRs):
 //   0  the full second stage (any rows);
-//   1  at most 32 kept rows within two 32-bin blocks (a.k2n > 0, a.rows <= 32): each k1 has at most
-//      one kept bin, so each lane pair forms the direct 32-term sum of that bin's block only;
-//   2  more than 32 kept rows within two blocks: the direct sums of both blocks.
+//   1  at most 32 + kStage2Extra kept rows within two 32-bin blocks (a.k2n > 0): the first 32
+//      rows hold at most one bin per k1, so each lane pair forms the direct 32-term sum of that
+//      bin's block only; each further row (bin k = row0 + 32 + e) is one 64-term sum over the
+//      lanes, X[k] = sum_n2 Z[n2][k mod 32] W_64^(n2 (k / 32)), reduced across the wave;
+//   2  more rows within two blocks: the direct sums of both blocks.
+constexpr int kStage2Extra = 2;
 // W = 16: one workgroup per CU, twiddles in LDS; W = 8 / 4: two / four workgroups per CU (their
 // loads and barriers overlap the others' arithmetic), twiddles read from the L1-resident
 // [k1][lane] table.
@@ -351,6 +353,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
     __shared__ float lmean[kWaves];
     __shared__ cf tw2[2][kQ];
     __shared__ cf tw64[2];
+    __shared__ cf w64t[STAGE2 == 1 ? 64 : 1];   // W_64^j, the extra rows' second-stage twiddles
 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
@@ -376,11 +379,19 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                 tw64[t] = cf{(float)cs, (float)sn};
             }
         }
+        if constexpr (STAGE2 == 1) {
+            if (threadIdx.x < 64) {
+                double sn, cs;
+                sincospi((double)threadIdx.x / 32.0, &sn, &cs);
+                w64t[threadIdx.x] = cf{(float)cs, (float)sn};
+            }
+        }
         // made visible to all waves by the first barrier of the group loop
     }
 
     constexpr int VEC = 16 / sizeof(RawT);
     constexpr int NLD = kN / (64 * VEC);
+    constexpr int kEx = STAGE2 == 1 ? kStage2Extra : 1;     // extra rows read (STAGE2 == 1)
     const long long ngroups = (a.M + kWaves - 1) / kWaves;
 
     for (long long g = blockIdx.x; g < ngroups; g += gridDim.x) {
@@ -424,8 +435,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) s += d[k][e];
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            s = wave_sum_xor(s, (lane ^ 32) << 2);
             const float m = s * (1.0f / kN);
 #pragma unroll
             for (int k = 0; k < NLD; ++k) {
@@ -465,13 +475,20 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                     wt[qq][4 * t4 + 3] = v.w;
                 }
             }
+            unsigned bsb[kCP];                   // byte offsets of the band starts in a region
+#pragma unroll
+            for (int qq = 0; qq < kCP; ++qq) bsb[qq] = 4u * (unsigned)bs[qq];
 #pragma unroll
             for (int aa = 0; aa < kWaves; ++aa) {
-                const float* Da = &buf[aa][0];
                 const float ma = lmean[aa];
 #pragma unroll
                 for (int qq = 0; qq < kCP; ++qq) {
-                    const float* src = Da + bs[qq];
+                    // one opaque byte address per (A-scan, column): the taps then use the LDS
+                    // instructions' immediate offsets (left visible, the compiler forms one
+                    // address per pair of taps, since a region's offset exceeds that field)
+                    unsigned ba = bsb[qq] + (unsigned)(aa * kRegionF * 4);
+                    asm volatile("" : "+v"(ba));
+                    const float* src = reinterpret_cast<const float*>(reinterpret_cast<const char*>(&buf[0][0]) + ba);
                     float acc = 0.f;
 #pragma unroll
                     for (int t = 0; t < TP; ++t) acc = __builtin_fmaf(wt[qq][t], src[t], acc);
@@ -528,16 +545,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             // (scheduling barriers keep each pass's stores and loads together: mixing them with
             // the transform's arithmetic would hold both halves and the arithmetic's values live)
             float vr[kQ];
+            float exr[kEx], exi[kEx];                    // Z[lane][k1] of the extra rows (STAGE2 == 1)
             __builtin_amdgcn_sched_barrier(0);
             static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].x; });
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < kQ; ++m) vr[m] = R[kk1 * kStride + 2 * m + h];
+            if constexpr (STAGE2 == 1) {
+#pragma unroll
+                for (int e = 0; e < kEx; ++e) exr[e] = R[((row0 + e) & 31) * kStride + lane];
+            }
             wave_lds_sync();
             static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].y; });
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < kQ; ++m) v[m] = cf{vr[m], R[kk1 * kStride + 2 * m + h]};
+            if constexpr (STAGE2 == 1) {
+#pragma unroll
+                for (int e = 0; e < kEx; ++e) exi[e] = R[((row0 + e) & 31) * kStride + lane];
+            }
             __builtin_amdgcn_sched_barrier(0);
             // this A-scan's output row, indexed by the depth bin k (k - row0 in [0, rows) kept)
             const long long obase = p * a.out_stride - row0;
@@ -560,6 +586,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                         reinterpret_cast<float*>(a.out)[obase + k] = __builtin_amdgcn_sqrtf(o.x * o.x + o.y * o.y);
                     } else {
                         reinterpret_cast<cf*>(a.out)[obase + k] = o;
+                    }
+                }
+                // rows row0 + 32 + e: bin k1 = (row0 + e) mod 32, k2 = (row0 + 32 + e) / 32
+#pragma unroll
+                for (int e = 0; e < kEx; ++e) {
+                    if (e < rows - 32) {                 // uniform
+                        const int ke = row0 + 32 + e;
+                        const cf w = w64t[(lane * (ke >> 5)) & 63];
+                        float xr = __builtin_fmaf(exr[e], w.x, -exi[e] * w.y);
+                        float xi = __builtin_fmaf(exr[e], w.y, exi[e] * w.x);
+                        xr = wave_sum_xor(xr, (lane ^ 32) << 2);
+                        xi = wave_sum_xor(xi, (lane ^ 32) << 2);
+                        if (lane == 0) {
+                            if constexpr (KIND == KIND_MAG) {
+                                reinterpret_cast<float*>(a.out)[obase + ke] = __builtin_amdgcn_sqrtf(xr * xr + xi * xi);
+                            } else {
+                                reinterpret_cast<cf*>(a.out)[obase + ke] = cf{xr, xi};
+                            }
+                        }
                     }
                 }
             } else if constexpr (STAGE2 == 2) {

@@ -4,6 +4,8 @@ Tolerances (float32 arithmetic on both sides):
   * magnitudes: quantised with one clim (the oracle's finite dB range, 65534 codes), at least
     99.99 % of voxels within 1 LSB of the float32 reference CPU path and of the float64 oracle;
   * linearised interferogram / complex A-scans: within 2e-6 / 1e-5 of the largest value;
+  * row windows (pruned second stage): within 1e-6 of the largest magnitude of the full
+    transform's rows and 2e-6 of the float64 oracle's;
   * stitching contributions: within 1e-5 of the largest value;
   * legacy MATLAB plane: < 1e-2 dB and no worse than 1.5x the float32 CPU path + 1e-4 dB
     (the reference's own backend criterion, tests/test_torch_backend.py:66-73);
@@ -147,6 +149,29 @@ def test_rows_linearise_complex(xp):
     ocs = R.complex_fp64(raw, 25, geo)
     assert np.abs(cs - ocs).max() <= 1e-5 * np.abs(ocs).max()
     assert np.array_equal(np.abs(cs).astype(np.float32) > 0, full > 0)
+
+
+@pytest.mark.parametrize("row0,rows", [(512, 32), (511, 32), (500, 20), (511, 33), (512, 33), (500, 34),
+                                       (495, 40), (480, 64), (0, 33), (991, 33)])
+def test_row_windows_of_the_pruned_stage(xp, row0, rows):
+    """Row windows inside two 32-bin blocks take the pruned second stage: lane-pair sums for the
+    first 32 rows, wave-wide sums for up to two more, both blocks' sums beyond that.  Each window
+    agrees with the full transform's rows and with the float64 oracle."""
+    from oct_recon_af_v1_amd import synthetic as SY
+    _say(f"case: row window [{row0}, {row0 + rows})")
+    spec = SY.SpectrumSpec()
+    lam = spec.lambda_nm()
+    raw = SY.make_bscans(2, 37, spec, beta=3e-5, seed=11).numpy()    # 74 A-scans: ragged last group
+    sg, sp = _proc(xp, lam, 4e7)
+    geo = R.spectral_geometry(lam, 4e7, 1.33)
+    t = xp.asarray(raw)
+    full = sp.magnitude(t).cpu().numpy()
+    part = sp.magnitude_rows(t, row0, rows).cpu().numpy()
+    assert part.shape == full[..., row0:row0 + rows].shape
+    scale = np.abs(full).max()
+    assert np.abs(part - full[..., row0:row0 + rows]).max() <= 1e-6 * scale
+    o = R.magnitude_fp64(raw, 25, geo)[..., row0:row0 + rows]
+    assert np.abs(part - o).max() <= 2e-6 * scale
 
 
 def test_edge_inputs(xp):
