@@ -293,10 +293,19 @@ __global__ __launch_bounds__(256, 2) void opgemm_kernel(OpGemmArgs a) {
 // opgemm_kernel with the data fragments kept in registers.  Each lane loads and converts the
 // samples of its own B fragments directly (raw - bg -> fp16 hi / lo): for the wave's two 32-column
 // tiles c, lanes 0-31 take samples k..k+7 and lanes 32-63 k+8..k+15 of A-scans 32 c + (lane & 31).
-// These are exactly opgemm_kernel's fragments, and the products run in the same order, so the
-// results are the same bits.  Only the operator rows go through LDS, double-buffered with one
-// workgroup barrier per K step; the next step's samples and operator rows are loaded into
-// registers during the current step's products.  RT row tiles of 32 per workgroup (1-4).
+// These are exactly opgemm_kernel's fragments, and every accumulator receives the same products
+// in the same order, so the results are the same bits.  Only the operator rows go through LDS,
+// double-buffered with one workgroup barrier per K step of 32.  RT row tiles of 32 per workgroup.
+//
+// Software pipeline, per K step `it` in two halves of 16 (ks = 0, 1):
+//   half 0: the products of (it, ks 0), and beside them the conversion of (it, ks 1) and the
+//           load of step it + NS's second 16 samples;
+//   half 1: the products of (it, ks 1), and beside them the conversion of (it + 1, ks 0) and the
+//           load of step it + 1 + NS's first 16 samples.
+// Each half is one basic block (clamped addresses instead of branches: lanes past the last
+// A-scan load a valid row and their columns are never stored), so the conversion's VALU work
+// issues in the shadow of the matrix instructions.  Raw samples are loaded NS steps ahead into a
+// ring of NS register slots (NS = 2 for 16-bit samples, 1 for float32).
 // kLdsBg (files of at least 256 A-scans, so a workgroup's A-scans lie in at most two files): the
 // two background rows are read once into LDS instead of by every lane at every step.
 // LDS 2 x RT x 5 KB (+ 16 KB).
@@ -327,20 +336,20 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     const int nk = a.N / kOpK;
 
     // This lane's data fragments: A-scans pc[c] = col0 + 64 wave + 32 c + (lane & 31), c = 0, 1,
-    // samples k0 + ks + ko .. + 7 (ko = 8 (lane >> 5)): the B-fragment layout of the matrix cores,
-    // loaded directly (a wave's load touches 32 A-scan rows, two lanes per 32-byte segment).
+    // samples k0 + 16 ks + ko .. + 7 (ko = 8 (lane >> 5)): the B-fragment layout of the matrix
+    // cores, loaded directly (a wave's load touches 32 A-scan rows, two lanes per 32-byte segment).
+    // A lane past the last A-scan reads the last A-scan's row (in the block's files).
     const int ko = 8 * (lane >> 5);
     const long long fb0 = col0 / a.nX;                           // first file of the block
-    bool pvalid[2];
     const RawT* rrow[2];
     const float* brow[2];
     int fsel[2];                                                 // kLdsBg: file - fb0 (0 or 1)
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
-        pvalid[c] = pc < a.M;
-        const long long b = pvalid[c] ? pc / a.nX : fb0;
-        const long long x = pvalid[c] ? pc - b * a.nX : 0;
+        long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        pc = pc < a.M ? pc : a.M - 1;
+        const long long b = pc / a.nX;
+        const long long x = pc - b * a.nX;
         rrow[c] = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N + ko;
         brow[c] = a.bg + b * a.N + ko;
         fsel[c] = (int)(b - fb0);
@@ -362,21 +371,28 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
     const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
     constexpr int kRawVec = 16 / sizeof(RawT);
     constexpr int kFragLd = 8 / kRawVec;                         // 16-byte loads per fragment
-    uint4 rv[2][kOpK / 16][kFragLd];                             // [c][ks / 16]
+    constexpr int NS = sizeof(RawT) == 2 ? 2 : 1;                // raw register slots (steps ahead)
+    uint4 rv[NS][2][2][kFragLd];                                 // [slot][ks][c]
     half8_t oh[2], ol[2];
-    half8_t dh[kOpK / 16][2], dl[kOpK / 16][2];                  // fragments [ks / 16][c]
-    auto fetch = [&](int k0) {
+    half8_t dh[2][2], dl[2][2];                                  // fragments [ks][c]
+    // -1 kept opaque: fma(hi, -1, d) is then formed by the mixed-precision FMA (f16 operand, f32
+    // addend) instead of a conversion back to f32 and a subtraction; d - hi is exact, so the
+    // rounded low part has the same bits either way
+    float mone = -1.0f;
+    asm volatile("" : "+v"(mone));
+
+    auto fetch_raw = [&](auto slotI, auto jI, int step) {
+        constexpr int S = decltype(slotI)::value, J = decltype(jI)::value;
+        const int k0 = (step < nk ? step : nk - 1) * kOpK + 16 * J;      // past the end: reload the last step
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            if (pvalid[c]) {
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-                for (int j = 0; j < kOpK / 16; ++j)
-#pragma unroll
-                    for (int q = 0; q < kFragLd; ++q)
-                        rv[c][j][q] = *reinterpret_cast<const uint4*>(rrow[c] + k0 + 16 * j + kRawVec * q);
-            }
-        }
+            for (int q = 0; q < kFragLd; ++q)
+                rv[S][J][c][q] = *reinterpret_cast<const uint4*>(rrow[c] + k0 + kRawVec * q);
+    };
+    auto fetch_op = [&](int step) {
         if (ostage) {
+            const int k0 = (step < nk ? step : nk - 1) * kOpK;
             const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
             const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
             oh[0] = sh[0];
@@ -385,46 +401,43 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
             ol[1] = sl[1];
         }
     };
-    auto convert = [&](int k0) {
+    auto convert = [&](auto slotI, auto jI, int step) {
+        constexpr int S = decltype(slotI)::value, J = decltype(jI)::value;
+        const int k0 = (step < nk ? step : nk - 1) * kOpK + 16 * J;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
+            float f[8];
+            if constexpr (sizeof(RawT) == 2) {
+                unpack16<RawT>(rv[S][J][c][0], f);
+            } else {
+                float f0[4], f1[4];
+                unpack16<RawT>(rv[S][J][c][0], f0);
+                unpack16<RawT>(rv[S][J][c][1], f1);
 #pragma unroll
-            for (int j = 0; j < kOpK / 16; ++j) {
-                half8_t h = {}, l = {};
-                if (pvalid[c]) {
-                    float f[8];
-                    if constexpr (sizeof(RawT) == 2) {
-                        unpack16<RawT>(rv[c][j][0], f);
-                    } else {
-                        float f0[4], f1[4];
-                        unpack16<RawT>(rv[c][j][0], f0);
-                        unpack16<RawT>(rv[c][j][1], f1);
-#pragma unroll
-                        for (int e = 0; e < 4; ++e) {
-                            f[e] = f0[e];
-                            f[4 + e] = f1[e];
-                        }
-                    }
-                    float4 b0, b1;
-                    if constexpr (kLdsBg) {
-                        b0 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + 16 * j + ko]);
-                        b1 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + 16 * j + ko + 4]);
-                    } else {
-                        b0 = *reinterpret_cast<const float4*>(brow[c] + k0 + 16 * j);
-                        b1 = *reinterpret_cast<const float4*>(brow[c] + k0 + 16 * j + 4);
-                    }
-                    const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const float d = f[e] - bb[e];
-                        const _Float16 hv = (_Float16)d;
-                        h[e] = hv;
-                        l[e] = (_Float16)(d - (float)hv);
-                    }
+                for (int e = 0; e < 4; ++e) {
+                    f[e] = f0[e];
+                    f[4 + e] = f1[e];
                 }
-                dh[j][c] = h;
-                dl[j][c] = l;
             }
+            float4 b0, b1;
+            if constexpr (kLdsBg) {
+                b0 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + ko]);
+                b1 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + ko + 4]);
+            } else {
+                b0 = *reinterpret_cast<const float4*>(brow[c] + k0);
+                b1 = *reinterpret_cast<const float4*>(brow[c] + k0 + 4);
+            }
+            const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            half8_t h, l;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float d = f[e] - bb[e];
+                const _Float16 hv = (_Float16)d;
+                h[e] = hv;
+                l[e] = (_Float16)__builtin_fmaf((float)hv, mone, d);
+            }
+            dh[J][c] = h;
+            dl[J][c] = l;
         }
     };
     auto stage_op = [&](int sb) {
@@ -445,38 +458,54 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[r][c][v] = 0.f;
 
-    fetch(0);
-    stage_op(0);
-    if constexpr (kLdsBg) __syncthreads();                       // background rows staged
-    convert(0);
-    if (nk > 1) fetch(kOpK);
-    __syncthreads();
-    for (int it = 0; it < nk; ++it) {
-        const int sb = it & 1;
+    auto products = [&](int sb, auto jI) {
+        constexpr int J = decltype(jI)::value;
+        const int kk = 16 * J + ko;
 #pragma unroll
-        for (int ks = 0; ks < kOpK; ks += 16) {
-            const int kk = ks + 8 * (lane >> 5);
-            const half8_t bh[2] = {dh[ks / 16][0], dh[ks / 16][1]};
-            const half8_t bl[2] = {dl[ks / 16][0], dl[ks / 16][1]};
+        for (int r = 0; r < RT; ++r) {
+            const int row = r * 32 + (lane & 31);
+            const half8_t ah = *reinterpret_cast<const half8_t*>(&ahi[sb][row * kOpPad + kk]);
+            const half8_t al = *reinterpret_cast<const half8_t*>(&alo[sb][row * kOpPad + kk]);
 #pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                const int row = r * 32 + (lane & 31);
-                const half8_t ah = *reinterpret_cast<const half8_t*>(&ahi[sb][row * kOpPad + kk]);
-                const half8_t al = *reinterpret_cast<const half8_t*>(&alo[sb][row * kOpPad + kk]);
-#pragma unroll
-                for (int c = 0; c < 2; ++c) {
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[c], acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[c], acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[c], acc[r][c], 0, 0, 0);
-                }
+            for (int c = 0; c < 2; ++c) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, dh[J][c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, dl[J][c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, dh[J][c], acc[r][c], 0, 0, 0);
             }
         }
-        if (it + 1 < nk) {                                       // stage step it + 1
-            stage_op(sb ^ 1);                                    // read by nobody in this step
-            convert((it + 1) * kOpK);
-            if (it + 2 < nk) fetch((it + 2) * kOpK);
-        }
-        __syncthreads();
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    static_for<0, NS>([&](auto s) {                              // steps 0 .. NS-1, both halves
+        fetch_raw(s, I0{}, decltype(s)::value);
+        fetch_raw(s, I1{}, decltype(s)::value);
+    });
+    fetch_op(0);
+    stage_op(0);
+    if constexpr (kLdsBg) __syncthreads();                       // background rows staged
+    convert(I0{}, I0{}, 0);
+    fetch_raw(I0{}, I0{}, NS);
+    fetch_op(1);
+    __syncthreads();
+    for (int it0 = 0; it0 < nk; it0 += NS) {
+        static_for<0, NS>([&](auto sI) {
+            constexpr int S = decltype(sI)::value;
+            constexpr int S1 = (S + 1) % NS;
+            const int it = it0 + S;
+            if (it < nk) {
+                const int sb = it & 1;
+                if (it + 1 < nk) stage_op(sb ^ 1);               // read by nobody in this step
+                fetch_op(it + 2);
+                products(sb, I0{});
+                convert(sI, I1{}, it);
+                fetch_raw(sI, I1{}, it + NS);
+                products(sb, I1{});
+                convert(std::integral_constant<int, S1>{}, I0{}, it + 1);
+                fetch_raw(std::integral_constant<int, S1>{}, I0{}, it + 1 + NS);
+                __syncthreads();
+            }
+        });
     }
 
 #pragma unroll
