@@ -303,6 +303,11 @@ int oct_logmag_crop(const float* mag, int n_bscans, int n_x, long long x_stride,
 /* conv 3x3 / stride 2 / pad 1 + ReLU: x (n, ci, h, w) -> y (n, co, h/2, w/2); w (co, ci, 3, 3); co % 16 == 0 */
 int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
                      int wd, int co, void* stream);
+/* The same layer on the matrix cores (bf16 products, float32 sums) where ci >= 8 and w/2 >= 16,
+ * the FP32-VALU kernels of oct_cnn_conv_fwd elsewhere: the inference forward.  Same arguments and
+ * errors; outputs within one bf16 ulp of oct_cnn_conv_fwd's (the sums' order differs). */
+int oct_cnn_conv_fwd_mfma(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
+                          int wd, int co, void* stream);
 /* dx = conv^T(dpre) (n, ci, h, w) float32, times (xmask > 0) when xmask is given; ci % 16 == 0 */
 int oct_cnn_conv_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci, int h,
                           int wd, int co, void* stream);

@@ -81,6 +81,7 @@ SIGNATURES = {
     "oct_depth_median_opc": (_i, [_vp, _i, _i, _ll, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "oct_logmag_crop": (_i, [_vp, _i, _i, _ll, _i, _i, _i, _i, _i, _i, _f, _f, _vp, _i, _vp]),
     "oct_cnn_conv_fwd": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "oct_cnn_conv_fwd_mfma": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_data": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_weight_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "oct_cnn_conv_bwd_weight": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),

@@ -132,6 +132,129 @@ __global__ __launch_bounds__(kCnnThreads) void conv_fwd_kernel(const uint16_t* _
     }
 }
 
+// The same convolution on the matrix cores (v_mfma_f32_16x16x32_bf16, float32 accumulation), for
+// Co a multiple of 16, at least 8 input channels and outputs at least 16 wide.  As a GEMM, y[co][p] = sum_k W[co][k] X[k][p]
+// over k = (tap, ci), the block's TY x TX output positions p of one image (64 PQ, PQ = 1, 2 or 4)
+// and its 16 CT output channels.  Input channels are staged 16 at a time:
+//   patch: the tile's (2 TY + 1) x (2 TX + 2) input pixels with the 16 channels innermost (48-byte
+//          pixels: 16-byte fragment reads, at most 2-way bank conflicts), zeros outside the image
+//          and past Ci;
+//   wl:    per output channel the 20 groups g = 2 tap + c8 of 8 channels (tap 9 zero), so that
+//          one 32-deep product takes the groups 4 m .. 4 m + 3: taps 2 m, 2 m + 1 of 16 channels.
+// Lane l of a wave holds, for one 16 x 16 x 32 product, k group 4 m + (l >> 4) of output channel
+// row (l & 15) (A) and of position column (l & 15) (B); the result rows 4 (l >> 4) + i, column
+// (l & 15).  Wave w owns the PQ position tiles 16 (PQ w .. PQ w + PQ - 1) and the block's CT
+// channel tiles.  Bias is added to the finished sum, then ReLU and the bf16 rounding.  Grid
+// (ceil(Wo / TX), ceil(Ho / TY), N Co / (16 CT)): blocks never span images and every output's
+// sum is the same for any tile shape (k order, then one product per k block), so an image's
+// outputs do not depend on the batch or on the launch shape.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int kCmCC = 16;                     // input channels per staged chunk
+constexpr int kCmCS = 24;                     // bf16 slots per staged pixel (16 channels + 8 pad)
+constexpr int kCmGroups = 20;                 // weight groups per output channel and chunk
+constexpr int kCmWRow = kCmGroups * 8 + 8;    // bf16 per staged weight row (336 bytes)
+
+template <int TX, int TY, int CT>
+__global__ __launch_bounds__(kCnnThreads) void conv_fwd_mfma_kernel(const uint16_t* __restrict__ x,
+                                                                    const uint16_t* __restrict__ w,
+                                                                    const float* __restrict__ bias,
+                                                                    uint16_t* __restrict__ y, int N, int Ci, int H,
+                                                                    int W, int Co) {
+    constexpr int PQ = TX * TY / 64;                             // position tiles per wave
+    static_assert(PQ == 1 || PQ == 2 || PQ == 4, "64, 128 or 256 positions per block");
+    constexpr int PH = 2 * TY + 1, PW = 2 * TX + 2;              // pixel column px = input ix - (2 oxb - 2)
+    constexpr int kPairs = PH * (PW / 2);
+    __shared__ __align__(16) uint16_t patch[PH * PW * kCmCS];
+    __shared__ __align__(16) uint16_t wl[16 * CT * kCmWRow];
+    const int Ho = H >> 1, Wo = W >> 1;
+    const int splits = Co / (16 * CT);
+    const int n = blockIdx.z / splits, cob = (blockIdx.z - (blockIdx.z / splits) * splits) * 16 * CT;
+    const int oyb = blockIdx.y * TY, oxb = blockIdx.x * TX;
+    const int iyb = 2 * oyb - 1, ixb = 2 * oxb - 2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int grp = lane >> 4;
+    int boff[PQ];                                                // pixel (2 ty, 2 tx + 1): tap (0, 0)
+#pragma unroll
+    for (int q = 0; q < PQ; ++q) {
+        const int pos = 16 * (PQ * wave + q) + (lane & 15);
+        const int ty = pos / TX, tx = pos - (pos / TX) * TX;
+        boff[q] = ((2 * ty) * PW + 2 * tx + 1) * kCmCS;
+    }
+    f32x4_t acc[CT][PQ];
+#pragma unroll
+    for (int t = 0; t < CT; ++t)
+#pragma unroll
+        for (int q = 0; q < PQ; ++q) acc[t][q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* xn = x + (long long)n * Ci * H * W;
+    for (int c0 = 0; c0 < Ci; c0 += kCmCC) {
+        if (c0) __syncthreads();                                 // the last chunk's reads are done
+        // patch: item = (8-channel group c8, row py, pixel pair k): 8 channels of pixels 2k, 2k+1
+        for (int e = tid; e < 2 * kPairs; e += kCnnThreads) {
+            const int c8 = e / kPairs, r = e - (e / kPairs) * kPairs;
+            const int py = r / (PW / 2), k = r - (r / (PW / 2)) * (PW / 2);
+            const int iy = iyb + py, ix = ixb + 2 * k;
+            const bool inside = (unsigned)iy < (unsigned)H && (unsigned)ix < (unsigned)W;   // W even: whole pairs
+            uint32_t v[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int ci = c0 + 8 * c8 + j;
+                v[j] = (inside && ci < Ci) ? *reinterpret_cast<const uint32_t*>(xn + ((long long)ci * H + iy) * W + ix) : 0u;
+            }
+            uint4 lo, hi;                                        // pixel 2k: low halves, 2k + 1: high halves
+            lo.x = __builtin_amdgcn_perm(v[1], v[0], 0x05040100u);
+            lo.y = __builtin_amdgcn_perm(v[3], v[2], 0x05040100u);
+            lo.z = __builtin_amdgcn_perm(v[5], v[4], 0x05040100u);
+            lo.w = __builtin_amdgcn_perm(v[7], v[6], 0x05040100u);
+            hi.x = __builtin_amdgcn_perm(v[1], v[0], 0x07060302u);
+            hi.y = __builtin_amdgcn_perm(v[3], v[2], 0x07060302u);
+            hi.z = __builtin_amdgcn_perm(v[5], v[4], 0x07060302u);
+            hi.w = __builtin_amdgcn_perm(v[7], v[6], 0x07060302u);
+            *reinterpret_cast<uint4*>(&patch[(py * PW + 2 * k) * kCmCS + 8 * c8]) = lo;
+            *reinterpret_cast<uint4*>(&patch[(py * PW + 2 * k + 1) * kCmCS + 8 * c8]) = hi;
+        }
+        // weights: element (co, g, j) = w[co][c0 + 8 (g & 1) + j][g >> 1], zero for tap 9 / past Ci
+        for (int e = tid; e < 16 * CT * kCmGroups * 8; e += kCnnThreads) {
+            const int co = e / (kCmGroups * 8), r = e - (e / (kCmGroups * 8)) * (kCmGroups * 8);
+            const int g = r >> 3, tap = g >> 1, ci = c0 + 8 * (g & 1) + (r & 7);
+            wl[co * kCmWRow + r] = (tap < 9 && ci < Ci) ? w[((long long)(cob + co) * Ci + ci) * 9 + tap] : (uint16_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kCmGroups / 4; ++m) {
+            const int gl = 4 * m + grp;
+            const int tap = gl >> 1;
+            const int tp = tap < 9 ? tap : 0;                    // tap 9 has zero weights: read any pixel
+            const int ky = tp / 3, kx = tp - (tp / 3) * 3;
+            const int doff = (ky * PW + kx) * kCmCS + 8 * (gl & 1);
+            bf16x8_t b[PQ];
+#pragma unroll
+            for (int q = 0; q < PQ; ++q) b[q] = *reinterpret_cast<const bf16x8_t*>(&patch[boff[q] + doff]);
+#pragma unroll
+            for (int t = 0; t < CT; ++t) {
+                const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&wl[(16 * t + (lane & 15)) * kCmWRow + 8 * gl]);
+#pragma unroll
+                for (int q = 0; q < PQ; ++q) acc[t][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[q], acc[t][q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < PQ; ++q) {
+        const int pos = 16 * (PQ * wave + q) + (lane & 15);
+        const int ty = pos / TX, tx = pos - (pos / TX) * TX;
+        const int oy = oyb + ty, ox = oxb + tx;
+        if (oy < Ho && ox < Wo) {
+#pragma unroll
+            for (int t = 0; t < CT; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int co = cob + 16 * t + 4 * grp + i;
+                    y[((long long)n * Co + co) * Ho * Wo + (long long)oy * Wo + ox] = f2bf(fmaxf(acc[t][q][i] + bias[co], 0.f));
+                }
+        }
+    }
+}
+
 // The same convolution for layers with few output positions (small batches, deep layers): one
 // wave per (P horizontally adjacent output positions, 16 output channels); lanes split the
 // reduction over kk = ci 9 + tap (the weights of one output channel are contiguous in kk) and use

@@ -175,6 +175,20 @@ int launch_background(const void* raw, float* bg, int B, int lines, int N, int a
     return OCT_OK;
 }
 
+template <int TX, int TY>
+void launch_conv_mfma(int ct, const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n,
+                             int ci, int h, int wd, int co, hipStream_t s) {
+    const int Ho = h / 2, Wo = wd / 2;
+    const dim3 grid((unsigned)((Wo + TX - 1) / TX), (unsigned)((Ho + TY - 1) / TY), (unsigned)(n * (co / (16 * ct))));
+    const dim3 threads(oct::kCnnThreads);
+    if (ct == 4)
+        hipLaunchKernelGGL((oct::conv_fwd_mfma_kernel<TX, TY, 4>), grid, threads, 0, s, x, w, bias, y, n, ci, h, wd, co);
+    else if (ct == 2)
+        hipLaunchKernelGGL((oct::conv_fwd_mfma_kernel<TX, TY, 2>), grid, threads, 0, s, x, w, bias, y, n, ci, h, wd, co);
+    else
+        hipLaunchKernelGGL((oct::conv_fwd_mfma_kernel<TX, TY, 1>), grid, threads, 0, s, x, w, bias, y, n, ci, h, wd, co);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1112,8 +1126,8 @@ static int conv_shape_ok(int n, int ci, int h, int w, int co) {
     return n > 0 && ci > 0 && co > 0 && h >= 2 && w >= 2 && (h % 2) == 0 && (w % 2) == 0;
 }
 
-int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
-                     int wd, int co, void* stream) {
+static int conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h, int wd,
+                    int co, bool matrix_cores, void* stream) {
     if (!x || !w || !bias || !y) return fail(OCT_EINVAL, "null argument");
     if (!conv_shape_ok(n, ci, h, wd, co)) return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d", n, ci, h, wd, co);
     if (co % oct::kConvCoB) return fail(OCT_EINVAL, "output channels %d not a multiple of %d", co, oct::kConvCoB);
@@ -1121,7 +1135,20 @@ int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
     if ((P * (co / 16) + 3) / 4 > 0x7FFFFFFFLL) return fail(OCT_EINVAL, "batch too large");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int Ho = h / 2, Wo = wd / 2;
-    if (Wo >= 2 * oct::kConvTX) {
+    if (matrix_cores && co % 16 == 0 && ci >= 8 && Wo >= 16) {
+        // matrix cores: 8 x 32 output positions x all channels per block; 2 x 32 when that would
+        // leave fewer than 512 blocks; 4 x 16 positions x 16 channels below 32 columns (fewer
+        // input channels than 8: the LDS-tiled kernel below, whose chunks are not padded)
+        if ((long long)n * co / 16 > 65535) return fail(OCT_EINVAL, "batch too large for one launch");
+        const long long big = (long long)((Wo + 31) / 32) * ((Ho + 7) / 8) * n;
+        const int ct = co % 64 == 0 ? 4 : co % 32 == 0 ? 2 : 1;
+        if (Wo >= 32 && big >= 512)
+            launch_conv_mfma<32, 8>(ct, x, w, bias, y, n, ci, h, wd, co, s);
+        else if (Wo >= 32)
+            launch_conv_mfma<32, 2>(ct, x, w, bias, y, n, ci, h, wd, co, s);
+        else
+            launch_conv_mfma<16, 4>(1, x, w, bias, y, n, ci, h, wd, co, s);
+    } else if (Wo >= 2 * oct::kConvTX) {
         // LDS-tiled: 8 x 32 output positions x 16 channels per block, 8 channels when that
         // would leave fewer than 2 blocks per CU
         const unsigned gx = (Wo + oct::kConvTX - 1) / oct::kConvTX, gy = (Ho + oct::kConvTY - 1) / oct::kConvTY;
@@ -1151,6 +1178,16 @@ int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, ui
     }
     HIP_TRY(hipGetLastError());
     return OCT_OK;
+}
+
+int oct_cnn_conv_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
+                     int wd, int co, void* stream) {
+    return conv_fwd(x, w, bias, y, n, ci, h, wd, co, false, stream);
+}
+
+int oct_cnn_conv_fwd_mfma(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y, int n, int ci, int h,
+                          int wd, int co, void* stream) {
+    return conv_fwd(x, w, bias, y, n, ci, h, wd, co, true, stream);
 }
 
 int oct_cnn_conv_bwd_data(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci, int h,

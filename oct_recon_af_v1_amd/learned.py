@@ -153,8 +153,11 @@ class EstimatorNet:
         return b
 
     # -- passes
-    def forward(self, x):
-        """x (n, 2, 256, 256) bf16 device tensor -> head (n, 4) float32 (kept for backward)."""
+    def forward(self, x, matrix_cores: bool = True):
+        """x (n, 2, 256, 256) bf16 device tensor -> head (n, 4) float32 (kept for backward).
+        matrix_cores: convolutions through oct_cnn_conv_fwd_mfma (inference); False: the FP32-VALU
+        oct_cnn_conv_fwd, which training uses (with the matrix-core forward one of four stage
+        trainings of the 20 000-B-scan recipe ended without learning beta; with this one none did)."""
         torch = self.torch
         if x.dtype != torch.bfloat16 or tuple(x.shape[1:]) != (ARCH_CONV[0][0], IMG, IMG) or not x.is_contiguous():
             raise ValueError(f"input must be contiguous bf16 (n, {ARCH_CONV[0][0]}, {IMG}, {IMG}), got "
@@ -167,7 +170,8 @@ class EstimatorNet:
         for i, (ci, co) in enumerate(ARCH_CONV):
             _need(cur, n * ci * h * h, f"conv{i} input")
             _need(b["acts"][i], n * co * (h // 2) ** 2, f"conv{i} output")
-            _lib.check(L.oct_cnn_conv_fwd(cur.data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
+            conv = L.oct_cnn_conv_fwd_mfma if matrix_cores else L.oct_cnn_conv_fwd
+            _lib.check(conv(cur.data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
                                           self._ptr(self.p32, f"conv{i}.b"), b["acts"][i].data_ptr(), n, ci, h, h, co,
                                           s), f"conv{i} forward")
             cur, h = b["acts"][i], h // 2
@@ -371,7 +375,7 @@ def train(net: EstimatorNet, X, Y, epochs: int = 20, batch: int = 64, lr: float 
             idx = perm[k * batch:(k + 1) * batch]
             torch.index_select(X, 0, idx, out=xb)
             torch.index_select(Y, 0, idx, out=yb)
-            net.forward(xb)
+            net.forward(xb, matrix_cores=False)
             acc += net.loss(batch, yb, w)
             net.backward(batch)
             f = min(1.0, (step + 1) / warmup) * 0.5 * (1 + math.cos(math.pi * step / total))

@@ -46,6 +46,8 @@ class _ExtentLib:
         self._chk(y, 2 * n * co * (h // 2) * (wd // 2), "conv y")
         return 0
 
+    oct_cnn_conv_fwd_mfma = oct_cnn_conv_fwd
+
     def oct_cnn_conv_bwd_data(self, dpre, w, xmask, dx, n, ci, h, wd, co, s):
         self.calls.append(("conv_bwd_data", n, ci, h, co))
         self._chk(dpre, 4 * n * co * (h // 2) * (wd // 2), "dpre")

@@ -71,12 +71,15 @@ def test_conv_layer_kernels(xp, ci, co, h, n):
     b = (rng.standard_normal(co) * 0.1).astype(np.float32)
     xd, wd, bd = _bf(xp, x), _bf(xp, w), _f32(xp, b)
     y = torch.empty((n, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
-    _lib.check(L.oct_cnn_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), n, ci, h, h, co,
-                                  xp.stream()), "conv fwd")
     pre = C.conv_fwd(x, w, b)
-    eq, worst = _bf_close(_np(y), np.maximum(pre, 0), C.conv_fwd(np.abs(x), np.abs(w), np.abs(b)))
-    print(f"forward: {eq:.4%} bit-identical, worst {worst:.2f} of the allowed error", flush=True)
-    assert eq >= 0.995 and worst <= 1.0
+    sabs = C.conv_fwd(np.abs(x), np.abs(w), np.abs(b))
+    for name in ("oct_cnn_conv_fwd", "oct_cnn_conv_fwd_mfma"):   # FP32 VALU; matrix cores
+        y.zero_()
+        _lib.check(getattr(L, name)(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(), y.data_ptr(), n, ci, h, h, co,
+                                    xp.stream()), name)
+        eq, worst = _bf_close(_np(y), np.maximum(pre, 0), sabs)
+        print(f"{name}: {eq:.4%} bit-identical, worst {worst:.2f} of the allowed error", flush=True)
+        assert eq >= 0.995 and worst <= 1.0
     dpre = rng.standard_normal(pre.shape).astype(np.float32)
     dd = _f32(xp, dpre)
     ws_bytes = L.oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, h, co)
@@ -97,11 +100,13 @@ def test_conv_layer_kernels(xp, ci, co, h, n):
         assert _rel(dx.cpu().numpy(), dx_ref) <= 2e-5
 
 
+@pytest.mark.parametrize("name", ["oct_cnn_conv_fwd", "oct_cnn_conv_fwd_mfma"])
 @pytest.mark.parametrize("ci,co,h", [(32, 32, 64), (32, 64, 32), (2, 16, 256), (16, 32, 128)])
-def test_conv_forward_batch_invariant(xp, ci, co, h):
+def test_conv_forward_batch_invariant(xp, ci, co, h, name):
     """An image's convolution is bit-identical whether it runs in a batch of 16 (4 or 2 positions
-    per wave on the narrow layers) or alone (1 position per wave): the launch shape changes only
-    which lanes hold which sums, not the order they are summed in."""
+    per wave on the narrow layers; 8 x 32 position tiles on the matrix cores) or alone (1 position
+    per wave; 2 x 32 tiles): the launch shape changes only which lanes hold which sums, not the
+    order they are summed in."""
     from oct_recon_af_v1_amd import _lib
     print(f"case: conv {ci}->{co} at {h}x{h}, batch 16 vs 1", flush=True)
     torch = xp.torch
@@ -112,13 +117,14 @@ def test_conv_forward_batch_invariant(xp, ci, co, h):
     w = _bf(xp, (rng.standard_normal((co, ci, 3, 3)) / np.sqrt(9 * ci)).astype(np.float32))
     b = _f32(xp, rng.standard_normal(co) * 0.1)
     y = torch.empty((n, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
-    _lib.check(L.oct_cnn_conv_fwd(x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(), n, ci, h, h, co,
-                                  xp.stream()), "conv fwd (batch)")
+    conv = getattr(L, name)
+    _lib.check(conv(x.data_ptr(), w.data_ptr(), b.data_ptr(), y.data_ptr(), n, ci, h, h, co, xp.stream()),
+               "conv fwd (batch)")
     y1 = torch.empty((1, co, h // 2, h // 2), dtype=torch.bfloat16, device=xp.device)
     for k in (0, 5, 15):
         xk = x[k:k + 1].contiguous()
-        _lib.check(L.oct_cnn_conv_fwd(xk.data_ptr(), w.data_ptr(), b.data_ptr(), y1.data_ptr(), 1, ci, h, h, co,
-                                      xp.stream()), "conv fwd (one image)")
+        _lib.check(conv(xk.data_ptr(), w.data_ptr(), b.data_ptr(), y1.data_ptr(), 1, ci, h, h, co, xp.stream()),
+                   "conv fwd (one image)")
         assert torch.equal(y1[0].view(torch.int16), y[k].view(torch.int16)), k
 
 
@@ -200,7 +206,7 @@ def test_network_gradients_vs_oracle(xp):
     tgt = rng.standard_normal((4, 2)).astype(np.float32) * 0.5
     dh = (rng.standard_normal((4, 4)) * 0.1).astype(np.float32)
     xd, td, wd, dhd = _bf(xp, x), _f32(xp, tgt), _f32(xp, [1.0, 1.0]), _f32(xp, dh)
-    head = net.forward(xd)
+    head = net.forward(xd, matrix_cores=False)                 # the training forward
     loss = net.loss(4, td, wd)
     net.backward(4, dhead=dhd)
     xp.synchronize()
