@@ -147,6 +147,25 @@ void launch_opgemm_rf_rt(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
         hipLaunchKernelGGL((oct::opgemm_rf_kernel<RawT, false, RT>), dim3(grid), dim3(256), 0, s, a);
 }
 
+template <typename RawT, int RT>
+void launch_opgemm_lt_rt(const oct::OpGemmArgs& a, int grid, hipStream_t s) {
+    if (a.nX >= oct::kOpCols && a.N == oct::kOpBgN)
+        hipLaunchKernelGGL((oct::opgemm_lt_kernel<RawT, true, RT>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((oct::opgemm_lt_kernel<RawT, false, RT>), dim3(grid), dim3(256), 0, s, a);
+}
+
+// 16-bit samples: the coalesced-load kernel
+template <typename RawT>
+void launch_opgemm_lt(const oct::OpGemmArgs& a, int rt, int grid, hipStream_t s) {
+    switch (rt) {
+        case 1: launch_opgemm_lt_rt<RawT, 1>(a, grid, s); break;
+        case 2: launch_opgemm_lt_rt<RawT, 2>(a, grid, s); break;
+        case 3: launch_opgemm_lt_rt<RawT, 3>(a, grid, s); break;
+        default: launch_opgemm_lt_rt<RawT, 4>(a, grid, s); break;
+    }
+}
+
 template <typename RawT>
 void launch_opgemm_rf(const oct::OpGemmArgs& a, int rt, int grid, hipStream_t s) {
     switch (rt) {
@@ -903,6 +922,8 @@ struct oct_opplan {
     double max_win;               // largest |window| of the creation window (sets sa)
     _Float16* op_hi;
     _Float16* op_lo;
+    _Float16* opt_hi;             // step-tiled copies for opgemm_rf_kernel (opack_kernel)
+    _Float16* opt_lo;
     // kept for in-place rebuilds with another window (oct_opplan_set_window)
     int* d_start;
     int* d_count;
@@ -911,6 +932,13 @@ struct oct_opplan {
     double* d_u;
     double2* d_c;                 // (nzc) rank-1 term, rebuilt with each window
 };
+
+// the rf kernel's step-tiled operator copy, rebuilt after every opbuild_a_kernel
+static void launch_opack(const oct_opplan* p, hipStream_t s) {
+    const long long chunks = (long long)p->r_pad * (p->n / 8);
+    hipLaunchKernelGGL(oct::opack_kernel, dim3((unsigned)((chunks + 255) / 256)), dim3(256), 0, s, p->op_hi, p->op_lo,
+                       p->r_pad, p->n, 32 * p->rt, p->opt_hi, p->opt_lo);
+}
 
 int oct_opplan_rows(int nzc, int* row_tiles) {
     if (nzc < 1) return fail(OCT_EINVAL, "%d bins", nzc);
@@ -994,6 +1022,7 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
     int rc = OCT_OK;
     const size_t opbytes = sizeof(_Float16) * (size_t)r_pad * N;
     if (hipMalloc(&p->op_hi, opbytes) != hipSuccess || hipMalloc(&p->op_lo, opbytes) != hipSuccess ||
+        hipMalloc(&p->opt_hi, opbytes) != hipSuccess || hipMalloc(&p->opt_lo, opbytes) != hipSuccess ||
         hipMemset(p->op_hi, 0, opbytes) != hipSuccess || hipMemset(p->op_lo, 0, opbytes) != hipSuccess ||
         hipMalloc(&d_start, sizeof(int) * N) != hipSuccess || hipMalloc(&d_count, sizeof(int) * N) != hipSuccess ||
         hipMalloc(&d_w, sizeof(double) * N * TT) != hipSuccess || hipMalloc(&d_win, sizeof(double) * 2 * N) != hipSuccess ||
@@ -1010,6 +1039,7 @@ int oct_opplan_create(oct_opplan** out, int device, int n_lambda, int taps, cons
         hipLaunchKernelGGL(oct::opbuild_c_kernel, dim3(nzc), dim3(256), 0, 0, d_win, d_u, N, z0, d_c);
         hipLaunchKernelGGL(oct::opbuild_a_kernel, dim3((N + 255) / 256, nzc_pad), dim3(256), 0, 0, d_start, d_count,
                            d_w, TT, d_win, d_c, N, z0, nzc, nzc_pad, std::ldexp(1.0, sa), p->op_hi, p->op_lo, d_op64);
+        launch_opack(p, 0);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e == hipSuccess && op64_host)
@@ -1040,6 +1070,8 @@ int oct_opplan_destroy(oct_opplan* p) {
     hipSetDevice(p->device);
     if (p->op_hi) hipFree(p->op_hi);
     if (p->op_lo) hipFree(p->op_lo);
+    if (p->opt_hi) hipFree(p->opt_hi);
+    if (p->opt_lo) hipFree(p->opt_lo);
     if (p->d_start) hipFree(p->d_start);
     if (p->d_count) hipFree(p->d_count);
     if (p->d_w) hipFree(p->d_w);
@@ -1065,6 +1097,7 @@ int oct_opplan_set_window(oct_opplan* p, const double* window, void* stream) {
     hipLaunchKernelGGL(oct::opbuild_a_kernel, dim3((p->n + 255) / 256, p->nzc_pad), dim3(256), 0, s, p->d_start,
                        p->d_count, p->d_w, p->tt, win, p->d_c, p->n, p->z0, p->nzc, p->nzc_pad, std::ldexp(1.0, p->sa),
                        p->op_hi, p->op_lo, (double*)nullptr);
+    launch_opack(p, s);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
@@ -1082,6 +1115,8 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     a.bg = bg;
     a.op_hi = p->op_hi;
     a.op_lo = p->op_lo;
+    a.opt_hi = p->opt_hi;
+    a.opt_lo = p->opt_lo;
     a.out = out;
     a.M = (long long)n_files * nX;
     a.lines = lines;
@@ -1098,8 +1133,8 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     if (nblk > (1LL << 30)) return fail(OCT_EINVAL, "batch too large");
     const int grid = (int)((nblk + 7) / 8 * 8);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // OCT_OPGEMM=classic selects the kernel that stages the data fragments through LDS (same
-    // results), for comparisons
+    // OCT_OPGEMM=classic selects the kernel that stages the data fragments through LDS, =rf the
+    // one that loads them directly into registers (same results), for comparisons
     const char* sel = std::getenv("OCT_OPGEMM");
     if (sel && std::strcmp(sel, "classic") == 0 && p->r_pad % oct::kOpRows == 0) {
         switch (raw_dtype) {
@@ -1107,10 +1142,16 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
             case OCT_RAW_U16: hipLaunchKernelGGL(oct::opgemm_kernel<uint16_t>, dim3(grid), dim3(256), 0, s, a); break;
             default: hipLaunchKernelGGL(oct::opgemm_kernel<float>, dim3(grid), dim3(256), 0, s, a); break;
         }
-    } else {
+    } else if (sel && std::strcmp(sel, "rf") == 0) {
         switch (raw_dtype) {
             case OCT_RAW_I16: launch_opgemm_rf<int16_t>(a, rt, grid, s); break;
             case OCT_RAW_U16: launch_opgemm_rf<uint16_t>(a, rt, grid, s); break;
+            default: launch_opgemm_rf<float>(a, rt, grid, s); break;
+        }
+    } else {
+        switch (raw_dtype) {
+            case OCT_RAW_I16: launch_opgemm_lt<int16_t>(a, rt, grid, s); break;
+            case OCT_RAW_U16: launch_opgemm_lt<uint16_t>(a, rt, grid, s); break;
             default: launch_opgemm_rf<float>(a, rt, grid, s); break;
         }
     }

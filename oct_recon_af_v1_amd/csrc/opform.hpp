@@ -19,6 +19,7 @@ namespace oct {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------ operator build (FP64)
 // c[zl] = (1/N) sum_i (win_i / N) e^{2 pi i z i / N} u_i,  u = 1 - S 1.  One block per z.
@@ -96,6 +97,26 @@ __global__ __launch_bounds__(256) void opbuild_a_kernel(const int* __restrict__ 
     op_lo[ri] = (_Float16)(sim - (double)hi_);
 }
 
+// Step-tiled copy of the operator for opgemm_rf_kernel: for row block rb of kR rows and K step s
+// of 32, the kR x 32 halves are contiguous (row-major inside the tile), so a workgroup stages one
+// step's rows with 16-byte loads that cover 1 KB per wave instruction (8 whole cache lines) instead
+// of 64 bytes from each of 32 rows.  dst[((rb nk + s) kR + rr) 32 + kk] = src[(rb kR + rr) N + 32 s + kk].
+// One thread per 16-byte chunk of the row-major source.
+__global__ __launch_bounds__(256) void opack_kernel(const _Float16* __restrict__ src_hi, const _Float16* __restrict__ src_lo,
+                                                    int r_pad, int N, int kR, _Float16* __restrict__ dst_hi,
+                                                    _Float16* __restrict__ dst_lo) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per_row = N / 8;
+    if (e >= (long long)r_pad * per_row) return;
+    const int row = (int)(e / per_row);
+    const int k = 8 * (int)(e - (long long)row * per_row);
+    const int rb = row / kR, rr = row - rb * kR;
+    const int s = k >> 5, kk = k & 31;
+    const long long d = (((long long)rb * (N / 32) + s) * kR + rr) * 32 + kk;
+    *reinterpret_cast<half8_t*>(dst_hi + d) = *reinterpret_cast<const half8_t*>(src_hi + 8 * e);
+    *reinterpret_cast<half8_t*>(dst_lo + d) = *reinterpret_cast<const half8_t*>(src_lo + 8 * e);
+}
+
 // ------------------------------------------------------------------ GEMM + magnitude
 constexpr int kOpCols = 256;       // A-scans per workgroup (4 waves x 64)
 constexpr int kOpRows = 128;       // operator rows per workgroup (4 tiles of 32 = 64 z bins)
@@ -108,6 +129,8 @@ struct OpGemmArgs {
     const float* bg;
     const _Float16* op_hi;        // (R_pad, N)
     const _Float16* op_lo;
+    const _Float16* opt_hi;       // step-tiled copies (opack_kernel) for opgemm_rf_kernel
+    const _Float16* opt_lo;
     float* out;                   // magnitude (M, out_stride), bins [0, nzc)
     long long M;
     int lines, line0, nX;
@@ -364,11 +387,12 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
             *reinterpret_cast<float4*>(&bgs[f][4 * q]) = v;
         }
     }
-    const bool ostage = tid < 2 * kR;                            // stages operator row tid / 2 (whole waves)
-    const int orow = ostage ? tid >> 1 : 0;
-    const int ohalf = (tid & 1) * 16;
-    const _Float16* ohp = a.op_hi + (long long)(row0 + orow) * a.N + ohalf;
-    const _Float16* olp = a.op_lo + (long long)(row0 + orow) * a.N + ohalf;
+    // operator staging from the step-tiled copy: a step is 4 kR 16-byte chunks per part (hi, lo);
+    // thread tid stages chunks tid and tid + 256 (chunk e: row e / 4, halves 8 (e % 4) ..)
+    constexpr int kChunks = 4 * kR;
+    constexpr int kPer = (kChunks + 255) / 256;                  // 1 or 2
+    const _Float16* ohp = a.opt_hi + (long long)rb * nk * kR * kOpK + 8 * tid;
+    const _Float16* olp = a.opt_lo + (long long)rb * nk * kR * kOpK + 8 * tid;
     constexpr int kRawVec = 16 / sizeof(RawT);
     constexpr int kFragLd = 8 / kRawVec;                         // 16-byte loads per fragment
     constexpr int NS = sizeof(RawT) == 2 ? 2 : 1;                // raw register slots (steps ahead)
@@ -391,14 +415,13 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
                 rv[S][J][c][q] = *reinterpret_cast<const uint4*>(rrow[c] + k0 + kRawVec * q);
     };
     auto fetch_op = [&](int step) {
-        if (ostage) {
-            const int k0 = (step < nk ? step : nk - 1) * kOpK;
-            const half8_t* sh = reinterpret_cast<const half8_t*>(ohp + k0);
-            const half8_t* sl = reinterpret_cast<const half8_t*>(olp + k0);
-            oh[0] = sh[0];
-            oh[1] = sh[1];
-            ol[0] = sl[0];
-            ol[1] = sl[1];
+        const long long o = (long long)(step < nk ? step : nk - 1) * kR * kOpK;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            if (tid + 256 * i < kChunks) {
+                oh[i] = *reinterpret_cast<const half8_t*>(ohp + o + 2048 * i);
+                ol[i] = *reinterpret_cast<const half8_t*>(olp + o + 2048 * i);
+            }
         }
     };
     auto convert = [&](auto slotI, auto jI, int step) {
@@ -441,13 +464,14 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
         }
     };
     auto stage_op = [&](int sb) {
-        if (!ostage) return;
-        half8_t* th = reinterpret_cast<half8_t*>(&ahi[sb][orow * kOpPad + ohalf]);
-        half8_t* tl = reinterpret_cast<half8_t*>(&alo[sb][orow * kOpPad + ohalf]);
-        th[0] = oh[0];
-        th[1] = oh[1];
-        tl[0] = ol[0];
-        tl[1] = ol[1];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = tid + 256 * i;
+            if (e < kChunks) {
+                *reinterpret_cast<half8_t*>(&ahi[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = oh[i];
+                *reinterpret_cast<half8_t*>(&alo[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = ol[i];
+            }
+        }
     };
 
     f32x16_t acc[RT][2];
@@ -508,6 +532,241 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
         });
     }
 
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        if (pc >= a.M) continue;
+        float* orow_out = a.out + pc * a.out_stride;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                const int m = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);         // 0..15
+                const int zl = (row0 / 32 + r) * 16 + m;
+                if (zl < a.nzc) {
+                    const float re = acc[r][c][v], im = acc[r][c][v + 8];
+                    orow_out[zl] = __builtin_sqrtf(re * re + im * im) * a.inv_scale;
+                }
+            }
+        }
+    }
+}
+
+// opgemm_rf_kernel with coalesced sample loads (16-bit samples).  The direct fragment loads of
+// opgemm_rf_kernel touch 32 A-scan rows per wave instruction (32 bytes of each), and the vector
+// L1 cannot keep a 128-byte line until its four parts have been read, so every part is a separate
+// L2 request.  Here a wave loads whole lines: lane l takes 16-byte chunk q = l % 8 of the 128-byte
+// line (64 samples = two K steps) of A-scans 8 i + l / 8, i = 0..7 (8 instructions per two steps,
+// 8 whole lines each).  Each K step the wave writes that step's half of the lines (chunks
+// 4 h .. 4 h + 3) into its own LDS region [64 A-scans][32 samples + 8 pad] and reads the B
+// fragments back (16 bytes per lane: A-scan 32 c + l % 32, samples 16 ks + 8 (l / 32)); the
+// padded 80-byte rows make both the chunk writes and the fragment reads free of bank conflicts.
+// The fragments, the conversion and the products are opgemm_rf_kernel's, so the results are the
+// same bits.  The region is the wave's alone, so its writes and reads need no barrier.
+// Loads of a line pair are issued one K step before its first half is written.
+// LDS 2 x RT x 5 KB (operator) + 20 KB (samples) + 16 KB (background rows).
+template <typename RawT, bool kLdsBg, int RT>
+__global__ __launch_bounds__(256, 2) void opgemm_lt_kernel(OpGemmArgs a) {
+    static_assert(sizeof(RawT) == 2, "16-bit samples");
+    constexpr int kR = 32 * RT;
+    constexpr int kRawPad = 40;                                  // samples per staged A-scan row
+    __shared__ __align__(16) _Float16 ahi[2][kR * kOpPad];
+    __shared__ __align__(16) _Float16 alo[2][kR * kOpPad];
+    __shared__ __align__(16) RawT sraw[4][64 * kRawPad];
+    __shared__ __align__(16) float bgs[kLdsBg ? 2 : 1][kLdsBg ? kOpBgN : 4];
+
+    const int nblk = a.row_blocks * a.col_blocks;
+    int rb, cb;
+    {
+        const int id = blockIdx.x;
+        const int xcd = id % 8;
+        const int slot = id / 8;
+        const int per_xcd = (nblk + 7) / 8;
+        const int lin = xcd * per_xcd + slot;
+        if (lin >= nblk) return;                                 // whole workgroup
+        cb = lin / a.row_blocks;
+        rb = lin - cb * a.row_blocks;
+    }
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const long long col0 = (long long)cb * kOpCols;
+    const int nk = a.N / kOpK;                                   // even (N = 2048)
+
+    // fragment side: background rows of A-scans 64 wave + 32 c + lane % 32
+    const int ko = 8 * (lane >> 5);
+    const long long fb0 = col0 / a.nX;
+    const float* brow[2];
+    int fsel[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
+        pc = pc < a.M ? pc : a.M - 1;
+        const long long b = pc / a.nX;
+        brow[c] = a.bg + b * a.N + ko;
+        fsel[c] = (int)(b - fb0);
+    }
+    // load side: raw rows of A-scans 64 wave + 8 i + lane / 8 (past the last A-scan: the last row)
+    const int q = lane & 7;
+    const RawT* rrow[8];
+    static_for<0, 8>([&](auto iI) {
+        constexpr int i = decltype(iI)::value;
+        long long pc = col0 + wave * 64 + 8 * i + (lane >> 3);
+        pc = pc < a.M ? pc : a.M - 1;
+        const long long b = pc / a.nX;
+        const long long x = pc - b * a.nX;
+        rrow[i] = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N + 8 * q;
+    });
+    if constexpr (kLdsBg) {
+        const long long nfiles = (a.M + a.nX - 1) / a.nX;
+        for (int e = tid; e < 2 * (kOpBgN / 4); e += 256) {
+            const int f = e / (kOpBgN / 4), qq = e - f * (kOpBgN / 4);
+            float4 v = {0.f, 0.f, 0.f, 0.f};
+            if (fb0 + f < nfiles) v = *reinterpret_cast<const float4*>(a.bg + (fb0 + f) * a.N + 4 * qq);
+            *reinterpret_cast<float4*>(&bgs[f][4 * qq]) = v;
+        }
+    }
+    constexpr int kChunks = 4 * kR;
+    constexpr int kPer = (kChunks + 255) / 256;
+    const _Float16* ohp = a.opt_hi + (long long)rb * nk * kR * kOpK + 8 * tid;
+    const _Float16* olp = a.opt_lo + (long long)rb * nk * kR * kOpK + 8 * tid;
+    RawT* const wraw = &sraw[wave][0];
+    u32x4_t rv[8];                                               // a native vector: uint4 copies keep the ring in scratch
+    half8_t oh[2], ol[2];
+    half8_t dh[2][2], dl[2][2];
+    float mone = -1.0f;
+    asm volatile("" : "+v"(mone));
+
+    // rv is indexed with compile-time constants only (static_for), so it stays in registers
+    auto load_pair = [&](int m) {                                // samples 64 m .. 64 m + 63
+        const int k0 = 64 * (m < nk / 2 ? m : nk / 2 - 1);
+        static_for<0, 8>([&](auto iI) {
+            constexpr int i = decltype(iI)::value;
+            rv[i] = *reinterpret_cast<const u32x4_t*>(rrow[i] + k0);
+        });
+    };
+    RawT* const wdst = wraw + (lane >> 3) * kRawPad + 8 * (q & 3);
+    auto write_half = [&](int h) {                               // the chunks of step 2 m + h
+        if ((q >> 2) == h) {
+            static_for<0, 8>([&](auto iI) {
+                constexpr int i = decltype(iI)::value;
+                *reinterpret_cast<u32x4_t*>(wdst + 8 * i * kRawPad) = rv[i];
+            });
+        }
+    };
+    // the region is written and read by one wave, whose LDS instructions execute in issue order:
+    // the compiler must only keep the order (a hardware wait here would also wait for the raw
+    // loads in flight)
+    auto fence = [] { asm volatile("" ::: "memory"); };
+    auto fetch_op = [&](int step) {
+        const long long o = (long long)(step < nk ? step : nk - 1) * kR * kOpK;
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            if (tid + 256 * i < kChunks) {
+                oh[i] = *reinterpret_cast<const half8_t*>(ohp + o + 2048 * i);
+                ol[i] = *reinterpret_cast<const half8_t*>(olp + o + 2048 * i);
+            }
+        }
+    };
+    auto stage_op = [&](int sb) {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = tid + 256 * i;
+            if (e < kChunks) {
+                *reinterpret_cast<half8_t*>(&ahi[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = oh[i];
+                *reinterpret_cast<half8_t*>(&alo[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = ol[i];
+            }
+        }
+    };
+    auto convert = [&](auto jI, int step) {                      // fragments (step, ks J) from the region
+        constexpr int J = decltype(jI)::value;
+        const int k0 = (step < nk ? step : nk - 1) * kOpK + 16 * J;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const uint4 u = *reinterpret_cast<const uint4*>(wraw + (32 * c + (lane & 31)) * kRawPad + 16 * J + ko);
+            float f[8];
+            unpack16<RawT>(u, f);
+            float4 b0, b1;
+            if constexpr (kLdsBg) {
+                b0 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + ko]);
+                b1 = *reinterpret_cast<const float4*>(&bgs[fsel[c]][k0 + ko + 4]);
+            } else {
+                b0 = *reinterpret_cast<const float4*>(brow[c] + k0);
+                b1 = *reinterpret_cast<const float4*>(brow[c] + k0 + 4);
+            }
+            const float bb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            half8_t h, l;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float d = f[e] - bb[e];
+                const _Float16 hv = (_Float16)d;
+                h[e] = hv;
+                l[e] = (_Float16)__builtin_fmaf((float)hv, mone, d);
+            }
+            dh[J][c] = h;
+            dl[J][c] = l;
+        }
+    };
+
+    f32x16_t acc[RT][2];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[r][c][v] = 0.f;
+
+    auto products = [&](int sb, auto jI) {
+        constexpr int J = decltype(jI)::value;
+        const int kk = 16 * J + ko;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int row = r * 32 + (lane & 31);
+            const half8_t ah = *reinterpret_cast<const half8_t*>(&ahi[sb][row * kOpPad + kk]);
+            const half8_t al = *reinterpret_cast<const half8_t*>(&alo[sb][row * kOpPad + kk]);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, dh[J][c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, dl[J][c], acc[r][c], 0, 0, 0);
+                acc[r][c] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, dh[J][c], acc[r][c], 0, 0, 0);
+            }
+        }
+    };
+
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    load_pair(0);
+    fetch_op(0);
+    stage_op(0);
+    write_half(0);
+    fence();
+    if constexpr (kLdsBg) __syncthreads();                       // background rows staged
+    convert(I0{}, 0);
+    fetch_op(1);
+    __syncthreads();
+    // step it: the region holds step it's samples, dh / dl [0] its first fragments, operator
+    // buffer it % 2 its rows
+    auto step = [&](int it, auto parity) {
+        constexpr int P = decltype(parity)::value;               // it % 2
+        const int sb = P;
+        if (it + 1 < nk) stage_op(sb ^ 1);
+        fetch_op(it + 2);
+        products(sb, I0{});
+        convert(I1{}, it);
+        fence();
+        write_half(1 - P);                                       // step it + 1's samples
+        if constexpr (P == 0) load_pair(it / 2 + 1);             // the next line pair
+        fence();
+        products(sb, I1{});
+        convert(I0{}, it + 1);
+        __syncthreads();
+    };
+    for (int it = 0; it < nk; it += 2) {
+        step(it, I0{});
+        step(it + 1, I1{});
+    }
+
+    const int row0 = rb * kR;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
