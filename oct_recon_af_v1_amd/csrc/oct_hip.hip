@@ -1130,7 +1130,7 @@ int oct_opspectral(const oct_opplan* p, const void* raw, int raw_dtype, int n_fi
     a.out_stride = out_stride;
     a.inv_scale = (float)std::ldexp(1.0, -p->sa);
     const long long nblk = (long long)a.row_blocks * a.col_blocks;
-    if (nblk > (1LL << 30)) return fail(OCT_EINVAL, "batch too large");
+    if (nblk > (1LL << 30) || a.M >= (1LL << 31)) return fail(OCT_EINVAL, "batch too large");
     const int grid = (int)((nblk + 7) / 8 * 8);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // OCT_OPGEMM=classic selects the kernel that stages the data fragments through LDS, =rf the

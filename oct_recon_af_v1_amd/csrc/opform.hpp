@@ -555,15 +555,19 @@ __global__ __launch_bounds__(256, 2) void opgemm_rf_kernel(OpGemmArgs a) {
 // opgemm_rf_kernel with coalesced sample loads (16-bit samples).  The direct fragment loads of
 // opgemm_rf_kernel touch 32 A-scan rows per wave instruction (32 bytes of each), and the vector
 // L1 cannot keep a 128-byte line until its four parts have been read, so every part is a separate
-// L2 request.  Here a wave loads whole lines: lane l takes 16-byte chunk q = l % 8 of the 128-byte
-// line (64 samples = two K steps) of A-scans 8 i + l / 8, i = 0..7 (8 instructions per two steps,
-// 8 whole lines each).  Each K step the wave writes that step's half of the lines (chunks
-// 4 h .. 4 h + 3) into its own LDS region [64 A-scans][32 samples + 8 pad] and reads the B
-// fragments back (16 bytes per lane: A-scan 32 c + l % 32, samples 16 ks + 8 (l / 32)); the
-// padded 80-byte rows make both the chunk writes and the fragment reads free of bank conflicts.
-// The fragments, the conversion and the products are opgemm_rf_kernel's, so the results are the
-// same bits.  The region is the wave's alone, so its writes and reads need no barrier.
-// Loads of a line pair are issued one K step before its first half is written.
+// L2 request.  Here a wave loads each line (64 samples = two K steps) with two back-to-back
+// instructions: instruction (h, j) gives lane l the 16-byte chunk l % 4 of half h of the line of
+// A-scan 16 j + l / 4 (16 half lines per instruction, the two halves of the same 16 lines issued
+// one after the other).  Each K step 2 m + h the wave writes its four half-h registers into its
+// own LDS region [64 A-scans][32 samples + 8 pad] and reads the B fragments back (16 bytes per
+// lane: A-scan 32 c + l % 32, samples 16 ks + 8 (l / 32)); the padded 80-byte rows keep the
+// fragment reads free of bank conflicts.  The fragments, the conversion and the products are
+// opgemm_rf_kernel's, so the results are the same bits.  The region is the wave's alone, so its
+// writes and reads need no barrier.
+// The step body has no branches and its loads are issued in the order they are consumed
+// (operator rows for step it + 2 at the top of step it, the next line pair in the middle of even
+// steps), so each wait for loaded data waits for exactly that data and not for the prefetches
+// issued after it.
 // LDS 2 x RT x 5 KB (operator) + 20 KB (samples) + 16 KB (background rows).
 template <typename RawT, bool kLdsBg, int RT>
 __global__ __launch_bounds__(256, 2) void opgemm_lt_kernel(OpGemmArgs a) {
@@ -592,30 +596,33 @@ __global__ __launch_bounds__(256, 2) void opgemm_lt_kernel(OpGemmArgs a) {
     const int wave = tid >> 6;
     const long long col0 = (long long)cb * kOpCols;
     const int nk = a.N / kOpK;                                   // even (N = 2048)
+    // A-scan indices and files in 32 bits (the host keeps M below 2^31)
+    const unsigned nX = (unsigned)a.nX, last = (unsigned)(a.M - 1);
+    const unsigned w0 = (unsigned)col0 + 64u * wave;
 
-    // fragment side: background rows of A-scans 64 wave + 32 c + lane % 32
+    // fragment side: background rows of A-scans w0 + 32 c + lane % 32
     const int ko = 8 * (lane >> 5);
-    const long long fb0 = col0 / a.nX;
+    const unsigned fb0 = (unsigned)col0 / nX;
     const float* brow[2];
     int fsel[2];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        long long pc = col0 + wave * 64 + c * 32 + (lane & 31);
-        pc = pc < a.M ? pc : a.M - 1;
-        const long long b = pc / a.nX;
-        brow[c] = a.bg + b * a.N + ko;
+        unsigned pc = w0 + 32u * c + (lane & 31);
+        pc = pc < last ? pc : last;
+        const unsigned b = pc / nX;
+        brow[c] = a.bg + (long long)b * a.N + ko;
         fsel[c] = (int)(b - fb0);
     }
-    // load side: raw rows of A-scans 64 wave + 8 i + lane / 8 (past the last A-scan: the last row)
-    const int q = lane & 7;
-    const RawT* rrow[8];
-    static_for<0, 8>([&](auto iI) {
-        constexpr int i = decltype(iI)::value;
-        long long pc = col0 + wave * 64 + 8 * i + (lane >> 3);
-        pc = pc < a.M ? pc : a.M - 1;
-        const long long b = pc / a.nX;
-        const long long x = pc - b * a.nX;
-        rrow[i] = reinterpret_cast<const RawT*>(a.raw) + (b * a.lines + a.line0 + x) * (long long)a.N + 8 * q;
+    // load side: raw rows of A-scans w0 + 16 j + lane / 4 (past the last A-scan: the last row)
+    const RawT* rrow[4];
+    static_for<0, 4>([&](auto jI) {
+        constexpr int j = decltype(jI)::value;
+        unsigned pc = w0 + 16u * j + (lane >> 2);
+        pc = pc < last ? pc : last;
+        const unsigned b = pc / nX;
+        const unsigned x = pc - b * nX;
+        rrow[j] = reinterpret_cast<const RawT*>(a.raw) + ((long long)b * a.lines + a.line0 + x) * (long long)a.N +
+                  8 * (lane & 3);
     });
     if constexpr (kLdsBg) {
         const long long nfiles = (a.M + a.nX - 1) / a.nX;
@@ -626,56 +633,59 @@ __global__ __launch_bounds__(256, 2) void opgemm_lt_kernel(OpGemmArgs a) {
             *reinterpret_cast<float4*>(&bgs[f][4 * qq]) = v;
         }
     }
+    // operator staging: chunk e = tid + 256 i of the step tile (row e / 4, halves 8 (e % 4) ..);
+    // a thread past the tile's chunks repeats a chunk of the first round (same data, same place)
     constexpr int kChunks = 4 * kR;
     constexpr int kPer = (kChunks + 255) / 256;
-    const _Float16* ohp = a.opt_hi + (long long)rb * nk * kR * kOpK + 8 * tid;
-    const _Float16* olp = a.opt_lo + (long long)rb * nk * kR * kOpK + 8 * tid;
+    int ech[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = tid + 256 * i;
+        ech[i] = e < kChunks ? e : (e - 256 * i) % kChunks;
+    }
+    const _Float16* ohp = a.opt_hi + (long long)rb * nk * kR * kOpK;
+    const _Float16* olp = a.opt_lo + (long long)rb * nk * kR * kOpK;
     RawT* const wraw = &sraw[wave][0];
-    u32x4_t rv[8];                                               // a native vector: uint4 copies keep the ring in scratch
-    half8_t oh[2], ol[2];
+    RawT* const wdst = wraw + (lane >> 2) * kRawPad + 8 * (lane & 3);
+    u32x4_t rv[2][4];                                            // [half][j]; a native vector type, indexed
+    half8_t oh[kPer], ol[kPer];                                  // with constants only, stays in registers
     half8_t dh[2][2], dl[2][2];
     float mone = -1.0f;
     asm volatile("" : "+v"(mone));
 
-    // rv is indexed with compile-time constants only (static_for), so it stays in registers
     auto load_pair = [&](int m) {                                // samples 64 m .. 64 m + 63
         const int k0 = 64 * (m < nk / 2 ? m : nk / 2 - 1);
-        static_for<0, 8>([&](auto iI) {
-            constexpr int i = decltype(iI)::value;
-            rv[i] = *reinterpret_cast<const u32x4_t*>(rrow[i] + k0);
+        static_for<0, 4>([&](auto jI) {
+            constexpr int j = decltype(jI)::value;
+            rv[0][j] = *reinterpret_cast<const u32x4_t*>(rrow[j] + k0);
+            rv[1][j] = *reinterpret_cast<const u32x4_t*>(rrow[j] + k0 + 32);
         });
     };
-    RawT* const wdst = wraw + (lane >> 3) * kRawPad + 8 * (q & 3);
-    auto write_half = [&](int h) {                               // the chunks of step 2 m + h
-        if ((q >> 2) == h) {
-            static_for<0, 8>([&](auto iI) {
-                constexpr int i = decltype(iI)::value;
-                *reinterpret_cast<u32x4_t*>(wdst + 8 * i * kRawPad) = rv[i];
-            });
-        }
+    auto write_half = [&](auto hI) {                             // step 2 m + h's samples into the region
+        constexpr int h = decltype(hI)::value;
+        static_for<0, 4>([&](auto jI) {
+            constexpr int j = decltype(jI)::value;
+            *reinterpret_cast<u32x4_t*>(wdst + 16 * j * kRawPad) = rv[h][j];
+        });
     };
     // the region is written and read by one wave, whose LDS instructions execute in issue order:
-    // the compiler must only keep the order (a hardware wait here would also wait for the raw
-    // loads in flight)
+    // the compiler must only keep the order (a hardware wait here would also wait for the loads
+    // in flight)
     auto fence = [] { asm volatile("" ::: "memory"); };
     auto fetch_op = [&](int step) {
         const long long o = (long long)(step < nk ? step : nk - 1) * kR * kOpK;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            if (tid + 256 * i < kChunks) {
-                oh[i] = *reinterpret_cast<const half8_t*>(ohp + o + 2048 * i);
-                ol[i] = *reinterpret_cast<const half8_t*>(olp + o + 2048 * i);
-            }
+            oh[i] = *reinterpret_cast<const half8_t*>(ohp + o + 8 * ech[i]);
+            ol[i] = *reinterpret_cast<const half8_t*>(olp + o + 8 * ech[i]);
         }
     };
     auto stage_op = [&](int sb) {
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int e = tid + 256 * i;
-            if (e < kChunks) {
-                *reinterpret_cast<half8_t*>(&ahi[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = oh[i];
-                *reinterpret_cast<half8_t*>(&alo[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = ol[i];
-            }
+            const int e = ech[i];
+            *reinterpret_cast<half8_t*>(&ahi[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = oh[i];
+            *reinterpret_cast<half8_t*>(&alo[sb][(e >> 2) * kOpPad + 8 * (e & 3)]) = ol[i];
         }
     };
     auto convert = [&](auto jI, int step) {                      // fragments (step, ks J) from the region
@@ -735,29 +745,28 @@ __global__ __launch_bounds__(256, 2) void opgemm_lt_kernel(OpGemmArgs a) {
 
     using I0 = std::integral_constant<int, 0>;
     using I1 = std::integral_constant<int, 1>;
-    load_pair(0);
     fetch_op(0);
+    load_pair(0);
     stage_op(0);
-    write_half(0);
+    fetch_op(1);
+    write_half(I0{});
     fence();
     if constexpr (kLdsBg) __syncthreads();                       // background rows staged
     convert(I0{}, 0);
-    fetch_op(1);
     __syncthreads();
     // step it: the region holds step it's samples, dh / dl [0] its first fragments, operator
-    // buffer it % 2 its rows
+    // buffer it % 2 its rows, oh / ol step it + 1's rows; rv holds the line pair of step it + 1
     auto step = [&](int it, auto parity) {
         constexpr int P = decltype(parity)::value;               // it % 2
-        const int sb = P;
-        if (it + 1 < nk) stage_op(sb ^ 1);
+        stage_op(P ^ 1);                                         // step it + 1's rows (unused after the last step)
         fetch_op(it + 2);
-        products(sb, I0{});
+        products(P, I0{});
         convert(I1{}, it);
         fence();
-        write_half(1 - P);                                       // step it + 1's samples
+        write_half(std::integral_constant<int, P ^ 1>{});        // step it + 1's samples
         if constexpr (P == 0) load_pair(it / 2 + 1);             // the next line pair
         fence();
-        products(sb, I1{});
+        products(P, I1{});
         convert(I0{}, it + 1);
         __syncthreads();
     };
