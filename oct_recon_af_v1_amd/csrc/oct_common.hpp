@@ -116,6 +116,54 @@ __device__ __forceinline__ void dft_inv_dif(cf (&v)[NP]) {
     });
 }
 
+// Radix-2 decimation-in-time butterfly with the compile-time twiddle W = exp(+2 pi i J / LEN),
+// J < LEN / 2:  x <- x + W y,  y <- x - W y.  A non-trivial twiddle takes 6 instructions instead
+// of 8: x + W y is formed with fused multiply-adds and x - W y as 2 x - (x + W y).
+template <long J, long LEN>
+__device__ __forceinline__ void bfly_dit(cf& x, cf& y) {
+    constexpr long k = J % LEN;
+    if constexpr (k == 0) {
+        const cf a{x.x + y.x, x.y + y.y}, b{x.x - y.x, x.y - y.y};
+        x = a;
+        y = b;
+    } else if constexpr (4 * k == LEN) {                        // W = i
+        const cf a{x.x - y.y, x.y + y.x}, b{x.x + y.y, x.y - y.x};
+        x = a;
+        y = b;
+    } else if constexpr (8 * k == LEN || 8 * k == 3 * LEN) {    // odd multiple of pi / 4
+        constexpr float c = (float)ct_cos2pi(k, LEN);
+        constexpr float sc = (ct_sin2pi(k, LEN) > 0) == (ct_cos2pi(k, LEN) > 0) ? 1.0f : -1.0f;
+        const float t1 = y.x - sc * y.y, t2 = sc * y.x + y.y;   // W y = c (t1, t2)
+        const cf a{__builtin_fmaf(c, t1, x.x), __builtin_fmaf(c, t2, x.y)};
+        y = cf{__builtin_fmaf(2.0f, x.x, -a.x), __builtin_fmaf(2.0f, x.y, -a.y)};
+        x = a;
+    } else {
+        constexpr float c = (float)ct_cos2pi(k, LEN);
+        constexpr float s = (float)ct_sin2pi(k, LEN);
+        const cf a{__builtin_fmaf(c, y.x, __builtin_fmaf(-s, y.y, x.x)), __builtin_fmaf(c, y.y, __builtin_fmaf(s, y.x, x.y))};
+        y = cf{__builtin_fmaf(2.0f, x.x, -a.x), __builtin_fmaf(2.0f, x.y, -a.y)};
+        x = a;
+    }
+}
+
+// Radix-2 decimation-in-time DFT of size NP with kernel exp(+2 pi i / NP), stages S0 .. log2 NP - 1
+// (the caller has done the stages before S0).  Input in bit-reversed order (v[i] = x[bitrev(i)]),
+// output natural (v[k] = Y[k]).
+template <int NP, int S0>
+__device__ __forceinline__ void dft_inv_dit(cf (&v)[NP]) {
+    constexpr int L = ct_log2(NP);
+    static_for<S0, L>([&](auto s) {
+        constexpr int len = 2 << decltype(s)::value;
+        constexpr int half = len / 2;
+        static_for<0, NP / len>([&](auto blk) {
+            static_for<0, half>([&](auto j) {
+                constexpr int a = decltype(blk)::value * len + decltype(j)::value;
+                bfly_dit<decltype(j)::value, len>(v[a], v[a + half]);
+            });
+        });
+    });
+}
+
 // Wave-local LDS ordering point: orders this wave's LDS accesses before/after, without a
 // workgroup barrier (all lanes of a wave execute in lockstep).
 __device__ __forceinline__ void wave_lds_sync() {

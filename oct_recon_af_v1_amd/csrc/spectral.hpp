@@ -521,17 +521,22 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
         if (wave < nvalid) {
             const float2* wg = reinterpret_cast<const float2*>(a.win + (long long)rep * a.win_rstride +
                                                                (long long)b * a.win_fstride);
+            // the 32-point DFT over q by decimation in time: bit-reversed positions, so its first
+            // radix-2 stage pairs samples q and q + 16, with the window multiply fused in
             cf v[kQ];
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const float e = R[lane + 64 * q];
-                const float2 w = wg[lane + 64 * q];
-                v[q] = cf{e * w.x, e * w.y};
-            }
+            static_for<0, kQ / 2>([&](auto iI) {
+                constexpr int i = decltype(iI)::value;
+                constexpr int q = bitrev(2 * i, 5);              // < 16; position 2 i + 1 holds q + 16
+                const float e0 = R[lane + 64 * q], e1 = R[lane + 64 * (q + 16)];
+                const float2 w0 = wg[lane + 64 * q], w1 = wg[lane + 64 * (q + 16)];
+                const float px = e1 * w1.x, py = e1 * w1.y;
+                v[2 * i] = cf{__builtin_fmaf(e0, w0.x, px), __builtin_fmaf(e0, w0.y, py)};
+                v[2 * i + 1] = cf{__builtin_fmaf(e0, w0.x, -px), __builtin_fmaf(e0, w0.y, -py)};
+            });
             wave_lds_sync();
-            dft_inv_dif<kQ>(v);                          // Y[k1] in v[bitrev(k1)]
+            dft_inv_dit<kQ, 1>(v);                       // Y[k1] in v[k1]
             static_for<1, kQ>([&](auto k1) {
-                constexpr int r = bitrev(decltype(k1)::value, 5);
+                constexpr int r = decltype(k1)::value;
                 if constexpr (kLdsTw) {
                     v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
                 } else {
@@ -547,7 +552,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             float vr[kQ];
             float exr[kEx], exi[kEx];                    // Z[lane][k1] of the extra rows (STAGE2 == 1)
             __builtin_amdgcn_sched_barrier(0);
-            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].x; });
+            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[decltype(k1)::value].x; });
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < kQ; ++m) vr[m] = R[kk1 * kStride + 2 * m + h];
@@ -556,7 +561,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                 for (int e = 0; e < kEx; ++e) exr[e] = R[((row0 + e) & 31) * kStride + lane];
             }
             wave_lds_sync();
-            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[bitrev(decltype(k1)::value, 5)].y; });
+            static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[decltype(k1)::value].y; });
             wave_lds_sync();
 #pragma unroll
             for (int m = 0; m < kQ; ++m) v[m] = cf{vr[m], R[kk1 * kStride + 2 * m + h]};
