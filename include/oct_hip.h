@@ -315,6 +315,18 @@ size_t oct_cnn_conv_bwd_weight_workspace_bytes(int n, int ci, int h, int wd, int
 /* dw (co, ci, 3, 3) and db (co) float32 from dpre (n, co, h/2, w/2) and x (n, ci, h, w) */
 int oct_cnn_conv_bwd_weight(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int ci, int h, int wd,
                             int co, void* workspace, size_t workspace_bytes, void* stream);
+/* The two backward convolutions on the matrix cores (bf16 products, float32 sums; dpre enters as
+ * two bf16 parts that hold it to 2^-16): the training backward.  Same arguments and results as
+ * oct_cnn_conv_bwd_data / oct_cnn_conv_bwd_weight to float32 rounding, with a workspace (16-byte
+ * aligned, size from the *_workspace_bytes query, 0 = unsupported shape).
+ * bwd_weight: co % 16 == 0; im2col of x, a split GEMM over the positions, a reduction.
+ * bwd_data: ci % 16 == 0 and ci <= 64, co % 8 == 0; one GEMM per input-pixel parity class. */
+size_t oct_cnn_conv_bwd_weight_mfma_workspace_bytes(int n, int ci, int h, int wd, int co);
+int oct_cnn_conv_bwd_weight_mfma(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int ci, int h,
+                                 int wd, int co, void* workspace, size_t workspace_bytes, void* stream);
+size_t oct_cnn_conv_bwd_data_mfma_workspace_bytes(int n, int ci, int h, int wd, int co);
+int oct_cnn_conv_bwd_data_mfma(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci,
+                               int h, int wd, int co, void* workspace, size_t workspace_bytes, void* stream);
 /* fully connected y = x w^T + b: exactly one of y_relu_bf16 (ReLU, bf16) or y_linear (float32) */
 int oct_cnn_fc_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y_relu_bf16, float* y_linear,
                    int n, int k, int o, void* stream);

@@ -85,6 +85,10 @@ SIGNATURES = {
     "oct_cnn_conv_bwd_data": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "oct_cnn_conv_bwd_weight_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
     "oct_cnn_conv_bwd_weight": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "oct_cnn_conv_bwd_weight_mfma_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "oct_cnn_conv_bwd_weight_mfma": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
+    "oct_cnn_conv_bwd_data_mfma_workspace_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "oct_cnn_conv_bwd_data_mfma": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp, _sz, _vp]),
     "oct_cnn_fc_fwd": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "oct_cnn_fc_bwd_data": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "oct_cnn_fc_bwd_weight": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

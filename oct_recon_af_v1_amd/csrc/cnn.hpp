@@ -466,6 +466,285 @@ __global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_reduce_kernel(con
     }
 }
 
+// ---------------------------------------------------------------- backward on the matrix cores
+// Both backward convolutions as GEMMs on v_mfma_f32_16x16x32_bf16 with float32 sums.  The float32
+// output gradient enters as two bf16 parts, hi = bf16(d) and lo = bf16(d - hi) (together within
+// 2^-16 |d| of d), and every product is made for both; x and w are bf16 already, so each product
+// is exact and the sums are the only float32 rounding.
+
+__device__ __forceinline__ void split_bf16(float d, uint16_t& hi, uint16_t& lo) {
+    hi = f2bf(d);
+    lo = f2bf(d - bf2f(hi));                 // d - hi is exact in float32
+}
+
+// Weight gradient, step 1: xcol[c][p] (bf16; rows c < ncols_pad, columns p < ppad = P rounded up
+// to 32) with row c = 9 ci + 3 ky + kx holding x[n, ci, 2 oy - 1 + ky, 2 ox - 1 + kx] (0 outside
+// the image) at p = (n Ho + oy) Wo + ox, row 9 Ci holding 1 (the bias column), and 0 in the other
+// rows and past P.  One thread per 8 consecutive p of one row (one 16-byte store).
+__global__ __launch_bounds__(kCnnThreads) void conv_im2col_kernel(const uint16_t* __restrict__ x,
+                                                                  uint16_t* __restrict__ xcol, int Ci, int H,
+                                                                  int W, int P, int ppad, int ncols_pad) {
+    const long long e = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    const int per_row = ppad / 8;
+    if (e >= (long long)ncols_pad * per_row) return;
+    const int c = (int)(e / per_row);
+    const int p0 = 8 * (int)(e - (long long)c * per_row);
+    const int Ho = H >> 1, Wo = W >> 1;
+    uint16_t v[8];
+    if (c < 9 * Ci) {
+        const int ci = c / 9, tap = c - 9 * ci, ky = tap / 3, kx = tap - 3 * ky;
+        int ox = p0 % Wo, t = p0 / Wo;
+        int oy = t % Ho, n = t / Ho;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int iy = 2 * oy - 1 + ky, ix = 2 * ox - 1 + kx;
+            const bool in = p0 + j < P && iy >= 0 && iy < H && ix >= 0 && ix < W;
+            v[j] = in ? x[(((long long)n * Ci + ci) * H + iy) * W + ix] : (uint16_t)0;
+            if (++ox == Wo) {
+                ox = 0;
+                if (++oy == Ho) {
+                    oy = 0;
+                    ++n;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (c == 9 * Ci && p0 + j < P) ? (uint16_t)0x3F80u : (uint16_t)0;   // bf16 1.0
+    }
+    uint4 o;
+    o.x = v[0] | ((uint32_t)v[1] << 16);
+    o.y = v[2] | ((uint32_t)v[3] << 16);
+    o.z = v[4] | ((uint32_t)v[5] << 16);
+    o.w = v[6] | ((uint32_t)v[7] << 16);
+    *reinterpret_cast<uint4*>(xcol + (long long)c * ppad + p0) = o;
+}
+
+// Weight gradient, step 2: part[s][co][c] = sum over slice s of the positions of
+// dpre[n, co, p] xcol[c][p].  One wave per (16 output channels, 4 column tiles of 16, slice);
+// per 32 positions each lane holds 8 consecutive positions of its channel (A: both bf16 parts)
+// and of its column (B).  HoWo % 8 == 0: a lane's 8 positions lie in one image (two 16-byte
+// loads); otherwise they are gathered one by one.  Slices are whole multiples of 32 positions.
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_mfma_kernel(const float* __restrict__ dpre,
+                                                                           const uint16_t* __restrict__ xcol,
+                                                                           float* __restrict__ part, int Co, int HoWo,
+                                                                           int P, int ppad, int ncols_pad, int S,
+                                                                           int slice) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ntiles = ncols_pad / 16, ngroups = (ntiles + 3) / 4;
+    const long long task = (long long)blockIdx.x * (kCnnThreads / 64) + wave;
+    if (task >= (long long)(Co / 16) * ngroups * S) return;                // whole wave
+    const int s = (int)(task % S);
+    const int rest = (int)(task / S);
+    const int ng = rest % ngroups, mt = rest / ngroups;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int co = 16 * mt + l16;
+    const bool vec = (HoWo % 8) == 0;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    const int k_begin = s * slice;
+    const int k_end = min(ppad, k_begin + slice);
+    for (int k0 = k_begin; k0 < k_end; k0 += 32) {
+        const int p = k0 + 8 * g;
+        float d[8];
+        if (vec) {
+            if (p < P) {
+                const int n = p / HoWo, q = p - n * HoWo;
+                const float4* src = reinterpret_cast<const float4*>(dpre + ((long long)n * Co + co) * HoWo + q);
+                const float4 a0 = src[0], a1 = src[1];
+                d[0] = a0.x; d[1] = a0.y; d[2] = a0.z; d[3] = a0.w;
+                d[4] = a1.x; d[5] = a1.y; d[6] = a1.z; d[7] = a1.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) d[j] = 0.f;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int pj = p + j;
+                const int n = pj / HoWo, q = pj - n * HoWo;
+                d[j] = pj < P ? dpre[((long long)n * Co + co) * HoWo + q] : 0.f;
+            }
+        }
+        bf16x8_t ah, al;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            uint16_t h, l;
+            split_bf16(d[j], h, l);
+            ah[j] = __builtin_bit_cast(__bf16, h);
+            al[j] = __builtin_bit_cast(__bf16, l);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int tile = 4 * ng + t;
+            if (tile < ntiles) {                                           // uniform
+                const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(xcol + (long long)(16 * tile + l16) * ppad + p);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, b, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int tile = 4 * ng + t;
+        if (tile < ntiles) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                part[((long long)s * Co + 16 * mt + 4 * g + i) * ncols_pad + 16 * tile + l16] = acc[t][i];
+        }
+    }
+}
+
+// Weight gradient, step 3: dw[co][c] = sum_s part[s][co][c] for c < 9 Ci, db[co] = the bias column.
+// Block = 16 columns x 16 slice groups: thread (r, j) sums slices r, r + 16, ... of column
+// 16 blockIdx.x + j (reads along c), then the 16 group sums are added in group order through LDS
+// (a fixed order: the same bits on every run).  Grid (ceil((9 Ci + 1) / 16), Co).
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_weight_mfma_reduce_kernel(const float* __restrict__ part,
+                                                                                  float* __restrict__ dw,
+                                                                                  float* __restrict__ db, int Ci,
+                                                                                  int Co, int ncols_pad, int S) {
+    const int j = threadIdx.x & 15, r = threadIdx.x >> 4;
+    const int co = blockIdx.y, c = 16 * blockIdx.x + j;
+    const int nc = 9 * Ci + 1;
+    float v = 0.f;
+    if (c < nc)
+        for (int s = r; s < S; s += 16) v += part[((long long)s * Co + co) * ncols_pad + c];
+    __shared__ float red[16][17];
+    red[r][j] = v;
+    __syncthreads();
+    if (r == 0 && c < nc) {
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t += red[q][j];
+        if (c < 9 * Ci)
+            dw[(long long)co * 9 * Ci + c] = t;
+        else
+            db[co] = t;
+    }
+}
+
+// Data gradient, step 1: dth / dtl [p][co] = the two bf16 parts of dpre[n, co, p] (position-major,
+// so that 8 consecutive channels are one 16-byte load).  One thread per (8 channels, position);
+// consecutive threads take consecutive positions (coalesced reads of each channel).
+__global__ __launch_bounds__(kCnnThreads) void conv_dpre_split_kernel(const float* __restrict__ dpre,
+                                                                      uint16_t* __restrict__ dth,
+                                                                      uint16_t* __restrict__ dtl, int Co, int HoWo,
+                                                                      int P) {
+    const long long e = (long long)blockIdx.x * kCnnThreads + threadIdx.x;
+    if (e >= (long long)(Co / 8) * P) return;
+    const int c8 = (int)(e / P), p = (int)(e - (long long)c8 * P);
+    const int n = p / HoWo, q = p - n * HoWo;
+    uint16_t h[8], l[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) split_bf16(dpre[((long long)n * Co + 8 * c8 + j) * HoWo + q], h[j], l[j]);
+    uint4 oh, ol;
+    oh.x = h[0] | ((uint32_t)h[1] << 16); oh.y = h[2] | ((uint32_t)h[3] << 16);
+    oh.z = h[4] | ((uint32_t)h[5] << 16); oh.w = h[6] | ((uint32_t)h[7] << 16);
+    ol.x = l[0] | ((uint32_t)l[1] << 16); ol.y = l[2] | ((uint32_t)l[3] << 16);
+    ol.z = l[4] | ((uint32_t)l[5] << 16); ol.w = l[6] | ((uint32_t)l[7] << 16);
+    *reinterpret_cast<uint4*>(dth + (long long)p * Co + 8 * c8) = oh;
+    *reinterpret_cast<uint4*>(dtl + (long long)p * Co + 8 * c8) = ol;
+}
+
+// Data gradient, step 2: wt[tap][ci][co] = w[co][ci][tap] (bf16).
+__global__ __launch_bounds__(kCnnThreads) void conv_w_transpose_kernel(const uint16_t* __restrict__ w,
+                                                                       uint16_t* __restrict__ wt, int Ci, int Co) {
+    const int e = blockIdx.x * kCnnThreads + threadIdx.x;
+    if (e >= 9 * Ci * Co) return;
+    const int co = e % Co, r = e / Co;
+    const int ci = r % Ci, tap = r / Ci;
+    wt[e] = w[((long long)co * Ci + ci) * 9 + tap];
+}
+
+// Data gradient, step 3.  Input pixels of one parity class (py, px) = (iy % 2, ix % 2) take the
+// taps with ky = 1 (py = 0) or ky in {0, 2} (py = 1), the same for x, and the output position of
+// tap ky is oy = a + (ky == 0) for iy = 2 a + py:
+//   dx[n, ci, 2 a + py, 2 b + px] = sum_taps sum_co w[co, ci, ky, kx] dpre[n, co, a + dy, b + dx]
+// (terms past the last output row or column are 0), times (xmask > 0) when xmask is given.  A GEMM
+// per class: M = Ci, K = (taps, co), N = the N Ho Wo positions (a, b).  One wave per (class,
+// 16 positions), all Ci / 16 row tiles; K in steps of 32 (a lane's 8 values lie in one tap).
+// Grid (ceil(N Ho Wo / 64), 4).
+__global__ __launch_bounds__(kCnnThreads) void conv_bwd_data_mfma_kernel(const uint16_t* __restrict__ dth,
+                                                                         const uint16_t* __restrict__ dtl,
+                                                                         const uint16_t* __restrict__ wt,
+                                                                         const uint16_t* __restrict__ xmask,
+                                                                         float* __restrict__ dx, int N, int Ci, int H,
+                                                                         int W, int Co) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int Ho = H >> 1, Wo = W >> 1;
+    const int P = N * Ho * Wo;
+    const int tile = blockIdx.x * (kCnnThreads / 64) + wave;
+    if (16 * tile >= P) return;                                            // whole wave
+    const int py = blockIdx.y >> 1, px = blockIdx.y & 1;
+    const int ny = py ? 2 : 1, nx = px ? 2 : 1;
+    const int K = ny * nx * Co;
+    const int g = lane >> 4, l16 = lane & 15;
+    const int pos = 16 * tile + l16;
+    const int posc = pos < P ? pos : P - 1;
+    const int b = posc % Wo, t0 = posc / Wo;
+    const int a = t0 % Ho, n = t0 / Ho;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    const int mtiles = Ci / 16;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const int k = k0 + 8 * g;
+        bf16x8_t bh, bl;
+        int ky = 0, kx = 0, co = 0;
+        bool valid = false;
+        if (k < K) {
+            const int t = k / Co;
+            co = k - t * Co;
+            const int ty = t / nx, tx = t - ty * nx;
+            ky = py ? 2 * ty : 1;                                          // ty 0 -> ky 0 (shift 1), 1 -> ky 2
+            kx = px ? 2 * tx : 1;
+            const int aa = a + (ky == 0), bb = b + (kx == 0);
+            valid = pos < P && aa < Ho && bb < Wo;
+            const long long row = ((long long)n * Ho + (valid ? aa : a)) * Wo + (valid ? bb : b);
+            bh = *reinterpret_cast<const bf16x8_t*>(dth + row * Co + co);
+            bl = *reinterpret_cast<const bf16x8_t*>(dtl + row * Co + co);
+        }
+        if (!valid) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                bh[j] = __builtin_bit_cast(__bf16, (uint16_t)0);
+                bl[j] = __builtin_bit_cast(__bf16, (uint16_t)0);
+            }
+        }
+        const int tap = 3 * ky + kx;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            if (m < mtiles) {                                              // uniform
+                bf16x8_t av;
+                if (k < K) {
+                    av = *reinterpret_cast<const bf16x8_t*>(wt + ((long long)tap * Ci + 16 * m + l16) * Co + co);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) av[j] = __builtin_bit_cast(__bf16, (uint16_t)0);
+                }
+                acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bh, acc[m], 0, 0, 0);
+                acc[m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bl, acc[m], 0, 0, 0);
+            }
+        }
+    }
+    if (pos >= P) return;
+    const int iy = 2 * a + py, ix = 2 * b + px;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (m < mtiles) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int ci = 16 * m + 4 * g + i;
+                const long long o = (((long long)n * Ci + ci) * H + iy) * W + ix;
+                float v = acc[m][i];
+                if (xmask && !(bf2f(xmask[o]) > 0.f)) v = 0.f;
+                dx[o] = v;
+            }
+        }
+    }
+}
+
 // y[n, o] = b[o] + sum_k w[o, k] x[n, k]: one wave per output (lanes over k, then a wave
 // reduction); ReLU + bf16 store, or a float32 store.  Grid ceil(N O / 4) blocks of 4 waves.
 __global__ __launch_bounds__(kCnnThreads) void fc_fwd_kernel(const uint16_t* __restrict__ x,

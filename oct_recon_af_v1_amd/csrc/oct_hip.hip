@@ -1278,6 +1278,110 @@ int oct_cnn_conv_bwd_weight(const float* dpre, const uint16_t* x, float* dw, flo
     return OCT_OK;
 }
 
+// ---- backward convolutions on the matrix cores
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct BwdWeightMfmaPlan {
+    int P, ppad, ncols_pad, S, slice;
+    size_t xcol_bytes, part_bytes;
+};
+
+static bool bwd_weight_mfma_plan(int n, int ci, int h, int wd, int co, BwdWeightMfmaPlan* q) {
+    if (!conv_shape_ok(n, ci, h, wd, co) || co % 16 || co > 65535) return false;
+    const long long P = (long long)n * (h / 2) * (wd / 2);
+    const long long ncols = 9LL * ci + 1;
+    if (P >= (1LL << 30) || ncols > (1 << 20)) return false;
+    q->P = (int)P;
+    q->ppad = (int)((P + 31) / 32 * 32);
+    q->ncols_pad = (int)((ncols + 15) / 16 * 16);
+    // slices of whole 32-position steps, about 2048 waves in all, at least 256 positions each
+    const long long tasks = (long long)(co / 16) * ((q->ncols_pad / 16 + 3) / 4);
+    long long s = std::max<long long>(1, (2048 + tasks - 1) / tasks);
+    long long slice = (q->ppad + s - 1) / s;
+    slice = std::max<long long>(256, (slice + 31) / 32 * 32);
+    q->slice = (int)std::min<long long>(slice, q->ppad);
+    q->S = (int)((q->ppad + q->slice - 1) / q->slice);
+    q->xcol_bytes = align256(sizeof(uint16_t) * (size_t)q->ncols_pad * (size_t)q->ppad);
+    q->part_bytes = align256(sizeof(float) * (size_t)q->S * (size_t)co * (size_t)q->ncols_pad);
+    return true;
+}
+
+size_t oct_cnn_conv_bwd_weight_mfma_workspace_bytes(int n, int ci, int h, int wd, int co) {
+    BwdWeightMfmaPlan q;
+    if (!bwd_weight_mfma_plan(n, ci, h, wd, co, &q)) return 0;
+    return q.xcol_bytes + q.part_bytes;
+}
+
+int oct_cnn_conv_bwd_weight_mfma(const float* dpre, const uint16_t* x, float* dw, float* db, int n, int ci, int h,
+                                 int wd, int co, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!dpre || !x || !dw || !db || !workspace) return fail(OCT_EINVAL, "null argument");
+    BwdWeightMfmaPlan q;
+    if (!bwd_weight_mfma_plan(n, ci, h, wd, co, &q))
+        return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d (output channels a multiple of 16)", n, ci, h,
+                    wd, co);
+    if (workspace_bytes < q.xcol_bytes + q.part_bytes)
+        return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes, q.xcol_bytes + q.part_bytes);
+    if (reinterpret_cast<uintptr_t>(workspace) % 16 || reinterpret_cast<uintptr_t>(dpre) % 16)
+        return fail(OCT_EINVAL, "workspace and dpre must be 16-byte aligned");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint16_t* xcol = static_cast<uint16_t*>(workspace);
+    float* part = reinterpret_cast<float*>(static_cast<char*>(workspace) + q.xcol_bytes);
+    const long long items = (long long)q.ncols_pad * (q.ppad / 8);
+    hipLaunchKernelGGL(oct::conv_im2col_kernel, dim3((unsigned)((items + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, s, x, xcol, ci, h, wd, q.P, q.ppad, q.ncols_pad);
+    HIP_TRY(hipGetLastError());
+    const long long waves = (long long)(co / 16) * ((q.ncols_pad / 16 + 3) / 4) * q.S;
+    hipLaunchKernelGGL(oct::conv_bwd_weight_mfma_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(oct::kCnnThreads), 0,
+                       s, dpre, xcol, part, co, (h / 2) * (wd / 2), q.P, q.ppad, q.ncols_pad, q.S, q.slice);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(oct::conv_bwd_weight_mfma_reduce_kernel, dim3((unsigned)((9LL * ci + 1 + 15) / 16), (unsigned)co),
+                       dim3(oct::kCnnThreads), 0, s, part, dw, db, ci, co, q.ncols_pad, q.S);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
+static bool bwd_data_mfma_sizes(int n, int ci, int h, int wd, int co, size_t* dt_bytes, size_t* wt_bytes) {
+    if (!conv_shape_ok(n, ci, h, wd, co) || co % 8 || ci % 16 || ci > 64) return false;
+    const long long P = (long long)n * (h / 2) * (wd / 2);
+    if (P >= (1LL << 30) || (long long)n * ci * h * wd >= (1LL << 40)) return false;
+    *dt_bytes = align256(sizeof(uint16_t) * (size_t)P * (size_t)co);
+    *wt_bytes = align256(sizeof(uint16_t) * 9 * (size_t)ci * (size_t)co);
+    return true;
+}
+
+size_t oct_cnn_conv_bwd_data_mfma_workspace_bytes(int n, int ci, int h, int wd, int co) {
+    size_t dt = 0, wt = 0;
+    if (!bwd_data_mfma_sizes(n, ci, h, wd, co, &dt, &wt)) return 0;
+    return 2 * dt + wt;
+}
+
+int oct_cnn_conv_bwd_data_mfma(const float* dpre, const uint16_t* w, const uint16_t* xmask, float* dx, int n, int ci,
+                               int h, int wd, int co, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!dpre || !w || !dx || !workspace) return fail(OCT_EINVAL, "null argument");
+    size_t dt = 0, wt = 0;
+    if (!bwd_data_mfma_sizes(n, ci, h, wd, co, &dt, &wt))
+        return fail(OCT_EINVAL, "bad conv shape (%d, %d, %d, %d) -> %d (input channels a multiple of 16 up to 64, "
+                    "output channels a multiple of 8)", n, ci, h, wd, co);
+    if (workspace_bytes < 2 * dt + wt) return fail(OCT_EINVAL, "workspace %zu bytes, need %zu", workspace_bytes, 2 * dt + wt);
+    if (reinterpret_cast<uintptr_t>(workspace) % 16) return fail(OCT_EINVAL, "workspace must be 16-byte aligned");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    uint16_t* dth = static_cast<uint16_t*>(workspace);
+    uint16_t* dtl = reinterpret_cast<uint16_t*>(static_cast<char*>(workspace) + dt);
+    uint16_t* wtb = reinterpret_cast<uint16_t*>(static_cast<char*>(workspace) + 2 * dt);
+    const int P = n * (h / 2) * (wd / 2);
+    const long long items = (long long)(co / 8) * P;
+    hipLaunchKernelGGL(oct::conv_dpre_split_kernel, dim3((unsigned)((items + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, s, dpre, dth, dtl, co, (h / 2) * (wd / 2), P);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(oct::conv_w_transpose_kernel, dim3((unsigned)((9 * ci * co + oct::kCnnThreads - 1) / oct::kCnnThreads)),
+                       dim3(oct::kCnnThreads), 0, s, w, wtb, ci, co);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(oct::conv_bwd_data_mfma_kernel, dim3((unsigned)((P + 63) / 64), 4), dim3(oct::kCnnThreads), 0, s,
+                       dth, dtl, wtb, xmask, dx, n, ci, h, wd, co);
+    HIP_TRY(hipGetLastError());
+    return OCT_OK;
+}
+
 int oct_cnn_fc_fwd(const uint16_t* x, const uint16_t* w, const float* bias, uint16_t* y_relu_bf16, float* y_linear,
                    int n, int k, int o, void* stream) {
     if (!x || !w || !bias) return fail(OCT_EINVAL, "null argument");

@@ -116,6 +116,9 @@ class EstimatorNet:
         self.v = torch.zeros_like(self.p32)
         self.steps = 0
         self._bufs = {}
+        # training backward convolutions on the matrix cores (oct_cnn_conv_bwd_*_mfma) where the
+        # shape allows; False: the FP32-VALU kernels
+        self.matrix_cores_backward = True
 
     # -- slices
     def _slice(self, flat, name):
@@ -134,7 +137,7 @@ class EstimatorNet:
             return self._bufs[n]
         torch = self.torch
         dev = self.xp.device
-        acts, dpre, ws = [], [], []
+        acts, dpre, ws, mw, md = [], [], [], [], []
         h = IMG
         L = _lib.lib()
         for ci, co in ARCH_CONV:
@@ -142,8 +145,13 @@ class EstimatorNet:
             dpre.append(torch.empty((n, co, h // 2, h // 2), dtype=torch.float32, device=dev))
             nb = L.oct_cnn_conv_bwd_weight_workspace_bytes(n, ci, h, h, co)
             ws.append(torch.empty((max(1, nb // 4),), dtype=torch.float32, device=dev))
+            mw.append(int(L.oct_cnn_conv_bwd_weight_mfma_workspace_bytes(n, ci, h, h, co)))
+            md.append(int(L.oct_cnn_conv_bwd_data_mfma_workspace_bytes(n, ci, h, h, co)))
             h //= 2
-        b = dict(acts=acts, dpre=dpre, ws=ws,
+        # one workspace for the matrix-core backward of every layer (the layers run in stream order)
+        mbytes = max(mw + md)
+        b = dict(acts=acts, dpre=dpre, ws=ws, mw=mw, md=md,
+                 mws=torch.empty((max(1, (mbytes + 3) // 4),), dtype=torch.float32, device=dev),
                  hid=torch.empty((n, FC_HIDDEN), dtype=torch.bfloat16, device=dev),
                  dhid=torch.empty((n, FC_HIDDEN), dtype=torch.float32, device=dev),
                  head=torch.empty((n, HEAD), dtype=torch.float32, device=dev),
@@ -217,13 +225,27 @@ class EstimatorNet:
             _need(b["dpre"][i], n * co * (hin // 2) ** 2, f"conv{i} output gradient")
             if i > 0:
                 _need(b["dpre"][i - 1], n * ci * hin * hin, f"conv{i} input gradient")
-            _lib.check(L.oct_cnn_conv_bwd_weight(b["dpre"][i].data_ptr(), xin.data_ptr(), self._ptr(g, f"conv{i}.w"),
-                                                 self._ptr(g, f"conv{i}.b"), n, ci, hin, hin, co, ws.data_ptr(),
-                                                 ws.numel() * 4, s), f"conv{i} backward (weights)")
+            mws, mcores = b["mws"], self.matrix_cores_backward
+            if mcores and b["mw"][i]:
+                _lib.check(L.oct_cnn_conv_bwd_weight_mfma(b["dpre"][i].data_ptr(), xin.data_ptr(),
+                                                          self._ptr(g, f"conv{i}.w"), self._ptr(g, f"conv{i}.b"), n, ci,
+                                                          hin, hin, co, mws.data_ptr(), mws.numel() * 4, s),
+                           f"conv{i} backward (weights, matrix cores)")
+            else:
+                _lib.check(L.oct_cnn_conv_bwd_weight(b["dpre"][i].data_ptr(), xin.data_ptr(),
+                                                     self._ptr(g, f"conv{i}.w"), self._ptr(g, f"conv{i}.b"), n, ci,
+                                                     hin, hin, co, ws.data_ptr(), ws.numel() * 4, s),
+                           f"conv{i} backward (weights)")
             if i > 0:
-                _lib.check(L.oct_cnn_conv_bwd_data(b["dpre"][i].data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
-                                                   xin.data_ptr(), b["dpre"][i - 1].data_ptr(), n, ci, hin, hin, co,
-                                                   s), f"conv{i} backward (data)")
+                if mcores and b["md"][i]:
+                    _lib.check(L.oct_cnn_conv_bwd_data_mfma(b["dpre"][i].data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
+                                                            xin.data_ptr(), b["dpre"][i - 1].data_ptr(), n, ci, hin,
+                                                            hin, co, mws.data_ptr(), mws.numel() * 4, s),
+                               f"conv{i} backward (data, matrix cores)")
+                else:
+                    _lib.check(L.oct_cnn_conv_bwd_data(b["dpre"][i].data_ptr(), self._ptr(self.pbf, f"conv{i}.w"),
+                                                       xin.data_ptr(), b["dpre"][i - 1].data_ptr(), n, ci, hin, hin,
+                                                       co, s), f"conv{i} backward (data)")
 
     def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
         self.steps += 1
