@@ -69,6 +69,35 @@ class _ExtentLib:
         assert wsb >= 40 * co * ci
         return 0
 
+    # the matrix-core backward: the same extents, plus a workspace of the size its query gives
+    # (the data gradient's query is 0 where the input channels are not a multiple of 16)
+    def oct_cnn_conv_bwd_weight_mfma_workspace_bytes(self, n, ci, h, wd, co):
+        return 2 * (9 * ci + 16) * (n * (h // 2) * (wd // 2) + 32) + 64 * co * (9 * ci + 16)
+
+    def oct_cnn_conv_bwd_weight_mfma(self, dpre, x, dw, db, n, ci, h, wd, co, ws, wsb, s):
+        self.calls.append(("conv_bwd_weight", n, ci, h, co))
+        self._chk(dpre, 4 * n * co * (h // 2) * (wd // 2), "dpre")
+        self._chk(x, 2 * n * ci * h * wd, "x")
+        self._chk(dw, 4 * co * ci * 9, "dw")
+        self._chk(db, 4 * co, "db")
+        self._chk(ws, wsb, "workspace")
+        assert wsb >= self.oct_cnn_conv_bwd_weight_mfma_workspace_bytes(n, ci, h, wd, co)
+        return 0
+
+    def oct_cnn_conv_bwd_data_mfma_workspace_bytes(self, n, ci, h, wd, co):
+        return 0 if ci % 16 else 4 * n * co * (h // 2) * (wd // 2) + 18 * ci * co
+
+    def oct_cnn_conv_bwd_data_mfma(self, dpre, w, xmask, dx, n, ci, h, wd, co, ws, wsb, s):
+        assert self.oct_cnn_conv_bwd_data_mfma_workspace_bytes(n, ci, h, wd, co) > 0
+        self.calls.append(("conv_bwd_data", n, ci, h, co))
+        self._chk(dpre, 4 * n * co * (h // 2) * (wd // 2), "dpre")
+        self._chk(w, 2 * co * ci * 9, "w")
+        self._chk(xmask, 2 * n * ci * h * wd, "xmask")
+        self._chk(dx, 4 * n * ci * h * wd, "dx")
+        self._chk(ws, wsb, "workspace")
+        assert wsb >= self.oct_cnn_conv_bwd_data_mfma_workspace_bytes(n, ci, h, wd, co)
+        return 0
+
     def oct_cnn_fc_fwd(self, x, w, b, ybf, yf, n, k, o, s):
         self.calls.append(("fc_fwd", n, k, o))
         assert (ybf is None) != (yf is None)
@@ -122,9 +151,11 @@ class _CpuXP:
         return None
 
 
-def test_network_launch_extents(monkeypatch):
+@pytest.mark.parametrize("matrix_cores", [True, False])
+def test_network_launch_extents(monkeypatch, matrix_cores):
     xp = _CpuXP()
     net = LN.EstimatorNet(xp, seed=0)
+    net.matrix_cores_backward = matrix_cores
     n = 3
     x = torch.zeros((n, 2, LN.IMG, LN.IMG), dtype=torch.bfloat16)
     tgt = torch.zeros((n, 2), dtype=torch.float32)
