@@ -188,15 +188,22 @@ int oct_average_repeats(const float* mag, float* out, int n_frames, int bscan_av
 /* ---- dispersion sharpness search (octrecon/estimation/dispersion.py:46-107, 154-239)
  * One search evaluation is the spectral path with the candidate's window
  * hann/rms * exp(-i beta (k - k0)^2).  oct_plan_use_window makes later oct_spectral launches
- * with this plan read `dev_window` ((N, 2) float32 device array, already multiplied by 1/N)
- * instead of the plan's own window; NULL restores it.  The pointer is captured at each launch,
- * so a batch of candidates can be queued without host synchronisation.
+ * with this plan read `dev_window` ((N, 2) float32 device array, 16-byte aligned, already
+ * multiplied by 1/N, in the pair order below) instead of the plan's own window; NULL restores it.
+ * The pointer is captured at each launch, so a batch of candidates can be queued without host
+ * synchronisation.
+ * Pair order (N = 2048): element i = l + 64 q (l < 64, q < 32) of a device window is stored at
+ * complex position 2 ((q mod 16) 64 + l) + q / 16, so that the transform's lane l reads its
+ * elements q and q + 16 with one 16-byte load (oct_dispersion_windows writes this order;
+ * spectral.pair_order converts a natural-order window).  oct_plan_create takes its host window
+ * in natural order.
  * oct_topfrac_metric: out[b] = log(mean of the k = max(1, int(M*frac)) largest |A|^2 over all
  * x and z >= z_min), M = n_x (n_z - z_min); mag (n_bscans, n_x, n_z) float32, out float64;
  * exact (radix selection), deterministic, workspace of oct_topfrac_metric_workspace_bytes. */
 int oct_plan_use_window(oct_plan* plan, const float* dev_window);
 /* Per-file candidates: file b of later oct_spectral launches uses the window at
- * dev_windows + b * file_stride floats (file_stride 0: one window for every file, >= 2N otherwise).
+ * dev_windows + b * file_stride floats (file_stride 0: one window for every file, else a multiple
+ * of 4 and >= 2N; dev_windows 16-byte aligned; each window in the pair order).
  * Used by the batched per-B-scan search (each B-scan its own bracket), which the reference runs
  * one B-scan at a time (octrecon/estimation/dispersion.py:226-239). */
 int oct_plan_use_windows(oct_plan* plan, const float* dev_windows, long long file_stride, long long repeat_stride);
@@ -242,7 +249,8 @@ int oct_order_stats(const float* mag, int n_bscans, int n_x, int n_z, int z_min,
  * (different streams, or threads) race and must be serialised by the caller. */
 int oct_topfrac_metric_noalloc(const float* mag, int n_bscans, int n_x, int n_z, int z_min, double frac, double* out,
                                void* stream);
-/* Candidate windows out (count, n, 2) float32: hann_over_n[j] exp(-i beta_b dk2[j]) in float64, with
+/* Candidate windows out (count, n, 2) float32 (n = 2048; out 16-byte aligned), each in the pair
+ * order of oct_plan_use_window: element j = hann_over_n[j] exp(-i beta_b dk2[j]) in float64, with
  * beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref (device
  * arrays; `head` lets a network's output set the candidates without a host round trip). */
 int oct_dispersion_windows(const double* betas, const float* head, int head_stride, double head_scale, double offset,

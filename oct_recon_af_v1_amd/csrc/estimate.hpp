@@ -283,7 +283,8 @@ __global__ __launch_bounds__(kTfThreads) void rowmax_kernel(const double* __rest
 
 // Dispersion search windows: out[b, j] = hann_n[j] exp(-i beta_b dk2[j]) (hann_n = Hann/rms/N,
 // dk2 = (k - mean k)^2 of the equispaced k grid, octrecon/estimation/dispersion.py:66-107), float64
-// phase, stored as float32 pairs, with
+// phase, stored as float32 pairs in the pair order of the spectral kernels (element j at float2
+// position pair_pos(j) of window b; n = 2048), with
 //   beta_b = ((head ? head[b * head_stride] * head_scale : betas[b]) + offset) * to_ref.
 // Grid (ceil(n / 256), count).
 __global__ __launch_bounds__(256) void windows_kernel(const double* __restrict__ betas, const float* __restrict__ head,
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) void windows_kernel(const double* __restrict__
     float2 w;
     w.x = (float)(hann_n[j] * c);
     w.y = (float)(-hann_n[j] * s);
-    reinterpret_cast<float2*>(out)[(long long)b * n + j] = w;
+    reinterpret_cast<float2*>(out)[(long long)b * n + pair_pos(j)] = w;           // the pair order
 }
 
 }  // namespace oct

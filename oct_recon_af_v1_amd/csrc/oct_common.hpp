@@ -80,6 +80,13 @@ __device__ __forceinline__ cf twiddle(cf a) {
     }
 }
 
+// ---------------------------------------------------------------- pair order of 2048-point tables
+// The transform's wave owns elements i = lane + 64 q (q < 32) of a 2048-point complex table (the
+// window, the twiddles).  In the pair order a lane's elements q and q + 16 are adjacent, so one
+// 16-byte load gives it both: element i sits at float2 position
+//   2 ((q mod 16) 64 + lane) + q / 16,  q = i / 64, lane = i mod 64.
+__host__ __device__ constexpr int pair_pos(int i) { return 2 * (((i >> 6) & 15) * 64 + (i & 63)) + (i >> 10); }
+
 // ---------------------------------------------------------------- compile-time loops
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {

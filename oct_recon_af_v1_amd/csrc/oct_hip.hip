@@ -47,11 +47,11 @@ struct oct_plan {
     int cus;
     int* band_start;
     float* band_w;
-    float* win;   // window / N
+    float* win;   // window / N, in the pair order (oct::pair_pos)
     const float* win_override;   // caller's device window / N (dispersion search), or null
     long long win_fstride;       // floats between per-file override windows (0: one window)
     long long win_rstride;       // floats between the override windows of consecutive batch repeats
-    float2* tw;   // (32, 64) transform twiddles [k1][n2]
+    float2* tw;   // (32, 64) transform twiddles W^(n2 k1), element k1 * 64 + n2 in the pair order
     int waves;    // workgroup shape of the fused kernel (see oct_plan_create)
 };
 
@@ -243,11 +243,16 @@ int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const i
     for (int k1 = 0; k1 < 32; ++k1)
         for (int n2 = 0; n2 < 64; ++n2) {
             const double ang = 2.0 * M_PI * (double)(n2 * k1) / (double)n_lambda;
-            twh[2 * (k1 * 64 + n2)] = (float)std::cos(ang);
-            twh[2 * (k1 * 64 + n2) + 1] = (float)std::sin(ang);
+            const int pp = oct::pair_pos(k1 * 64 + n2);
+            twh[2 * pp] = (float)std::cos(ang);
+            twh[2 * pp + 1] = (float)std::sin(ang);
         }
     float* win = new float[2 * n_lambda];
-    for (int i = 0; i < 2 * n_lambda; ++i) win[i] = window[i] * (1.0f / (float)n_lambda);   // exact: 2^-11
+    for (int i = 0; i < n_lambda; ++i) {                // / N is exact (2^-11); stored in the pair order
+        const int pp = oct::pair_pos(i);
+        win[2 * pp] = window[2 * i] * (1.0f / (float)n_lambda);
+        win[2 * pp + 1] = window[2 * i + 1] * (1.0f / (float)n_lambda);
+    }
     int rc = OCT_OK;
     if (hipMalloc(&p->band_start, sizeof(int) * n_lambda) != hipSuccess ||
         hipMalloc(&p->band_w, sizeof(float) * n_lambda * taps) != hipSuccess ||
@@ -698,6 +703,7 @@ int oct_quantize(const float* db, uint16_t* bits, int n_planes, long long plane_
 // ---------------------------------------------------------------- dispersion search support
 int oct_plan_use_window(oct_plan* p, const float* dev_window) {
     if (!p) return fail(OCT_EINVAL, "null plan");
+    if (reinterpret_cast<uintptr_t>(dev_window) % 16) return fail(OCT_EINVAL, "window must be 16-byte aligned");
     p->win_override = dev_window;
     p->win_fstride = 0;
     p->win_rstride = 0;
@@ -707,9 +713,9 @@ int oct_plan_use_window(oct_plan* p, const float* dev_window) {
 int oct_plan_use_windows(oct_plan* p, const float* dev_windows, long long file_stride, long long repeat_stride) {
     if (!p) return fail(OCT_EINVAL, "null plan");
     for (long long st : {file_stride, repeat_stride})
-        if (st < 0 || (st > 0 && st < 2LL * p->n) || st % 2)
-            return fail(OCT_EINVAL, "window stride %lld (0 or an even count >= 2N)", st);
-    if (reinterpret_cast<uintptr_t>(dev_windows) % 8) return fail(OCT_EINVAL, "windows must be 8-byte aligned");
+        if (st < 0 || (st > 0 && st < 2LL * p->n) || st % 4)
+            return fail(OCT_EINVAL, "window stride %lld (0 or a multiple of 4 floats >= 2N)", st);
+    if (reinterpret_cast<uintptr_t>(dev_windows) % 16) return fail(OCT_EINVAL, "windows must be 16-byte aligned");
     p->win_override = dev_windows;
     p->win_fstride = dev_windows ? file_stride : 0;
     p->win_rstride = dev_windows ? repeat_stride : 0;
@@ -904,8 +910,9 @@ int oct_dispersion_windows(const double* betas, const float* head, int head_stri
                            void* stream) {
     if (count <= 0) return OCT_OK;
     if ((!betas && !head) || !hann_over_n || !dk2 || !out) return fail(OCT_EINVAL, "null argument");
-    if (n <= 0 || count > 65535 || (head && head_stride <= 0)) return fail(OCT_EINVAL, "bad window shape (%d x %d)", count, n);
-    if (reinterpret_cast<uintptr_t>(out) % 8) return fail(OCT_EINVAL, "out must be 8-byte aligned");
+    if (n != oct::kN || count > 65535 || (head && head_stride <= 0))
+        return fail(OCT_EINVAL, "bad window shape (%d x %d; n = %d only)", count, n, oct::kN);
+    if (reinterpret_cast<uintptr_t>(out) % 16) return fail(OCT_EINVAL, "out must be 16-byte aligned");
     hipLaunchKernelGGL(oct::windows_kernel, dim3((n + 255) / 256, count), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                        head ? nullptr : betas, head, head_stride, head_scale, offset, to_ref, hann_over_n, dk2, n, out);
     HIP_TRY(hipGetLastError());

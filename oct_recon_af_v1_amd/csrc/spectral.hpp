@@ -238,7 +238,7 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
                 const float e = E[lane + 64 * q];
-                const float2 w = wg[lane + 64 * q];
+                const float2 w = wg[pair_pos(lane + 64 * q)];      // window in the pair order
                 v[q] = cf{e * w.x, e * w.y};
             }
             wave_lds_sync();
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
                 if constexpr (kLdsTw) {
                     v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
                 } else {
-                    const float2 t = a.tw[decltype(k1)::value * 64 + lane];
+                    const float2 t = a.tw[pair_pos(decltype(k1)::value * 64 + lane)];   // pair order
                     v[r] = cmul(v[r], cf{t.x, t.y});
                 }
             });
@@ -519,7 +519,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
 
         // ---- D: 2048-point inverse transform of this wave's A-scan
         if (wave < nvalid) {
-            const float2* wg = reinterpret_cast<const float2*>(a.win + (long long)rep * a.win_rstride +
+            // the window in the pair order: one 16-byte load gives a lane its samples q and q + 16
+            const float4* wg = reinterpret_cast<const float4*>(a.win + (long long)rep * a.win_rstride +
                                                                (long long)b * a.win_fstride);
             // the 32-point DFT over q by decimation in time: bit-reversed positions, so its first
             // radix-2 stage pairs samples q and q + 16, with the window multiply fused in
@@ -528,22 +529,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                 constexpr int i = decltype(iI)::value;
                 constexpr int q = bitrev(2 * i, 5);              // < 16; position 2 i + 1 holds q + 16
                 const float e0 = R[lane + 64 * q], e1 = R[lane + 64 * (q + 16)];
-                const float2 w0 = wg[lane + 64 * q], w1 = wg[lane + 64 * (q + 16)];
+                const float4 w01 = wg[q * 64 + lane];
+                const float2 w0{w01.x, w01.y}, w1{w01.z, w01.w};
                 const float px = e1 * w1.x, py = e1 * w1.y;
                 v[2 * i] = cf{__builtin_fmaf(e0, w0.x, px), __builtin_fmaf(e0, w0.y, py)};
                 v[2 * i + 1] = cf{__builtin_fmaf(e0, w0.x, -px), __builtin_fmaf(e0, w0.y, -py)};
             });
             wave_lds_sync();
             dft_inv_dit<kQ, 1>(v);                       // Y[k1] in v[k1]
-            static_for<1, kQ>([&](auto k1) {
-                constexpr int r = decltype(k1)::value;
-                if constexpr (kLdsTw) {
-                    v[r] = cmul(v[r], tws[decltype(k1)::value][lane]);
-                } else {
-                    const float2 t = a.tw[decltype(k1)::value * 64 + lane];
-                    v[r] = cmul(v[r], cf{t.x, t.y});
-                }
-            });
+            if constexpr (kLdsTw) {
+                static_for<1, kQ>([&](auto k1) { v[decltype(k1)::value] = cmul(v[decltype(k1)::value], tws[decltype(k1)::value][lane]); });
+            } else {
+                // twiddles in the pair order: one 16-byte load for k1 = j and j + 16
+                static_for<0, kQ / 2>([&](auto jJ) {
+                    constexpr int j = decltype(jJ)::value;
+                    const float4 t = reinterpret_cast<const float4*>(a.tw)[j * 64 + lane];
+                    if constexpr (j > 0) v[j] = cmul(v[j], cf{t.x, t.y});
+                    v[j + 16] = cmul(v[j + 16], cf{t.z, t.w});
+                });
+            }
             // transpose through the region: real parts, then imaginary parts
             const int kk1 = lane >> 1;
             const int h = lane & 1;

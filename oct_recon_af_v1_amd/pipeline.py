@@ -24,7 +24,7 @@ from . import _lib
 from .backend import get_xp, synchronize
 from .geometry import ScanGrid, build_spectral_geometry, build_tiled_geometry, dispersion_window
 from .quantize import DeviceFinalizer
-from .spectral import HipSpectralProcessor
+from .spectral import HipSpectralProcessor, pair_order
 from .stitching import HipTileStitcher, OpticalPathCorrection, TileStitcher, min_weight_threshold
 
 
@@ -146,7 +146,8 @@ class Reconstructor:
             return
         w = dispersion_window(self.sg.lambda_eq_nm, np.array([tile_betas[k] for k in keys], float))
         w64 = np.ascontiguousarray(np.stack([w.real, w.imag], -1))
-        w32 = np.ascontiguousarray(w64.astype(np.float32) / np.float32(self.sg.n))   # exact: 2^-11
+        # / N is exact (2^-11); the plain kernels read the pair order
+        w32 = np.ascontiguousarray(pair_order(w64.astype(np.float32) / np.float32(self.sg.n)))
         torch = self.torch
         self._tile_win = {
             "index": {k: i for i, k in enumerate(keys)},

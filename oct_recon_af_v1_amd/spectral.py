@@ -19,6 +19,32 @@ _RAW_CODE = {np.dtype(np.int16): _lib.OCT_RAW_I16, np.dtype(np.uint16): _lib.OCT
              np.dtype(np.float32): _lib.OCT_RAW_F32}
 
 
+def pair_positions(n: int = 2048) -> np.ndarray:
+    """Complex position of each element of a 2048-point table (window, twiddles) in the pair order
+    that the spectral kernels read.  The transform's lane owns elements i = lane + 64 q (q < 32);
+    its elements q and q + 16 are adjacent, so one 16-byte load gives both: element i sits at
+    2 ((q mod 16) 64 + lane) + q // 16 (oct_common.hpp pair_pos)."""
+    if n != 2048:
+        raise ValueError("the pair order is defined for 2048-point tables")
+    i = np.arange(n)
+    q, lane = i >> 6, i & 63
+    return 2 * ((q & 15) * 64 + lane) + (q >> 4)
+
+
+def pair_order(w: np.ndarray) -> np.ndarray:
+    """(..., 2048, 2) windows in natural order -> the pair order (oct_plan_use_window(s))."""
+    w = np.asarray(w)
+    out = np.empty_like(w)
+    out[..., pair_positions(w.shape[-2]), :] = w
+    return out
+
+
+def natural_order(w: np.ndarray) -> np.ndarray:
+    """Inverse of pair_order (windows read back from the device, e.g. oct_dispersion_windows)."""
+    w = np.asarray(w)
+    return w[..., pair_positions(w.shape[-2]), :]
+
+
 def _padded_band(sg: SpectralGeometry, dtype=np.float32):
     """Band operator padded to the kernel's tap window (12 or 16): weights are appended as
     zeros and the window start is pulled left where it would run past N-1, with the leading

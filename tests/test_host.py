@@ -202,3 +202,22 @@ def test_bench_refuses_more_than_one_gpu():
         assert r.returncode == 2, (extra_args, env_ws, r.stderr)
         assert "exactly one GPU" in r.stderr
         assert r.stdout == ""
+
+
+def test_pair_order_of_device_windows():
+    """The pair order of 2048-point device windows: a permutation, in which lane l's elements
+    l + 64 q and l + 64 (q + 16) (q < 16) fill the 16-byte slot q 64 + l, and which pair_order /
+    natural_order invert."""
+    from oct_recon_af_v1_amd.spectral import natural_order, pair_order, pair_positions
+    pos = pair_positions()
+    assert np.array_equal(np.sort(pos), np.arange(2048))
+    w = np.random.default_rng(0).standard_normal((3, 2048, 2)).astype(np.float32)
+    p = pair_order(w)
+    assert np.array_equal(natural_order(p), w)
+    slots = p.reshape(3, 1024, 4)
+    for q in (0, 7, 15):
+        for lane in (0, 31, 63):
+            assert np.array_equal(slots[:, q * 64 + lane, :2], w[:, lane + 64 * q])
+            assert np.array_equal(slots[:, q * 64 + lane, 2:], w[:, lane + 64 * (q + 16)])
+    with pytest.raises(ValueError):
+        pair_positions(1024)
