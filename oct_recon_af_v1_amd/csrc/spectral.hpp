@@ -39,11 +39,11 @@ struct SpectralArgs {
     const float* bg;         // (B, N) background row per raw file
     const int* band_start;   // (N) first source sample of each output's tap window
     const float* band_w;     // (N, TP) tap weights
-    const float* win;        // (N, 2) complex window scaled by 1/N
+    const float* win;        // (N, 2) complex window scaled by 1/N, in the pair order (pair_pos)
     long long win_fstride;   // floats between the windows of consecutive raw files (0: one window)
     long long win_rstride;   // floats between the windows of consecutive repeats of the batch
     int raw_files;           // files in raw / bg: A-scan file index f reads raw file f % raw_files
-    const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), [k1][n2] (W = 4 variant)
+    const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), element k1 * 64 + n2 in the pair order
     void* out;
     long long M;             // A-scans = B * nX
     int lines;               // A-lines per raw file
@@ -85,7 +85,7 @@ __device__ __forceinline__ void unpack16<float>(const uint4& u, float (&f)[4]) {
 
 // W = waves per workgroup = A-scans per group.  W = 8: one 148 KB workgroup per CU with the
 // twiddle table in LDS; W = 4: two 68 KB workgroups per CU (one overlaps the other's
-// barriers and loads), twiddles read from an L1-resident [k1][lane] table.
+// barriers and loads), twiddles read from an L1-resident table (pair order).
 template <typename RawT, int TP, int KIND, int W>
 __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a) {
     constexpr int kWaves = W;
@@ -339,8 +339,8 @@ constexpr int kRegionF = 32 * kStride;   // 2112 floats >= N
 //   2  more rows within two blocks: the direct sums of both blocks.
 constexpr int kStage2Extra = 2;
 // W = 16: one workgroup per CU, twiddles in LDS; W = 8 / 4: two / four workgroups per CU (their
-// loads and barriers overlap the others' arithmetic), twiddles read from the L1-resident
-// [k1][lane] table.
+// loads and barriers overlap the others' arithmetic), twiddles read from the L1-resident table in
+// the pair order, k1 = j and j + 16 with one 16-byte load.
 template <typename RawT, int TP, int KIND, int STAGE2, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void spectral16_kernel(SpectralArgs a) {
     constexpr bool PRUNED = STAGE2 > 0;
