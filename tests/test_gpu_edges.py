@@ -109,6 +109,16 @@ def test_rejected_arguments_leave_device_usable(xp):
     _, _, sp = _setup(xp, 0.0)
     with pytest.raises(ValueError):
         sp.run_operator(torch.zeros((1, 30, 1024), dtype=torch.int16, device=xp.device))   # 1024-pixel spectra
+    # device windows are read with 16-byte loads: misaligned windows and odd strides are refused
+    wins = torch.zeros((2, 2048, 2), dtype=torch.float32, device=xp.device)
+    with pytest.raises(ValueError, match="16-byte aligned"):
+        _lib.check(L.oct_plan_use_window(sp.plan.handle, wins.data_ptr() + 8), "window 8 bytes in")
+    with pytest.raises(ValueError, match="stride"):
+        _lib.check(L.oct_plan_use_windows(sp.plan.handle, wins.data_ptr(), 2 * 2048 + 2, 0), "odd window stride")
+    with pytest.raises(ValueError, match="bad window shape"):
+        _lib.check(L.oct_dispersion_windows(None, wins.data_ptr(), 4, 1.0, 0.0, 1.0, 1, wins.data_ptr(),
+                                            wins.data_ptr(), 1024, wins.data_ptr(), xp.stream()), "1024-point windows")
+    _lib.check(L.oct_plan_use_window(sp.plan.handle, None), "restore the plan's window")
     # the device still computes correctly after the rejected calls
     lam, bref, sp = _setup(xp, 1e-5)
     raw = SY.make_bscans(1, 64, beta=1e-5, seed=5)
