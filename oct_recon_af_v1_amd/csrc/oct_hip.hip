@@ -639,13 +639,19 @@ int oct_stitch_row_table(const oct_row_tile* tiles, const oct_row_tile* dev_tile
                            n_out_x, thr, db, plane0, partial, stream);
 }
 
+// plane clims (a wave per plane), then the global clim from them
+static void launch_clim(const int* partial, int nblk, int n_planes, float* plane_clim, float* clim, hipStream_t s) {
+    hipLaunchKernelGGL(oct::clim_planes_kernel, dim3((unsigned)((n_planes + 3) / 4)), dim3(256), 0, s, partial, nblk,
+                       n_planes, plane_clim);
+    hipLaunchKernelGGL(oct::clim_global_kernel, dim3(1), dim3(oct::kFinThreads), 0, s, plane_clim, n_planes, clim);
+}
+
 int oct_volume_clim(const int* partial, int n_planes, int blocks_per_plane, float* plane_clim, float* clim,
                     void* stream) {
     if (!partial || !plane_clim || !clim) return fail(OCT_EINVAL, "null argument");
     if (n_planes <= 0 || blocks_per_plane <= 0) return fail(OCT_EINVAL, "empty volume");
     if (reinterpret_cast<uintptr_t>(partial) & 7) return fail(OCT_EINVAL, "partials must be 8-byte aligned");
-    hipLaunchKernelGGL(oct::clim_kernel, dim3(1), dim3(oct::kFinThreads), 0, reinterpret_cast<hipStream_t>(stream),
-                       partial, blocks_per_plane, n_planes, plane_clim, clim);
+    launch_clim(partial, blocks_per_plane, n_planes, plane_clim, clim, reinterpret_cast<hipStream_t>(stream));
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }
@@ -674,8 +680,7 @@ int oct_finalize(const float* num, const float* den, float* db, int n_planes, lo
     hipLaunchKernelGGL(oct::finalize_kernel, dim3(nb, n_planes), dim3(oct::kFinThreads), 0, s, num, den, db,
                        plane_elems, thr, partial);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(oct::clim_kernel, dim3(1), dim3(oct::kFinThreads), 0, s, partial, nb, n_planes, plane_clim,
-                       clim);
+    launch_clim(partial, nb, n_planes, plane_clim, clim, s);
     HIP_TRY(hipGetLastError());
     return OCT_OK;
 }

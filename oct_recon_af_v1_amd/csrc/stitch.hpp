@@ -89,7 +89,7 @@ __device__ __forceinline__ int fkey(float f);
 //   num = sum_tiles sum_x Wx sum_r Wz fz valid mag,  den = the same without mag
 // in registers over every tile whose columns cover c, then finalises in place:
 // den < thr -> NaN, else 20 log10(num / den), and keeps per-block finite min/max partials
-// (same partial layout as finalize_kernel, so clim_kernel reduces a volume of rows).
+// (same partial layout as finalize_kernel, so clim_planes_kernel reduces a volume of rows).
 // Equivalent to the per-tile contributions + accumulation + finalise of
 // octrecon/pipeline.py:261-299, without num/den round trips through memory.
 constexpr int kRowMaxTiles = 40;   // keeps the by-value argument block under 4 KB
@@ -323,24 +323,41 @@ __global__ __launch_bounds__(kFinThreads) void finalize_kernel(const float* __re
 
 // Per-plane clim (NaN when the plane has no finite value) and the global clim
 // (nanmin / nanmax over planes), one block.
-__global__ __launch_bounds__(kFinThreads) void clim_kernel(const int* __restrict__ partial, int nblk, int nplanes,
-                                                           float* __restrict__ plane_clim,
-                                                           float* __restrict__ clim) {
-    int gmin = 0x7FFFFFFF, gmax = (int)0x80000000;
-    for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
-        int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
-        const int2* pp = reinterpret_cast<const int2*>(partial) + (long long)p * nblk;
-#pragma unroll 8
-        for (int i = 0; i < nblk; ++i) {                     // independent loads: overlapped
-            const int2 v = pp[i];
-            kmin = min(kmin, v.x);
-            kmax = max(kmax, v.y);
-        }
+// Plane and global clims from the per-(plane, block) finite min / max keys: one wave per plane
+// (its lanes take the blocks in turn, then a wave reduction), 4 planes per workgroup.  The global
+// clim follows from the plane clims (clim_global_kernel).
+__global__ __launch_bounds__(256) void clim_planes_kernel(const int* __restrict__ partial, int nblk, int nplanes,
+                                                          float* __restrict__ plane_clim) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= nplanes) return;                                   // whole wave
+    int kmin = 0x7FFFFFFF, kmax = (int)0x80000000;
+    const int2* pp = reinterpret_cast<const int2*>(partial) + (long long)p * nblk;
+    for (int i = lane; i < nblk; i += 64) {
+        const int2 v = pp[i];
+        kmin = min(kmin, v.x);
+        kmax = max(kmax, v.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        kmin = min(kmin, __shfl_xor(kmin, off));
+        kmax = max(kmax, __shfl_xor(kmax, off));
+    }
+    if (lane == 0) {
         const bool any = kmin <= kmax;
         plane_clim[2 * p] = any ? fkey_inv(kmin) : __builtin_nanf("");
         plane_clim[2 * p + 1] = any ? fkey_inv(kmax) : __builtin_nanf("");
-        gmin = min(gmin, kmin);
-        gmax = max(gmax, kmax);
+    }
+}
+// clim = the finite min / max over the plane clims (a plane without finite values holds NaN)
+__global__ __launch_bounds__(kFinThreads) void clim_global_kernel(const float* __restrict__ plane_clim, int nplanes,
+                                                                  float* __restrict__ clim) {
+    int gmin = 0x7FFFFFFF, gmax = (int)0x80000000;
+    for (int p = threadIdx.x; p < nplanes; p += blockDim.x) {
+        const float lo = plane_clim[2 * p], hi = plane_clim[2 * p + 1];
+        if (!__builtin_isnan(lo)) {
+            gmin = min(gmin, fkey(lo));
+            gmax = max(gmax, fkey(hi));
+        }
     }
     __shared__ int smin[kFinThreads], smax[kFinThreads];
     smin[threadIdx.x] = gmin;
