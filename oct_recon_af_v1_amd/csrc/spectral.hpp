@@ -558,10 +558,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             __builtin_amdgcn_sched_barrier(0);
             static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[decltype(k1)::value].x; });
             wave_lds_sync();
-            // (the full second stage, STAGE2 == 0, reads its 32 values in bit-reversed order for
-            // its transform by decimation in time)
 #pragma unroll
-            for (int m = 0; m < kQ; ++m) vr[m] = R[kk1 * kStride + 2 * (STAGE2 == 0 ? bitrev(m, 5) : m) + h];
+            for (int m = 0; m < kQ; ++m) vr[m] = R[kk1 * kStride + 2 * m + h];
             if constexpr (STAGE2 == 1) {
 #pragma unroll
                 for (int e = 0; e < kEx; ++e) exr[e] = R[((row0 + e) & 31) * kStride + lane];
@@ -570,7 +568,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             static_for<0, kQ>([&](auto k1) { R[decltype(k1)::value * kStride + lane] = v[decltype(k1)::value].y; });
             wave_lds_sync();
 #pragma unroll
-            for (int m = 0; m < kQ; ++m) v[m] = cf{vr[m], R[kk1 * kStride + 2 * (STAGE2 == 0 ? bitrev(m, 5) : m) + h]};
+            for (int m = 0; m < kQ; ++m) v[m] = cf{vr[m], R[kk1 * kStride + 2 * m + h]};
             if constexpr (STAGE2 == 1) {
 #pragma unroll
                 for (int e = 0; e < kEx; ++e) exi[e] = R[((row0 + e) & 31) * kStride + lane];
@@ -643,17 +641,17 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
                     }
                 }
             } else {
-                dft_inv_dit<kQ, 0>(v);                   // E_h[k2] in v[k2] (FMA butterflies)
+                dft_inv_dif<kQ>(v);                      // E_h[k2] in v[bitrev(k2)]
                 if (h) {
                     static_for<1, kQ>([&](auto k2) {
-                        constexpr int r = decltype(k2)::value;
+                        constexpr int r = bitrev(decltype(k2)::value, 5);
                         v[r] = twiddle<decltype(k2)::value, 64>(v[r]);
                     });
                 }
                 static_for<0, 16>([&](auto jj) {
                     constexpr int j = decltype(jj)::value;
-                    constexpr int rlo = j;
-                    constexpr int rhi = 16 + j;
+                    constexpr int rlo = bitrev(j, 5);
+                    constexpr int rhi = bitrev(16 + j, 5);
                     const cf send = h ? v[rlo] : v[rhi];
                     const cf keep = h ? v[rhi] : v[rlo];
                     const cf o{keep.x + swap_pair(send.x), keep.y + swap_pair(send.y)};
