@@ -80,10 +80,11 @@ int launch_spectral_w(const oct_plan* p, const oct::SpectralArgs& a, int kind, h
 
 template <typename RawT, int TP, int W>
 int launch_spectral16(const oct_plan* p, const oct::SpectralArgs& a, int kind, hipStream_t s) {
-    // four rounds of resident workgroups: workgroups that finish early take the remaining groups'
-    // share, so a launch's tail is shorter (about 1 % per tile against one round)
+    // sixteen rounds of resident workgroups: workgroups that finish early take the remaining
+    // groups' share, so a launch's tail is shorter (1.5-2.5 % per tile against one persistent
+    // round; a workgroup per group is no better than four rounds)
     const long long ngroups = (a.M + W - 1) / W;
-    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus * (16 / W) * 4);
+    const int grid = (int)std::min<long long>(ngroups, (long long)p->cus * (16 / W) * 16);
     if (grid <= 0) return OCT_OK;
     const dim3 threads(64 * W);
     const int st2 = a.k2n == 0 ? 0 : (a.rows <= 32 + oct::kStage2Extra ? 1 : 2);
