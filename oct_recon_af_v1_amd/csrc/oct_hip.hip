@@ -48,6 +48,7 @@ struct oct_plan {
     int* band_start;
     float* band_w;
     float* win;   // window / N, in the pair order (oct::pair_pos)
+    oct::cf* c64;   // exp(+2 pi i j / 64), j = 0..64 (oct::stage2_table_kernel)
     const float* win_override;   // caller's device window / N (dispersion search), or null
     long long win_fstride;       // floats between per-file override windows (0: one window)
     long long win_rstride;       // floats between the override windows of consecutive batch repeats
@@ -267,6 +268,15 @@ int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const i
         hipMemcpy(p->win, win, sizeof(float) * 2 * n_lambda, hipMemcpyHostToDevice) != hipSuccess) {
         rc = fail(OCT_EHIP, "plan upload failed: %s", hipGetErrorString(hipGetLastError()));
     }
+    if (rc == OCT_OK) {
+        hipError_t e2 = hipMalloc(&p->c64, sizeof(oct::cf) * 65);
+        if (e2 == hipSuccess) {
+            hipLaunchKernelGGL(oct::stage2_table_kernel, dim3(1), dim3(128), 0, nullptr, p->c64);
+            e2 = hipGetLastError();
+        }
+        if (e2 == hipSuccess) e2 = hipStreamSynchronize(nullptr);
+        if (e2 != hipSuccess) rc = fail(OCT_EHIP, "plan table build failed: %s", hipGetErrorString(e2));
+    }
     delete[] win;
     hipSetDevice(prev);
     if (rc != OCT_OK) {
@@ -286,6 +296,7 @@ int oct_plan_destroy(oct_plan* p) {
     if (p->band_w) hipFree(p->band_w);
     if (p->win) hipFree(p->win);
     if (p->tw) hipFree(p->tw);
+    if (p->c64) hipFree(p->c64);
     hipSetDevice(prev);
     delete p;
     return OCT_OK;
@@ -486,6 +497,7 @@ static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int 
     a.win_rstride = p->win_override ? p->win_rstride : 0;
     a.raw_files = n_files;
     a.tw = p->tw;
+    a.c64 = p->c64;
     a.out = out;
     a.M = (long long)n_repeat * n_files * nX;
     if (a.M >= (1LL << 31)) return fail(OCT_EINVAL, "%lld A-scans in one launch (at most 2^31 - 1)", a.M);

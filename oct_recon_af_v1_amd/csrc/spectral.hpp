@@ -44,6 +44,7 @@ struct SpectralArgs {
     long long win_rstride;   // floats between the windows of consecutive repeats of the batch
     int raw_files;           // files in raw / bg: A-scan file index f reads raw file f % raw_files
     const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), element k1 * 64 + n2 in the pair order
+    const cf* c64;           // (65) exp(+2 pi i j / 64), j = 0..64: the pruned second stage's twiddles
     void* out;
     long long M;             // A-scans = B * nX
     int lines;               // A-lines per raw file
@@ -53,6 +54,18 @@ struct SpectralArgs {
     int k2lo, k2n;           // k2n in {1, 2}: the rows lie in k2 blocks [k2lo, k2lo + k2n); 0: any rows
     long long out_stride;    // elements between consecutive A-scans in out
 };
+
+// exp(+2 pi i j / 64) for j = 0..64, built once per plan: the pruned second stage reads its
+// twiddles W_32^r (entry 2 r) and W_64^k2 (entry k2) from it instead of evaluating them in double
+// precision in every workgroup
+__global__ void stage2_table_kernel(cf* c64) {
+    const int j = threadIdx.x;
+    if (j <= 64) {
+        double sn, cs;
+        sincospi((double)j / 32.0, &sn, &cs);
+        c64[j] = cf{(float)cs, (float)sn};
+    }
+}
 
 template <typename RawT>
 __device__ __forceinline__ void unpack16(const uint4& u, float (&f)[16 / sizeof(RawT)]);
@@ -114,15 +127,12 @@ __global__ __launch_bounds__(64 * W, 8 / W) void spectral_kernel(SpectralArgs a)
     if constexpr (KIND != KIND_EQ) {
         if (a.k2n > 0 && threadIdx.x < 2 * kQ + 2) {
             const int t = threadIdx.x < 2 * kQ ? threadIdx.x / kQ : threadIdx.x - 2 * kQ;
-            const int k2 = a.k2lo + t;
-            double s, c;
+            const int k2 = a.k2lo + t;                  // <= 64
             if (threadIdx.x < 2 * kQ) {
                 const int m = threadIdx.x % kQ;
-                sincospi(2.0 * (double)((m * k2) % kQ) / kQ, &s, &c);
-                tw2[t][m] = cf{(float)c, (float)s};
+                tw2[t][m] = a.c64[2 * ((m * k2) % kQ)];     // W_32^(m k2)
             } else {
-                sincospi(2.0 * (double)k2 / 64.0, &s, &c);
-                tw64[t] = cf{(float)c, (float)s};
+                tw64[t] = a.c64[k2];                        // W_64^k2
             }
         }
         // made visible to all waves by the first barrier of the group loop
@@ -368,23 +378,16 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
         }
         if (PRUNED && threadIdx.x < 2 * kQ + 2) {
             const int t = threadIdx.x < 2 * kQ ? threadIdx.x / kQ : threadIdx.x - 2 * kQ;
-            const int k2 = a.k2lo + t;
-            double sn, cs;
+            const int k2 = a.k2lo + t;                  // <= 64
             if (threadIdx.x < 2 * kQ) {
                 const int m = threadIdx.x % kQ;
-                sincospi(2.0 * (double)((m * k2) % kQ) / kQ, &sn, &cs);
-                tw2[t][m] = cf{(float)cs, (float)sn};
+                tw2[t][m] = a.c64[2 * ((m * k2) % kQ)];     // W_32^(m k2)
             } else {
-                sincospi(2.0 * (double)k2 / 64.0, &sn, &cs);
-                tw64[t] = cf{(float)cs, (float)sn};
+                tw64[t] = a.c64[k2];                        // W_64^k2
             }
         }
         if constexpr (STAGE2 == 1) {
-            if (threadIdx.x < 64) {
-                double sn, cs;
-                sincospi((double)threadIdx.x / 32.0, &sn, &cs);
-                w64t[threadIdx.x] = cf{(float)cs, (float)sn};
-            }
+            if (threadIdx.x < 64) w64t[threadIdx.x] = a.c64[threadIdx.x];
         }
         // made visible to all waves by the first barrier of the group loop
     }
