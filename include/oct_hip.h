@@ -50,7 +50,8 @@ enum { OCT_OUT_MAG = 0, OCT_OUT_COMPLEX = 1, OCT_OUT_LINEARISED = 2 };
 
 typedef struct oct_plan oct_plan;
 
-/* Upload the spectral operators (host arrays) to device `device`.
+/* Upload the spectral operators (host arrays) to device `device` and build the transform's twiddle
+ * tables there (synchronous: call outside any graph capture).
  *   n_lambda     spectral pixels N (2048)
  *   taps         tap window width T of band_w (12 or 16)
  *   band_start   (N) int32: output i uses source samples band_start[i] .. band_start[i]+T-1
