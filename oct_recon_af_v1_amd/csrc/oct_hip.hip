@@ -251,6 +251,20 @@ int oct_plan_create(oct_plan** plan, int device, int n_lambda, int taps, const i
             twh[2 * pp] = (float)std::cos(ang);
             twh[2 * pp + 1] = (float)std::sin(ang);
         }
+    // eight slots per lane: (W^(16 n2) in slot 0, else W^(s n2); W^((s + 8) n2)) as float4
+    const size_t tw8_off = twh.size();
+    twh.resize(tw8_off + 8 * 64 * 4);
+    for (int sl = 0; sl < 8; ++sl)
+        for (int n2 = 0; n2 < 64; ++n2) {
+            const int ka = sl == 0 ? 16 : sl, kb = sl + 8;
+            const double aa = 2.0 * M_PI * (double)(n2 * ka) / (double)n_lambda;
+            const double ab = 2.0 * M_PI * (double)(n2 * kb) / (double)n_lambda;
+            float* e = &twh[tw8_off + (size_t)(sl * 64 + n2) * 4];
+            e[0] = (float)std::cos(aa);
+            e[1] = (float)std::sin(aa);
+            e[2] = (float)std::cos(ab);
+            e[3] = (float)std::sin(ab);
+        }
     float* win = new float[2 * n_lambda];
     for (int i = 0; i < n_lambda; ++i) {                // / N is exact (2^-11); stored in the pair order
         const int pp = oct::pair_pos(i);
@@ -497,6 +511,7 @@ static int spectral_impl(const oct_plan* p, const void* raw, int raw_dtype, int 
     a.win_rstride = p->win_override ? p->win_rstride : 0;
     a.raw_files = n_files;
     a.tw = p->tw;
+    a.tw8 = reinterpret_cast<const float4*>(reinterpret_cast<const float*>(p->tw) + 2 * 32 * 64);
     a.c64 = p->c64;
     a.out = out;
     a.M = (long long)n_repeat * n_files * nX;

@@ -45,6 +45,7 @@ struct SpectralArgs {
     int raw_files;           // files in raw / bg: A-scan file index f reads raw file f % raw_files
     const float2* tw;        // (32, 64) twiddles exp(+2 pi i n2 k1 / N), element k1 * 64 + n2 in the pair order
     const cf* c64;           // (65) exp(+2 pi i j / 64), j = 0..64: the pruned second stage's twiddles
+    const float4* tw8;       // (8, 64): slot s of lane n2 = (W^(16 n2) for s = 0 else W^(s n2), W^((s + 8) n2))
     void* out;
     long long M;             // A-scans = B * nX
     int lines;               // A-lines per raw file
@@ -543,12 +544,21 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) voi
             if constexpr (kLdsTw) {
                 static_for<1, kQ>([&](auto k1) { v[decltype(k1)::value] = cmul(v[decltype(k1)::value], tws[decltype(k1)::value][lane]); });
             } else {
-                // twiddles in the pair order: one 16-byte load for k1 = j and j + 16
-                static_for<0, kQ / 2>([&](auto jJ) {
-                    constexpr int j = decltype(jJ)::value;
-                    const float4 t = reinterpret_cast<const float4*>(a.tw)[j * 64 + lane];
-                    if constexpr (j > 0) v[j] = cmul(v[j], cf{t.x, t.y});
-                    v[j + 16] = cmul(v[j + 16], cf{t.z, t.w});
+                // W^(n2 (j + 16)) = W^(16 n2) W^(n2 j): eight 16-byte loads bring W^(16 n2) and
+                // W^(n2 j), j = 1..15; the upper half takes W^(16 n2) first
+                const float4 t0 = a.tw8[lane];
+                const cf u{t0.x, t0.y};
+                v[16] = cmul(v[16], u);
+                v[8] = cmul(v[8], cf{t0.z, t0.w});
+                v[24] = cmul(cmul(v[24], u), cf{t0.z, t0.w});
+                static_for<1, kQ / 4>([&](auto sS) {
+                    constexpr int sl = decltype(sS)::value;
+                    const float4 t = a.tw8[sl * 64 + lane];
+                    const cf ta{t.x, t.y}, tb{t.z, t.w};
+                    v[sl] = cmul(v[sl], ta);
+                    v[sl + 16] = cmul(cmul(v[sl + 16], u), ta);
+                    v[sl + 8] = cmul(v[sl + 8], tb);
+                    v[sl + 24] = cmul(cmul(v[sl + 24], u), tb);
                 });
             }
             // transpose through the region: real parts, then imaginary parts
