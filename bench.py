@@ -455,7 +455,7 @@ def main():
                      "traffic_source": "profiles/spectral_kernel_tiled_fp32.md (rocprofv3 FETCH_SIZE x2.0 + "
                                        "WRITE_SIZE x1.0, copy-calibrated)" if tpa else None,
                      "kernel": "spectral16_kernel<uint16_t,12,MAG,STAGE2,4>",
-                     "bound_note": "bound by the vector-memory path (TD 76 % busy) and latency, VALU issue next (non-packed FP32); the HBM fraction is reported as asked",
+                     "bound_note": "bound by the vector-memory path (TD 72 % busy) and latency, VALU issue next (non-packed FP32); the HBM fraction is reported as asked",
                      "bytes_per_ascan": bytes_per_ascan, "ascans_per_launch": per_launch,
                      "avg_launch_ms": avg_ms, "busy_ms_per_step": busy_ms, "busy_ms_per_launch": eff_ms,
                      "time_note": "achieved = a launch's bytes / (union of the step's launch intervals / launches)",
