@@ -349,9 +349,9 @@ constexpr int kRegionF = 32 * kStride;   // 2112 floats >= N
 //      lanes, X[k] = sum_n2 Z[n2][k mod 32] W_64^(n2 (k / 32)), reduced across the wave;
 //   2  more rows within two blocks: the direct sums of both blocks.
 constexpr int kStage2Extra = 2;
-// W = 16: one workgroup per CU, twiddles in LDS; W = 8 / 4: two / four workgroups per CU (their
-// loads and barriers overlap the others' arithmetic), twiddles read from the L1-resident table in
-// the pair order, k1 = j and j + 16 with one 16-byte load.
+// W = 16: one workgroup per CU, twiddles in LDS; W = 8 / 4 / 2: two / four / eight workgroups per
+// CU (their loads and barriers overlap the others' arithmetic), twiddles from eight 16-byte loads
+// of the L1-resident table a.tw8.
 template <typename RawT, int TP, int KIND, int STAGE2, int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) void spectral16_kernel(SpectralArgs a) {
     constexpr bool PRUNED = STAGE2 > 0;
